@@ -1,0 +1,166 @@
+/*
+ * voronota_ltr.h - C ABI of the MI355X tessellation library.
+ *
+ * Drop-in boundary for voronota-ltr's tessellation hot path: the crate's
+ * public Rust API (src/lib.rs:59-63) and the PyO3 module (src/python.rs).
+ * Plain C types only; results live in library-owned handles released with the
+ * matching *_free call. Every function returns an int status (VLTR_OK = 0)
+ * unless documented otherwise, never aborts across the ABI, and leaves a
+ * message for vltr_last_error() on failure.
+ */
+#ifndef VORONOTA_LTR_H_
+#define VORONOTA_LTR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status ---- */
+#define VLTR_OK 0
+#define VLTR_ERR_INVALID_ARGUMENT 1
+#define VLTR_ERR_DEVICE 2
+#define VLTR_ERR_OUT_OF_MEMORY 3
+#define VLTR_ERR_INTERNAL 4
+
+/* Sentinel for "no ball" in cell vertex / edge indices (types.rs:98 NULL_ID). */
+#define VLTR_NULL_ID UINT64_MAX
+
+/* ------------------------------------------------------------- types ---- */
+
+/* Ball: centre and radius (types.rs:12-21). 32 bytes, AoS. */
+typedef struct vltr_ball {
+  double x, y, z, r;
+} vltr_ball;
+
+/* Periodic box as three lattice vectors (types.rs:290-300 PeriodicBox). */
+typedef struct vltr_periodic_box {
+  double shift_a[3];
+  double shift_b[3];
+  double shift_c[3];
+} vltr_periodic_box;
+
+/* Contact between two balls (types.rs:66-83). In non-periodic results
+ * id_a < id_b; periodic results carry canonical ids as the reference does
+ * (tessellation.rs:488-497), which need not be ordered. */
+typedef struct vltr_contact {
+  uint64_t id_a;
+  uint64_t id_b;
+  double area;
+  double arc_length;
+  uint8_t central;
+  uint8_t reserved[7];
+} vltr_contact;
+
+/* Cell of one ball (types.rs:85-95). */
+typedef struct vltr_cell {
+  uint64_t index;
+  double sas_area;
+  double volume;
+} vltr_cell;
+
+/* Tessellation network vertex / edge (types.rs:100-144); VLTR_NULL_ID marks
+ * the solvent-accessible surface. */
+typedef struct vltr_cell_vertex {
+  uint64_t ball_indices[4];
+  double x, y, z;
+} vltr_cell_vertex;
+
+typedef struct vltr_cell_edge {
+  uint64_t ball_indices[3];
+  double length;
+} vltr_cell_edge;
+
+typedef struct vltr_context vltr_context;
+typedef struct vltr_result vltr_result;
+
+/* ------------------------------------------------------------ context --- */
+
+/* Binds a HIP device and a default-priority stream; owns device buffers that
+ * are reused across calls. Calls on distinct contexts may run concurrently;
+ * calls on one context must be serialized by the caller. No reference
+ * counterpart (the reference runs on the Rayon pool, main.rs:139-145). */
+int vltr_context_create(int device, vltr_context** out);
+void vltr_context_destroy(vltr_context* ctx);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* vltr_last_error(void);
+
+/* Per-phase device time of the last pipeline run on ctx, in milliseconds,
+ * measured with HIP events on the context stream. Order: prepare+grid,
+ * neighbors, pairs, cut, cells+compaction, total. Returns the count written. */
+int vltr_context_last_timings(const vltr_context* ctx, double* ms, int max_entries);
+
+/* ------------------------------------------------------- periodic box --- */
+
+/* PeriodicBox::from_corners (types.rs:302-310). */
+void vltr_periodic_box_from_corners(const double min_corner[3], const double max_corner[3], vltr_periodic_box* out);
+/* PeriodicBox::from_vectors (types.rs:312-320). */
+void vltr_periodic_box_from_vectors(const double a[3], const double b[3], const double c[3], vltr_periodic_box* out);
+
+/* ------------------------------------------------------- tessellation --- */
+
+/* compute_tessellation (tessellation.rs:37-48): contacts, per-ball SAS areas
+ * and volumes of the radical tessellation constrained inside the SAS.
+ * periodic_box and groups may be NULL; groups shorter than n only apply to
+ * the balls they cover (tessellation.rs:323-328). */
+int vltr_compute_tessellation(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe,
+                              const vltr_periodic_box* periodic_box, const int32_t* groups, size_t n_groups,
+                              int with_cell_vertices, vltr_result** out);
+
+/* compute_contacts_only (tessellation.rs:125-166): contacts without solid
+ * angles, volumes or cells. */
+int vltr_compute_contacts_only(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe,
+                               const vltr_periodic_box* periodic_box, const int32_t* groups, size_t n_groups,
+                               vltr_result** out);
+
+/* Device-resident variant of vltr_compute_tessellation / contacts_only
+ * (contacts_only != 0): d_balls is a device pointer to n vltr_ball; results
+ * stay on the device in buffers owned by ctx (valid until the next call on
+ * ctx) and are read with vltr_context_device_outputs. Non-periodic, no cell
+ * vertices. No reference counterpart: it is the host entry point minus the
+ * host<->device copies. */
+int vltr_compute_tessellation_device(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
+                                     const int32_t* d_groups, size_t n_groups, int contacts_only);
+/* Device pointers and counts of the last device-resident call. */
+int vltr_context_device_outputs(const vltr_context* ctx, const vltr_contact** d_contacts, size_t* n_contacts,
+                                const vltr_cell** d_cells, size_t* n_cells);
+
+/* Neighbor lists of the last pipeline run on ctx (diagnostics and parity
+ * tests): for each of the num_balls balls, its colliding spheres sorted by
+ * (distance from the ball's centre to the intersection-circle centre, index),
+ * the list SpheresSearcher::find_colliding_ids returns
+ * (spheres_searcher.rs:253-326). counts[i] is list i's length (0 for hidden
+ * balls), ids holds the lists back to back (at most max_ids entries written).
+ * Returns the total number of entries. */
+int64_t vltr_context_neighbor_lists(const vltr_context* ctx, size_t num_balls, int32_t* counts, int32_t* ids,
+                                    size_t max_ids);
+
+/* ------------------------------------------------------------ results --- */
+
+/* Results trait (types.rs:187-253) over TessellationResult (types.rs:256-287). */
+size_t vltr_result_num_balls(const vltr_result* r);
+size_t vltr_result_num_contacts(const vltr_result* r);
+const vltr_contact* vltr_result_contacts(const vltr_result* r);
+size_t vltr_result_num_cells(const vltr_result* r);
+const vltr_cell* vltr_result_cells(const vltr_result* r);
+int vltr_result_has_cell_vertices(const vltr_result* r);
+size_t vltr_result_num_cell_vertices(const vltr_result* r);
+const vltr_cell_vertex* vltr_result_cell_vertices(const vltr_result* r);
+size_t vltr_result_num_cell_edges(const vltr_result* r);
+const vltr_cell_edge* vltr_result_cell_edges(const vltr_result* r);
+double vltr_result_total_sas_area(const vltr_result* r);
+double vltr_result_total_volume(const vltr_result* r);
+double vltr_result_total_contact_area(const vltr_result* r);
+/* sas_areas()/volumes(): num_balls values, NaN where the reference has None. */
+void vltr_result_sas_areas(const vltr_result* r, double* out);
+void vltr_result_volumes(const vltr_result* r, double* out);
+void vltr_result_free(vltr_result* r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VORONOTA_LTR_H_ */

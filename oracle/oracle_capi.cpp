@@ -135,6 +135,42 @@ int oracle_collisions(const double* balls, size_t n, double probe, const double*
   return 0;
 }
 
+// Per-pair summaries of the collision step's pair list, in pair order
+// (contact.rs:347-404): out P x 6 (area, arc, solid_a, solid_b, pyr_a, pyr_b),
+// flags P (bit0 contact exists, bit1 central). mode 0 full, 1 area only.
+int oracle_pair_records(const double* balls, size_t n, double probe, const double* pbox9, const int32_t* groups,
+                        size_t n_groups, int mode, int threads, double* out, uint32_t* flags) {
+  auto b = load_balls<double>(balls, n);
+  PeriodicBox<double> pb;
+  if (pbox9) pb = load_pbox<double>(pbox9);
+  if (n == 0) return 0;
+  auto ctx = make_collision_context<double>(b, probe, pbox9 ? &pb : nullptr, groups, n_groups, threads);
+  parallel_for(ctx.pairs.size(), threads, [&](size_t k) {
+    const size_t a = ctx.pairs[k].first, bb = ctx.pairs[k].second;
+    for (int j = 0; j < 6; ++j) out[6 * k + j] = 0.0;
+    flags[k] = 0;
+    if (mode == 0) {
+      auto cd = construct_contact_descriptor(ctx.searcher.spheres(), a, bb, ctx.all_collisions[a]);
+      if (!cd) return;
+      const auto s = cd->to_summary();
+      out[6 * k + 0] = s.area;
+      out[6 * k + 1] = s.arc_length;
+      out[6 * k + 2] = s.solid_angle_a;
+      out[6 * k + 3] = s.solid_angle_b;
+      out[6 * k + 4] = s.pyramid_volume_a;
+      out[6 * k + 5] = s.pyramid_volume_b;
+      flags[k] = 1u | (s.central ? 2u : 0u);
+    } else {
+      auto ao = construct_contact_area(ctx.searcher.spheres(), a, bb, ctx.all_collisions[a]);
+      if (!ao) return;
+      out[6 * k + 0] = ao->area;
+      out[6 * k + 1] = ao->arc_length;
+      flags[k] = 1u | (ao->central ? 2u : 0u);
+    }
+  });
+  return 0;
+}
+
 size_t oracle_result_num_balls(const oracle_result* r) { return r->num_balls; }
 size_t oracle_result_num_contacts(const oracle_result* r) { return r->contacts.size(); }
 size_t oracle_result_num_cells(const oracle_result* r) { return r->cells.size(); }

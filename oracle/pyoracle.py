@@ -46,6 +46,8 @@ def _load():
     L.oracle_result_neighbors.argtypes = [_vp, _u64p, _u64p, _f64p, _u8p]
     L.oracle_result_pairs.argtypes = [_vp, _u64p]
     L.oracle_result_free.argtypes = [_vp]
+    L.oracle_pair_records.argtypes = [_f64p, sz, C.c_double, _f64p, _i32p, sz, C.c_int, C.c_int, _f64p,
+                                      C.POINTER(C.c_uint32)]
     L.oracle_count_flops.argtypes = [_f64p, sz, C.c_double, _f64p, _i32p, sz, C.c_int, _u64p, _u64p]
     L.oracle_energy.argtypes = [_f64p, sz, C.c_double, _i32p, _f64p, C.c_int, _f64p]
     L.oracle_energy_dual.argtypes = [_f64p, sz, C.c_double, _i32p, _f64p, sz, C.c_int, _f64p, _f64p]
@@ -196,6 +198,19 @@ def collisions(balls, probe, pbox=None, groups=None, threads=0):
         return offsets, index, value, excluded, pairs
     finally:
         L.oracle_result_free(h)
+
+
+def pair_records(balls, probe, n_pairs, pbox=None, groups=None, mode=0, threads=0):
+    """(P x 6 values, P flags) for the collision step's pair list, in pair order."""
+    L = _load()
+    b = _balls(balls)
+    pb = _pbox(pbox)
+    g, ng = _groups(groups)
+    out = np.zeros((n_pairs, 6))
+    flags = np.zeros(n_pairs, np.uint32)
+    L.oracle_pair_records(_ptr(b, _f64p), len(b), float(probe), _ptr(pb, _f64p), _ptr(g, _i32p), ng, int(mode),
+                          int(threads), _ptr(out, _f64p), flags.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out, flags
 
 
 def count_flops(balls, probe, pbox=None, groups=None, mode=0):

@@ -1,0 +1,485 @@
+// Contact clipping for one candidate pair (replaces src/contact.rs:196-848).
+//
+// Host+device code: the kernels in k_contact.hip instantiate it with
+// fixed-capacity register/scratch arrays or a global workspace, and the CPU
+// tests compile the same header for the host to check it pair by pair.
+#pragma once
+
+#include "vltr_geom.h"
+
+namespace vltr {
+
+// Contour capacity of the fast cut kernel (points while cutting / after the
+// restriction to the circle). Contacts that would exceed it are flagged and
+// re-run by the slow kernel, whose contour lives in a global workspace.
+constexpr int kMaxC = 16;
+constexpr int kMaxR = 2 * kMaxC;
+
+// Per-pair record written by the cut kernels (types.rs:380-415 summary).
+// flags: bit0 = contact exists (area > 0), bit1 = central, bit2 = overflow
+// (fast kernel ran out of contour capacity; slow kernel must redo it).
+struct PairRec {
+  double area;
+  double arc_length;
+  double solid_a, solid_b;
+  double pyr_a, pyr_b;
+  uint32_t flags;
+  uint32_t pad;
+};
+constexpr uint32_t kRecValid = 1u;
+constexpr uint32_t kRecCentral = 2u;
+constexpr uint32_t kRecOverflow = 4u;
+
+// Sphere i of an array of (x, y, z, R) quadruples.
+VLTR_HD Sph load_sph(const double* sph4, int64_t i) {
+  const double* q = sph4 + 4 * i;
+  return Sph{V3{q[0], q[1], q[2]}, q[3]};
+}
+
+// Contour storage while cutting: points plus (left, right) ids.
+template <int CAP>
+struct LocalCut {
+  static constexpr int cap = CAP;
+  double x[CAP], y[CAP], z[CAP];
+  int32_t l[CAP], r[CAP];
+  VLTR_HD void init(int) {}
+  VLTR_HD V3 p(int k) const { return V3{x[k], y[k], z[k]}; }
+  VLTR_HD void set(int k, V3 v, int32_t lid, int32_t rid) {
+    x[k] = v.x;
+    y[k] = v.y;
+    z[k] = v.z;
+    l[k] = lid;
+    r[k] = rid;
+  }
+  VLTR_HD void copy(int dst, int src) {
+    x[dst] = x[src];
+    y[dst] = y[src];
+    z[dst] = z[src];
+    l[dst] = l[src];
+    r[dst] = r[src];
+  }
+};
+
+// Restricted contour: points, ids and arc angles.
+template <int CAP>
+struct LocalArc {
+  static constexpr int cap = CAP;
+  double x[CAP], y[CAP], z[CAP], ang[CAP];
+  int32_t l[CAP], r[CAP];
+  VLTR_HD void init(int) {}
+  VLTR_HD V3 p(int k) const { return V3{x[k], y[k], z[k]}; }
+  VLTR_HD void set(int k, V3 v, int32_t lid, int32_t rid) {
+    x[k] = v.x;
+    y[k] = v.y;
+    z[k] = v.z;
+    l[k] = lid;
+    r[k] = rid;
+    ang[k] = 0.0;
+  }
+};
+
+// Global-workspace variants for the slow kernel (one region per lane).
+struct GlobalCut {
+  int cap;
+  double *x, *y, *z;
+  int32_t *l, *r;
+  VLTR_HD V3 p(int k) const { return V3{x[k], y[k], z[k]}; }
+  VLTR_HD void set(int k, V3 v, int32_t lid, int32_t rid) {
+    x[k] = v.x;
+    y[k] = v.y;
+    z[k] = v.z;
+    l[k] = lid;
+    r[k] = rid;
+  }
+  VLTR_HD void copy(int dst, int src) {
+    x[dst] = x[src];
+    y[dst] = y[src];
+    z[dst] = z[src];
+    l[dst] = l[src];
+    r[dst] = r[src];
+  }
+};
+struct GlobalArc {
+  int cap;
+  double *x, *y, *z, *ang;
+  int32_t *l, *r;
+  VLTR_HD V3 p(int k) const { return V3{x[k], y[k], z[k]}; }
+  VLTR_HD void set(int k, V3 v, int32_t lid, int32_t rid) {
+    x[k] = v.x;
+    y[k] = v.y;
+    z[k] = v.z;
+    l[k] = lid;
+    r[k] = rid;
+    ang[k] = 0.0;
+  }
+};
+
+template <class S>
+VLTR_HD int cap_of(const S& s) {
+  return s.cap;
+}
+
+enum ClipStatus : int { kNone = 0, kContact = 1, kOverflow = 2 };
+
+struct ClipOut {
+  Sph ic;
+  V3 axis;
+  V3 bary;
+  double sum_arc;
+  double area;
+  int m;  // restricted contour length (0: uncut full circle)
+};
+
+// contact.rs:472-481
+template <class S>
+VLTR_HD bool cuttable(const S& ct, int n, V3 a_c, V3 cut_pt) {
+  const double thr = norm2(cut_pt - a_c);
+  for (int k = 0; k < n; ++k)
+    if (norm2(ct.p(k) - a_c) >= thr) return true;
+  return false;
+}
+
+// contact.rs:484-627 (mark_and_cut_contour, mark_contour, cut_contour).
+// `nn` is unit(pn), used by the plane-segment intersections.
+template <class S>
+VLTR_HD int mark_and_cut(S& ct, int& n, V3 pc, V3 pn, V3 nn, int32_t cid) {
+  int outs = 0;
+  for (int k = 0; k < n; ++k)
+    if (halfspace(pc, pn, ct.p(k)) >= 0) {
+      ct.l[k] = cid;
+      ct.r[k] = cid;
+      ++outs;
+    }
+  if (outs == 0) return kContact;
+  if (outs >= n) {
+    n = 0;
+    return kContact;
+  }
+  if (n < 3) return kContact;
+  int first = -1, last = -1;
+  for (int k = 0; k < n; ++k)
+    if (ct.l[k] == cid && ct.r[k] == cid) {
+      if (first < 0) first = k;
+      last = k;
+    }
+  if (first < 0) return kContact;
+  int start = first, end = last;
+  if (start == 0 && end == n - 1) {
+    end = 0;
+    while (end + 1 < n && ct.l[end + 1] == cid && ct.r[end + 1] == cid) ++end;
+    start = n - 1;
+    while (start > 0 && ct.l[start - 1] == cid && ct.r[start - 1] == cid) --start;
+  }
+  if (start == end) {
+    if (n + 1 > cap_of(ct)) return kOverflow;
+    for (int k = n; k > start; --k) ct.copy(k, k - 1);
+    ++n;
+    end = start + 1;
+  } else if (start < end) {
+    if (start + 1 < end) {
+      const int d = end - (start + 1);
+      for (int k = start + 1; k + d < n; ++k) ct.copy(k, k + d);
+      n -= d;
+    }
+    end = start + 1;
+  } else {
+    if (start + 1 < n) n = start + 1;
+    if (end > 0) {
+      for (int k = 0; k + end < n; ++k) ct.copy(k, k + end);
+      n -= end;
+    }
+    start = n - 1;
+    end = 0;
+  }
+  const int il = start > 0 ? start - 1 : n - 1;
+  const V3 lx = plane_segment(pc, nn, ct.p(start), ct.p(il));
+  ct.set(start, lx, ct.r[il], ct.l[start]);
+  const int ir = end + 1 < n ? end + 1 : 0;
+  const V3 rx = plane_segment(pc, nn, ct.p(end), ct.p(ir));
+  ct.set(end, rx, ct.r[end], ct.l[ir]);
+  if (!f_gt(norm2(ct.p(end) - ct.p(start)), 0.0)) {
+    ct.r[start] = ct.r[end];
+    for (int k = end; k + 1 < n; ++k) ct.copy(k, k + 1);
+    --n;
+  }
+  return kContact;
+}
+
+template <class A>
+VLTR_HD bool arc_push(A& out, int& m, V3 v, int32_t lid, int32_t rid) {
+  if (m >= cap_of(out)) return false;
+  out.set(m, v, lid, rid);
+  ++m;
+  return true;
+}
+
+// contact.rs:631-769 as one streaming pass. Point i of the cut contour is
+// emitted after segment i has been processed (its ids are final then), its
+// insertions follow it; point 0's ids may still change in the last segment,
+// so its output slot is patched at the end.
+template <class S, class A>
+VLTR_HD int restrict_to_circle(S& ct, int n, A& out, int& m, const Sph& ic, V3 axis, int32_t a_id,
+                                                  double* sum) {
+  *sum = 0.0;
+  m = 0;
+  const double rr = ic.r * ic.r;
+  int outs = 0;
+  for (int k = 0; k < n; ++k) outs += (norm2(ct.p(k) - ic.c) <= rr) ? 0 : 1;
+  if (outs == 0) {
+    for (int k = 0; k < n; ++k)
+      if (!arc_push(out, m, ct.p(k), ct.l[k], ct.r[k])) return kOverflow;
+    return kContact;
+  }
+  int insertions = 0;
+  const bool in0 = norm2(ct.p(0) - ic.c) <= rr;
+  if (in0 && !arc_push(out, m, ct.p(0), ct.l[0], ct.r[0])) return kOverflow;
+  for (int i = 0; i < n; ++i) {
+    const int ni = (i + 1) % n;
+    const V3 pi = ct.p(i), pn = ct.p(ni);
+    const bool o1 = !(norm2(pi - ic.c) <= rr);
+    const bool o2 = !(norm2(pn - ic.c) <= rr);
+    V3 ins1 = pi, ins2 = pi;
+    int32_t i1l = 0, i1r = 0, i2l = 0, i2r = 0;
+    int nins = 0;
+    if (o1 || o2) {
+      if (o1 && o2) {
+        V3 mp;
+        if (project_inside_segment(ic.c, pi, pn, &mp) && norm2(mp - ic.c) <= rr) {
+          V3 ip1, ip2;
+          const bool ok1 = segment_circle(ic, mp, pi, &ip1);
+          const bool ok2 = segment_circle(ic, mp, pn, &ip2);
+          if (ok1 && ok2) {
+            ins1 = ip1;
+            i1l = a_id;
+            i1r = ct.r[i];
+            ins2 = ip2;
+            i2l = ct.l[ni];
+            i2r = a_id;
+            nins = 2;
+          }
+        }
+      } else if (o1) {
+        V3 ip;
+        if (segment_circle(ic, pn, pi, &ip)) {
+          ins1 = ip;
+          i1l = a_id;
+          i1r = ct.r[i];
+          nins = 1;
+        } else {
+          ct.l[ni] = a_id;
+          ct.r[ni] = ct.r[i];
+        }
+      } else {
+        V3 ip;
+        if (segment_circle(ic, pi, pn, &ip)) {
+          ins1 = ip;
+          i1l = ct.l[ni];
+          i1r = a_id;
+          nins = 1;
+        } else {
+          ct.l[i] = ct.l[ni];
+          ct.r[i] = a_id;
+        }
+      }
+    }
+    if (i > 0 && !o1 && !arc_push(out, m, pi, ct.l[i], ct.r[i])) return kOverflow;
+    if (nins >= 1 && !arc_push(out, m, ins1, i1l, i1r)) return kOverflow;
+    if (nins == 2 && !arc_push(out, m, ins2, i2l, i2r)) return kOverflow;
+    insertions += nins;
+  }
+  if (in0) {
+    out.l[0] = ct.l[0];
+    out.r[0] = ct.r[0];
+  }
+  if (insertions == 0 || m < 2) {
+    m = 0;
+    return kContact;
+  }
+  const V3 ctrl = ic.c + axis;
+  double s = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const int nk = (k + 1) % m;
+    if (out.r[k] == a_id && out.l[nk] == a_id) {
+      const double ang = directed_angle(ic.c, out.p(k), out.p(nk), ctrl);
+      out.ang[k] = ang;
+      s = s + ang;
+    }
+  }
+  *sum = s;
+  if (f_ge(s, kTau) || (m > 2 && f_eq(s, kTau))) {
+    *sum = kTau;
+    m = 0;
+  }
+  return kContact;
+}
+
+// contact.rs:773-793
+template <class A>
+VLTR_HD double contour_area(const A& ct, int m, const Sph& ic, V3* bary) {
+  V3 sum{0.0, 0.0, 0.0};
+  for (int k = 0; k < m; ++k) sum = sum + ct.p(k);
+  *bary = sum / static_cast<double>(m);
+  double area = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const int nk = (k + 1) % m;
+    area = area + triangle_area(*bary, ct.p(k), ct.p(nk));
+    const double ang = ct.ang[k];
+    if (ang > 0.0) area = area + ic.r * ic.r * (ang - sin(ang)) * 0.5;
+  }
+  return area;
+}
+
+// contact.rs:196-341 (clip_contact). `nbl` is a's sorted neighbor list.
+template <class S, class A>
+VLTR_HD int clip(const double* sph4, int32_t a_id, int32_t b_id, const int32_t* nbl, int nbn, const double* hex_s,
+                 const double* hex_c, S& ct, A& out, ClipOut* co) {
+  const Sph a = load_sph(sph4, a_id);
+  const Sph b = load_sph(sph4, b_id);
+  if (!intersects(a, b) || contains(a, b) || contains(b, a)) return kNone;
+  const Sph ic = intersection_circle(a, b);
+  const V3 axis = unit(b.c - a.c);
+  if (ic.r <= 0.0) return kNone;
+  const V3 ic_dir = unit(ic.c - a.c);
+  int n = 0;
+  bool initialized = false;
+  for (int k = 0; k < nbn; ++k) {
+    const int32_t cid = nbl[k];
+    if (cid == b_id) continue;
+    const Sph c = load_sph(sph4, cid);
+    if (!intersects(ic, c) || !intersects(b, c)) continue;
+    if (contains(c, a) || contains(c, b)) return kNone;
+    const V3 pc = circle_center(a, c);
+    const V3 pn = unit(c.c - a.c);
+    const double cosv = dot(ic_dir, unit(pc - a.c));
+    if (fabs(cosv) >= 1.0) {
+      if (halfspace(pc, pn, ic.c) > 0) return kNone;
+      continue;
+    }
+    const double l = fabs(dot(pn, ic.c - pc));
+    const double xl = l / sqrt(fma(cosv, -cosv, 1.0));
+    if (xl >= ic.r) {
+      if (halfspace(pc, pn, ic.c) >= 0) return kNone;
+      continue;
+    }
+    if (!initialized) {
+      // contact.rs:455-469: hexagon of circumradius 1.19 r around the circle
+      const V3 first = (any_normal(axis) * ic.r) * kHexagonScale;
+      ct.set(0, ic.c + first, a_id, a_id);
+      for (int h = 0; h < 5; ++h) ct.set(h + 1, ic.c + rotate(axis, hex_s[h], hex_c[h], first), a_id, a_id);
+      n = 6;
+      initialized = true;
+    } else if (!cuttable(ct, n, a.c, pc)) {
+      continue;
+    }
+    if (mark_and_cut(ct, n, pc, pn, unit(pn), cid) == kOverflow) return kOverflow;
+    if (n == 0) return kNone;
+  }
+  co->ic = ic;
+  co->axis = axis;
+  co->m = 0;
+  co->sum_arc = 0.0;
+  co->area = 0.0;
+  co->bary = V3{0.0, 0.0, 0.0};
+  if (!initialized) {
+    co->bary = ic.c;
+    co->sum_arc = kTau;
+    co->area = ic.r * ic.r * kPi;
+  } else {
+    int m = 0;
+    double s = 0.0;
+    if (restrict_to_circle(ct, n, out, m, ic, axis, a_id, &s) == kOverflow) return kOverflow;
+    co->sum_arc = s;
+    co->m = m;
+    if (m > 0) co->area = contour_area(out, m, ic, &co->bary);
+  }
+  return co->area <= 0.0 ? kNone : kContact;
+}
+
+// contact.rs:411-444
+template <class A>
+VLTR_HD bool is_central(V3 center, const A& ct, int m, V3 bary) {
+  const double fe = 1e-10;
+  for (int k = 0; k < m; ++k)
+    if ((ct.ang[k] - fe) > kPi) return true;
+  for (int i = 0; i < m; ++i) {
+    const V3 pi = ct.p(i);
+    const V3 pj = ct.p((i + 1) % m);
+    const V3 edge = pj - pi;
+    const double len = norm(edge);
+    if (len < fe) continue;
+    const V3 u = edge / len;
+    const V3 foot = pi + u * dot(u, bary - pi);
+    const V3 nb = bary - foot;
+    if (dot(nb, center - pi) < 0.0) return false;
+  }
+  return true;
+}
+
+// contact.rs:797-848
+template <class A>
+VLTR_HD double solid_angle(const Sph& a, const Sph& b, const Sph& ic, const A& ct, int m) {
+  double turn = 0.0;
+  if (m == 0) {
+    turn = kTau * norm(ic.c - a.c) / a.r;
+  } else {
+    for (int i = 0; i < m; ++i) {
+      const int pi = i > 0 ? i - 1 : m - 1;
+      const int ni = (i + 1) % m;
+      const V3 p0 = ct.p(pi), p1 = ct.p(i), p2 = ct.p(ni);
+      const double a0 = ct.ang[pi], a1 = ct.ang[i];
+      if (a0 > 0.0) {
+        V3 d = cross(b.c - a.c, p1 - ic.c);
+        const double t = dot(d, p0 - p1);
+        if ((a0 < kPi && t < 0.0) || (a0 > kPi && t > 0.0)) d = -d;
+        turn = turn + (kPi - min_dihedral(a.c, p1, p1 + d, p2));
+      } else if (a1 > 0.0) {
+        V3 d = cross(b.c - a.c, p1 - ic.c);
+        const double t = dot(d, p2 - p1);
+        if ((a1 < kPi && t < 0.0) || (a1 > kPi && t > 0.0)) d = -d;
+        turn = turn + (kPi - min_dihedral(a.c, p1, p0, p1 + d));
+        turn = turn + a1 * (norm(ic.c - a.c) / a.r);
+      } else {
+        turn = turn + (kPi - min_dihedral(a.c, p1, p0, p2));
+      }
+    }
+  }
+  double solid = kTau - turn;
+  const V3 ic_a = ic.c - a.c;
+  const V3 ic_b = ic.c - b.c;
+  if (dot(ic_a, ic_b) > 0.0 && norm2(a.c - ic.c) < norm2(b.c - ic.c)) solid = -solid;
+  return solid;
+}
+
+// contact.rs:347-404 (descriptor summary / area-only) for one pair.
+template <class S, class A>
+VLTR_HD PairRec make_record(const double* sph4, int32_t a_id, int32_t b_id, const int32_t* nbl, int nbn, int mode,
+                            const double* hex_s, const double* hex_c, S& ct, A& out, int* status) {
+  PairRec rec;
+  rec.area = rec.arc_length = rec.solid_a = rec.solid_b = rec.pyr_a = rec.pyr_b = 0.0;
+  rec.flags = 0u;
+  rec.pad = 0u;
+  ClipOut co;
+  const int st = clip(sph4, a_id, b_id, nbl, nbn, hex_s, hex_c, ct, out, &co);
+  *status = st;
+  if (st != kContact) {
+    if (st == kOverflow) rec.flags = kRecOverflow;
+    return rec;
+  }
+  rec.area = co.area;
+  rec.arc_length = co.sum_arc * co.ic.r;
+  if (is_central(co.ic.c, out, co.m, co.bary)) rec.flags |= kRecCentral;
+  rec.flags |= kRecValid;
+  if (mode == 0) {
+    const Sph a = load_sph(sph4, a_id);
+    const Sph b = load_sph(sph4, b_id);
+    rec.solid_a = solid_angle(a, b, co.ic, out, co.m);
+    rec.solid_b = solid_angle(b, a, co.ic, out, co.m);
+    const double sa = rec.solid_a < 0.0 ? -1.0 : 1.0;
+    const double sb = rec.solid_b < 0.0 ? -1.0 : 1.0;
+    rec.pyr_a = norm(co.ic.c - a.c) * co.area / 3.0 * sa;
+    rec.pyr_b = norm(co.ic.c - b.c) * co.area / 3.0 * sb;
+  }
+  return rec;
+}
+
+}  // namespace vltr

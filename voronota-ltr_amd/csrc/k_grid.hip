@@ -1,0 +1,400 @@
+// Cell list and neighbor lists (replaces src/spheres_searcher.rs).
+//
+// The reference bins spheres in a uniform grid of edge max(2R + 0.25)
+// (spheres_searcher.rs:62-101) and, per ball, scans the 27 surrounding cells,
+// keeps intersecting candidates (hidden-ball rules at :283-310) and sorts them
+// by (distance to the intersection-circle centre, index) (:253-326,
+// types.rs:148-184). Any two intersecting spheres are less than one cell edge
+// apart, so the candidate set does not depend on how cells are stored; here
+// cells are hashed: a sphere's bucket is the Morton code of its cell
+// coordinates taken modulo 2^k per axis (k >= 2, so the 27 cells around any
+// cell fall in 27 distinct buckets). Buckets are built with a counting sort
+// (atomic histogram, exclusive scan, scatter) so each bucket is a contiguous,
+// spatially coherent run of the sorted sphere array.
+#include "vltr_internal.h"
+
+namespace vltr {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kNbBlock = 256;                 // 4 wavefronts per block
+constexpr int kNbWaves = kNbBlock / kWave;
+constexpr int kListCap = 512;                 // per-wave LDS candidate list
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+  const int lane = threadIdx.x & (kWave - 1);
+  return (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
+}
+
+// Rust `f64 as i32` of an already floored value: saturating, NaN -> 0.
+__device__ __forceinline__ int32_t sat_i32(double v) {
+  if (v != v) return 0;
+  if (v >= 2147483647.0) return 2147483647;
+  if (v <= -2147483648.0) return (-2147483647 - 1);
+  return static_cast<int32_t>(v);
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+__device__ __forceinline__ uint32_t morton(int32_t x, int32_t y, int32_t z, int32_t kbits) {
+  const uint32_t m = (1u << kbits) - 1u;
+  return spread3(static_cast<uint32_t>(x) & m) | (spread3(static_cast<uint32_t>(y) & m) << 1) |
+         (spread3(static_cast<uint32_t>(z) & m) << 2);
+}
+
+__device__ __forceinline__ double box_size(const unsigned long long* bits) {
+  return __longlong_as_double(static_cast<long long>(*bits));
+}
+
+__device__ __forceinline__ int64_t xcd_block() { return xcd_remap(blockIdx.x, gridDim.x); }
+
+__device__ __forceinline__ int32_t seg_of_ball(const GridSeg* segs, int32_t n_segs, int64_t i) {
+  int32_t lo = 0, hi = n_segs - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (segs[mid].ball_off <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_init(Pipeline p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *p.box_bits = static_cast<unsigned long long>(__double_as_longlong(1.0));
+    *p.status = 0u;
+    *p.overflow_cnt = 0u;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* a, int64_t n, uint32_t v) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    a[i] = v;
+}
+
+// types.rs:58-63 (R = r + probe) and spheres_searcher.rs:74-77 (box size).
+__global__ void k_prepare(Pipeline p) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= p.n) return;
+  const double4 b = p.balls_in[i];
+  const double R = b.w + p.probe;
+  p.sph[i] = make_double4(b.x, b.y, b.z, R);
+  p.seg_of[i] = p.n_segs > 1 ? seg_of_ball(p.segs, p.n_segs, i) : 0;
+  const double bs = fma(R, 2.0, 0.25);
+  if (bs > 0.0) atomicMax(p.box_bits, static_cast<unsigned long long>(__double_as_longlong(bs)));
+}
+
+// types.rs:362-375: image g of ball i is sphere (g + 1) n + i.
+__global__ void k_populate(Pipeline p) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= p.n * 26) return;
+  const int64_t g = t / p.n, i = t - g * p.n;
+  int sx, sy, sz;
+  lattice_shift(static_cast<int>(g), &sx, &sy, &sz);
+  const double4 s = p.sph[i];
+  const V3 c = shift_point(p.pbox, V3{s.x, s.y, s.z}, double(sx), double(sy), double(sz));
+  p.sph[(g + 1) * p.n + i] = make_double4(c.x, c.y, c.z, s.w);
+  p.seg_of[(g + 1) * p.n + i] = p.seg_of[i];
+}
+
+__device__ __forceinline__ void cell_of(double4 s, double box, int32_t* cx, int32_t* cy, int32_t* cz) {
+  *cx = sat_i32(floor(s.x / box));
+  *cy = sat_i32(floor(s.y / box));
+  *cz = sat_i32(floor(s.z / box));
+}
+
+__global__ void k_keys(Pipeline p) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= p.n_pop) return;
+  const double box = box_size(p.box_bits);
+  int32_t cx, cy, cz;
+  cell_of(p.sph[i], box, &cx, &cy, &cz);
+  const GridSeg sg = p.segs[p.seg_of[i]];
+  const uint32_t b = static_cast<uint32_t>(sg.tab_off) + morton(cx, cy, cz, sg.kbits);
+  p.bucket_key[i] = b;
+  p.bucket_rank[i] = atomicAdd(&p.bucket_cnt[b], 1u);
+}
+
+__global__ void k_scatter(Pipeline p) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= p.n_pop) return;
+  const uint32_t pos = p.bucket_cnt[p.bucket_key[i]] + p.bucket_rank[i];
+  p.sorted_id[pos] = static_cast<int32_t>(i);
+  p.sorted_sph[pos] = p.sph[i];
+}
+
+__device__ __forceinline__ bool same_group(const Pipeline& p, int64_t a, int64_t b) {
+  if (p.groups != nullptr && a < p.n_groups && b < p.n_groups) return p.groups[a] == p.groups[b];
+  return false;
+}
+
+// tessellation.rs:298-319 (and updateable.rs:516-561 without involvement):
+// (a, b) is a candidate pair when not in the same group and a < b, or b is a
+// periodic image.
+__device__ __forceinline__ bool pair_qualifies(const Pipeline& p, int64_t a, int64_t c) {
+  const int64_t cc = p.periodic ? c % p.n : c;
+  if (same_group(p, a, cc)) return false;
+  return (p.periodic && c >= p.n) || a < c;
+}
+
+struct NbEntry {
+  double key;
+  int32_t id;
+};
+
+__device__ __forceinline__ bool nb_less(const NbEntry& x, const NbEntry& y) {
+  if (x.key < y.key) return true;
+  if (x.key > y.key) return false;
+  return x.id < y.id;
+}
+
+// One wavefront per query ball: spheres_searcher.rs:253-326.
+__global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
+  __shared__ NbEntry list[kNbWaves][kListCap];
+  __shared__ uint32_t r_start[kNbWaves][32];
+  __shared__ uint32_t r_pre[kNbWaves][32];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const int64_t q0 = xcd_block() * kNbWaves + w;
+  // Waves past the end still reach every block barrier below.
+  const bool active = q0 < p.n;
+  const int64_t q = active ? q0 : p.n - 1;
+  // Non-periodic: walk balls in cell order for locality; periodic: the
+  // canonical balls are the first n spheres, taken in index order.
+  const int64_t a = p.periodic ? q : static_cast<int64_t>(p.sorted_id[q]);
+  const double4 A4 = p.sph[a];
+  const Sph A{V3{A4.x, A4.y, A4.z}, A4.w};
+  const double box = box_size(p.box_bits);
+  int32_t cx, cy, cz;
+  cell_of(A4, box, &cx, &cy, &cz);
+  const GridSeg sg = p.segs[p.seg_of[a]];
+
+  uint32_t rs = 0, rc = 0;
+  if (active && lane < 27) {
+    const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+    const uint32_t b = static_cast<uint32_t>(sg.tab_off) + morton(cx + dx, cy + dy, cz + dz, sg.kbits);
+    rs = p.bucket_cnt[b];
+    rc = p.bucket_cnt[b + 1] - rs;
+  }
+  // inclusive prefix of rc over lanes 0..26
+  uint32_t inc = rc;
+  for (int off = 1; off < 32; off <<= 1) {
+    const uint32_t v = __shfl_up(inc, off, kWave);
+    if (lane >= off) inc += v;
+  }
+  const uint32_t total = __shfl(inc, 26, kWave);
+  if (lane < 27) {
+    r_start[w][lane] = rs;
+    r_pre[w][lane] = inc - rc;
+  }
+  __syncthreads();
+
+  int cnt = 0;
+  bool hidden = false;
+  for (uint32_t base = 0; base < total; base += kWave) {
+    const uint32_t j = base + lane;
+    bool emit = false;
+    NbEntry e{0.0, 0};
+    if (j < total) {
+      int t = 0;
+      for (int k = 1; k < 27; ++k)
+        if (r_pre[w][k] <= j) t = k;
+      const uint32_t pos = r_start[w][t] + (j - r_pre[w][t]);
+      const int32_t c = p.sorted_id[pos];
+      if (c != a) {
+        const double4 C4 = p.sorted_sph[pos];
+        const Sph Cs{V3{C4.x, C4.y, C4.z}, C4.w};
+        if (intersects(A, Cs)) {
+          if (contains(Cs, A) && (!equals(Cs, A) || a > c)) {
+            hidden = true;
+          } else if (!contains(A, Cs)) {
+            emit = true;
+            e.key = circle_center_distance(A, Cs);
+            e.id = c;
+          }
+        }
+      }
+    }
+    const unsigned long long m = __ballot(emit);
+    if (emit) {
+      const int slot = cnt + __popcll(m & lanemask_lt());
+      if (slot < kListCap) list[w][slot] = e;
+    }
+    cnt += __popcll(m);
+  }
+  const bool any_hidden = __ballot(hidden) != 0ull;
+  const int stored = cnt < kListCap ? cnt : kListCap;
+  __syncthreads();
+
+  if (!active) return;
+  const int cap = p.nb_cap;
+  int32_t* out = p.nb + a * static_cast<int64_t>(cap);
+  uint32_t pairs = 0;
+  if (!any_hidden) {
+    for (int e = lane; e < stored; e += kWave) {
+      const NbEntry me = list[w][e];
+      int rank = 0;
+      for (int k = 0; k < stored; ++k) rank += nb_less(list[w][k], me) ? 1 : 0;
+      if (rank < cap) {
+        out[rank] = me.id;
+        if (pair_qualifies(p, a, me.id)) pairs += 1;
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) pairs += __shfl_xor(pairs, off, kWave);
+  if (lane == 0) {
+    const int kept = any_hidden ? 0 : (stored < cap ? stored : cap);
+    p.nb_cnt[a] = kept;
+    p.excluded[a] = any_hidden ? 1 : 0;
+    p.pair_cnt[a] = any_hidden ? 0u : pairs;
+    if (!any_hidden && cnt > cap) atomicOr(p.status, kStatNbOverflow);
+    if (cnt > kListCap) atomicOr(p.status, kStatNbOverflow);
+  }
+}
+
+// Pair list in reference order: by first ball, then neighbor order.
+__global__ void k_pair_fill(Pipeline p) {
+  const int64_t a = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (a >= p.n) return;
+  const int64_t base = p.pair_cnt[a];
+  const int cnt = p.nb_cnt[a];
+  const int32_t* lst = p.nb + a * static_cast<int64_t>(p.nb_cap);
+  int64_t j = 0;
+  for (int k = 0; k < cnt; ++k) {
+    const int32_t c = lst[k];
+    if (!pair_qualifies(p, a, c)) continue;
+    if (base + j < p.pair_cap) {
+      p.pair_a[base + j] = static_cast<int32_t>(a);
+      p.pair_b[base + j] = c;
+    } else {
+      atomicOr(p.status, kStatPairOverflow);
+    }
+    ++j;
+  }
+}
+
+// ------------------------------------------------------------------ scan ----
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* sh) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  uint32_t inc = v;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t t = __shfl_up(inc, off, kWave);
+    if (lane >= off) inc += t;
+  }
+  if (lane == kWave - 1) sh[w] = inc;
+  __syncthreads();
+  uint32_t wpre = 0, tot = 0;
+  for (int k = 0; k < kScanBlock / kWave; ++k) {
+    if (k < w) wpre += sh[k];
+    tot += sh[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return wpre + inc - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* in, int64_t n, uint32_t* block_sums) {
+  __shared__ uint32_t sh[kScanBlock / kWave];
+  const int64_t base = blockIdx.x * static_cast<int64_t>(kScanTile);
+  uint32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + static_cast<int64_t>(k) * kScanBlock + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  uint32_t tot;
+  (void)block_exclusive_scan(s, &tot, sh);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, int64_t nb) {
+  __shared__ uint32_t sh[kScanBlock / kWave];
+  uint32_t carry = 0;
+  for (int64_t base = 0; base < nb; base += kScanBlock) {
+    const int64_t i = base + threadIdx.x;
+    const uint32_t v = i < nb ? sums[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, &tot, sh);
+    if (i < nb) sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) sums[nb] = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t* data, int64_t n, const uint32_t* sums) {
+  __shared__ uint32_t sh[kScanBlock / kWave];
+  const int64_t base = blockIdx.x * static_cast<int64_t>(kScanTile) + static_cast<int64_t>(threadIdx.x) * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    v[k] = i < n ? data[i] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = sums[blockIdx.x] + block_exclusive_scan(s, &tot, sh);
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    if (i < n) data[i] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) data[n] = sums[gridDim.x];
+}
+
+inline int64_t blocks_for(int64_t n, int64_t per) { return (n + per - 1) / per; }
+
+}  // namespace
+
+int64_t scan_tmp_words(int64_t n) { return blocks_for(n, kScanTile) + 2; }
+
+hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream_t s) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counts, 1, 0u);
+    return hipGetLastError();
+  }
+  const int64_t nb = blocks_for(n, kScanTile);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, counts, n, tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, counts, n, tmp);
+  return hipGetLastError();
+}
+
+hipError_t launch_prepare(const Pipeline& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, s, p);
+  if (p.n > 0) hipLaunchKernelGGL(k_prepare, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p);
+  if (p.periodic && p.n > 0) hipLaunchKernelGGL(k_populate, dim3(blocks_for(p.n * 26, 256)), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid(const Pipeline& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(p.n_buckets + 1, 256) < 4096 ? blocks_for(p.n_buckets + 1, 256) : 4096),
+                     dim3(256), 0, s, p.bucket_cnt, p.n_buckets + 1, 0u);
+  if (p.n_pop > 0) hipLaunchKernelGGL(k_keys, dim3(blocks_for(p.n_pop, 256)), dim3(256), 0, s, p);
+  // scan scratch lives after the pair counts' own scratch: callers size it.
+  return hipGetLastError();
+}
+
+hipError_t launch_neighbors(const Pipeline& p, hipStream_t s) {
+  if (p.n_pop > 0) hipLaunchKernelGGL(k_scatter, dim3(blocks_for(p.n_pop, 256)), dim3(256), 0, s, p);
+  if (p.n > 0) hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.n, kNbWaves)), dim3(kNbBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs(const Pipeline& p, hipStream_t s) {
+  if (p.n > 0) hipLaunchKernelGGL(k_pair_fill, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace vltr

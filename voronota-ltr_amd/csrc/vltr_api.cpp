@@ -1,0 +1,590 @@
+// Host orchestration and C ABI (include/voronota_ltr.h).
+//
+// Replaces the drivers of src/tessellation.rs: compute_tessellation
+// (:37-48, :199-292, :406-520) and compute_contacts_only (:125-188). Each call
+// runs the device pipeline
+//   prepare -> cell hash -> neighbor lists -> pair list -> clip -> cells ->
+//   compaction
+// on the context's stream; the host reads back three small words (pair total,
+// status, contour overflows) and the compacted outputs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "vltr_internal.h"
+#include "voronota_ltr.h"
+
+using namespace vltr;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define VLTR_HIP(expr)                                                                                   \
+  do {                                                                                                   \
+    hipError_t e_ = (expr);                                                                              \
+    if (e_ != hipSuccess)                                                                                \
+      return fail(e_ == hipErrorOutOfMemory ? VLTR_ERR_OUT_OF_MEMORY : VLTR_ERR_DEVICE,                  \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                                    \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (ptr) {
+      hipError_t e = hipFree(ptr);
+      if (e != hipSuccess) return e;
+      ptr = nullptr;
+      bytes = 0;
+    }
+    const size_t sz = std::max<size_t>(want + want / 4, 256);
+    hipError_t e = hipMalloc(&ptr, sz);
+    if (e != hipSuccess) return e;
+    bytes = sz;
+    return hipSuccess;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+struct HostPinned {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    const size_t sz = std::max<size_t>(want + want / 4, 4096);
+    hipError_t e = hipHostMalloc(&ptr, sz, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    bytes = sz;
+    return hipSuccess;
+  }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+constexpr int kPhases = 6;
+
+}  // namespace
+
+struct vltr_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[kPhases + 1] = {};
+  double timings[kPhases] = {};
+  int nb_cap = 96;
+  // device buffers
+  DevBuf balls, groups, sph, seg_of, segs, bkey, brank, bcnt, sorted_id, sorted_sph, small, nb, nb_cnt, excluded,
+      pair_cnt, pair_a, pair_b, rec, cell, overflow_list, flags, scan_tmp, out_contacts, out_cells, slow_ws;
+  HostPinned pinned;
+  // last device-resident outputs
+  size_t last_contacts = 0, last_cells = 0;
+  size_t last_n = 0;
+  int last_nb_cap = 0;
+};
+
+struct vltr_result {
+  size_t num_balls = 0;
+  std::vector<vltr_contact> contacts;
+  std::vector<vltr_cell> cells;
+  bool has_network = false;
+  std::vector<vltr_cell_vertex> vertices;
+  std::vector<vltr_cell_edge> edges;
+};
+
+namespace {
+
+// Hexagon rotation constants: the reference accumulates FRAC_PI_3 while the
+// angle stays below TAU (contact.rs:455-469) and rotates by each angle with a
+// unit quaternion built from sin/cos of half the angle (nalgebra).
+void init_hexagon_constants() {
+  static bool done = false;
+  if (done) return;
+  double s[5] = {}, c[5] = {};
+  double rot = kFracPi3;
+  int k = 0;
+  while (rot < kTau && k < 5) {
+    s[k] = std::sin(rot / 2.0);
+    c[k] = std::cos(rot / 2.0);
+    rot += kFracPi3;
+    ++k;
+  }
+  set_hexagon_constants(s, c);
+  done = true;
+}
+
+int kbits_for(int64_t n) {
+  // 2^k cells per axis per structure: about cbrt(n / 8) (a compact structure
+  // at protein density spans fewer cells than that), at least 4 per axis.
+  int k = 2;
+  while (k < 9 && (int64_t(1) << (3 * k)) * 8 < n) ++k;
+  return k;
+}
+
+struct Run {
+  Pipeline p{};
+  int64_t n_pairs = 0;
+};
+
+void fill_pointers(vltr_context* ctx, Pipeline& p) {
+  p.balls_in = ctx->balls.as<double4>();
+  p.sph = ctx->sph.as<double4>();
+  p.seg_of = ctx->seg_of.as<int32_t>();
+  p.segs = ctx->segs.as<GridSeg>();
+  p.bucket_key = ctx->bkey.as<uint32_t>();
+  p.bucket_rank = ctx->brank.as<uint32_t>();
+  p.bucket_cnt = ctx->bcnt.as<uint32_t>();
+  p.sorted_id = ctx->sorted_id.as<int32_t>();
+  p.sorted_sph = ctx->sorted_sph.as<double4>();
+  unsigned char* small = ctx->small.as<unsigned char>();
+  p.box_bits = reinterpret_cast<unsigned long long*>(small);
+  p.status = reinterpret_cast<uint32_t*>(small + 8);
+  p.overflow_cnt = reinterpret_cast<uint32_t*>(small + 12);
+  p.nb = ctx->nb.as<int32_t>();
+  p.nb_cnt = ctx->nb_cnt.as<int32_t>();
+  p.excluded = ctx->excluded.as<uint8_t>();
+  p.pair_cnt = ctx->pair_cnt.as<uint32_t>();
+  p.pair_a = ctx->pair_a.as<int32_t>();
+  p.pair_b = ctx->pair_b.as<int32_t>();
+  p.rec = ctx->rec.as<PairRec>();
+  p.cell = ctx->cell.as<CellRec>();
+  p.overflow_list = ctx->overflow_list.as<int32_t>();
+  p.slow_ws = ctx->slow_ws.as<double>();
+}
+
+// Runs the device pipeline on ctx->balls (already on the device). On return
+// the compacted contacts/cells are in ctx->out_contacts / out_cells and their
+// counts in *n_contacts / *n_cells.
+int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic_box* pbox, const int32_t* d_groups,
+                 int64_t n_groups, int mode, size_t* n_contacts, size_t* n_cells) {
+  hipStream_t s = ctx->stream;
+  const bool periodic = pbox != nullptr;
+  const int64_t n_pop = periodic ? 27 * n : n;
+  Pipeline p{};
+  p.n = n;
+  p.n_pop = n_pop;
+  p.n_segs = 1;
+  p.periodic = periodic ? 1 : 0;
+  p.mode = mode;
+  p.probe = probe;
+  if (pbox) {
+    p.pbox.a = V3{pbox->shift_a[0], pbox->shift_a[1], pbox->shift_a[2]};
+    p.pbox.b = V3{pbox->shift_b[0], pbox->shift_b[1], pbox->shift_b[2]};
+    p.pbox.c = V3{pbox->shift_c[0], pbox->shift_c[1], pbox->shift_c[2]};
+  }
+  p.groups = d_groups;
+  p.n_groups = n_groups;
+
+  GridSeg seg{};
+  seg.ball_off = 0;
+  seg.tab_off = 0;
+  seg.kbits = kbits_for(n_pop);
+  seg.n = static_cast<int32_t>(n_pop);
+  p.n_buckets = int64_t(1) << (3 * seg.kbits);
+
+  VLTR_HIP(ctx->sph.ensure(n_pop * sizeof(double4)));
+  VLTR_HIP(ctx->seg_of.ensure(n_pop * sizeof(int32_t)));
+  VLTR_HIP(ctx->segs.ensure(sizeof(GridSeg)));
+  VLTR_HIP(ctx->bkey.ensure(n_pop * sizeof(uint32_t)));
+  VLTR_HIP(ctx->brank.ensure(n_pop * sizeof(uint32_t)));
+  VLTR_HIP(ctx->bcnt.ensure((p.n_buckets + 1) * sizeof(uint32_t)));
+  VLTR_HIP(ctx->sorted_id.ensure(n_pop * sizeof(int32_t)));
+  VLTR_HIP(ctx->sorted_sph.ensure(n_pop * sizeof(double4)));
+  VLTR_HIP(ctx->small.ensure(64));
+  VLTR_HIP(ctx->nb_cnt.ensure(n * sizeof(int32_t)));
+  VLTR_HIP(ctx->excluded.ensure(n));
+  VLTR_HIP(ctx->pair_cnt.ensure((n + 1) * sizeof(uint32_t)));
+  VLTR_HIP(ctx->cell.ensure(n * sizeof(CellRec)));
+  VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(p.n_buckets, 1) + n + 64) * sizeof(uint32_t) * 2));
+  VLTR_HIP(ctx->pinned.ensure(64));
+  VLTR_HIP(hipMemcpyAsync(ctx->segs.ptr, &seg, sizeof(seg), hipMemcpyHostToDevice, s));
+
+  uint32_t* host_words = static_cast<uint32_t*>(ctx->pinned.ptr);
+  VLTR_HIP(hipEventRecord(ctx->ev[0], s));
+  for (;;) {
+    p.nb_cap = ctx->nb_cap;
+    VLTR_HIP(ctx->nb.ensure(static_cast<size_t>(n) * p.nb_cap * sizeof(int32_t)));
+    fill_pointers(ctx, p);
+    VLTR_HIP(launch_prepare(p, s));
+    VLTR_HIP(launch_grid(p, s));
+    VLTR_HIP(launch_scan_u32(p.bucket_cnt, p.n_buckets, ctx->scan_tmp.as<uint32_t>(), s));
+    VLTR_HIP(hipEventRecord(ctx->ev[1], s));
+    VLTR_HIP(launch_neighbors(p, s));
+    VLTR_HIP(launch_scan_u32(p.pair_cnt, n, ctx->scan_tmp.as<uint32_t>(), s));
+    VLTR_HIP(hipMemcpyAsync(host_words, p.pair_cnt + n, 4, hipMemcpyDeviceToHost, s));
+    VLTR_HIP(hipMemcpyAsync(host_words + 1, p.status, 4, hipMemcpyDeviceToHost, s));
+    VLTR_HIP(hipEventRecord(ctx->ev[2], s));
+    VLTR_HIP(hipStreamSynchronize(s));
+    if (host_words[1] & kStatNbOverflow) {
+      if (ctx->nb_cap >= 4096) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
+      ctx->nb_cap *= 2;
+      continue;
+    }
+    break;
+  }
+  const int64_t P = host_words[0];
+  p.n_pairs = P;
+  p.pair_cap = std::max<int64_t>(P, 1);
+  VLTR_HIP(ctx->pair_a.ensure(p.pair_cap * sizeof(int32_t)));
+  VLTR_HIP(ctx->pair_b.ensure(p.pair_cap * sizeof(int32_t)));
+  VLTR_HIP(ctx->rec.ensure(p.pair_cap * sizeof(PairRec)));
+  VLTR_HIP(ctx->overflow_list.ensure(p.pair_cap * sizeof(int32_t)));
+  VLTR_HIP(ctx->flags.ensure((std::max<int64_t>(P, n) + 2) * sizeof(uint32_t)));
+  VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(P, n) + 64) * sizeof(uint32_t) * 2));
+  VLTR_HIP(ctx->out_contacts.ensure(p.pair_cap * sizeof(vltr_contact)));
+  VLTR_HIP(ctx->out_cells.ensure(std::max<int64_t>(n, 1) * sizeof(vltr_cell)));
+  fill_pointers(ctx, p);
+  VLTR_HIP(launch_pairs(p, s));
+  VLTR_HIP(hipEventRecord(ctx->ev[3], s));
+  VLTR_HIP(launch_cut(p, s));
+  VLTR_HIP(hipMemcpyAsync(host_words + 2, p.overflow_cnt, 4, hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipStreamSynchronize(s));
+  const int64_t n_over = host_words[2];
+  if (n_over > 0) {
+    const int cc = 6 + p.nb_cap, cr = 2 * cc;
+    p.slow_ws_pairs = std::min<int64_t>(n_over, 8192);
+    VLTR_HIP(ctx->slow_ws.ensure(static_cast<size_t>(p.slow_ws_pairs) * (5 * cc + 6 * cr) * sizeof(double)));
+    p.slow_ws = ctx->slow_ws.as<double>();
+    VLTR_HIP(launch_cut_slow(p, n_over, s));
+  }
+  VLTR_HIP(hipEventRecord(ctx->ev[4], s));
+  if (mode == 0) VLTR_HIP(launch_cells(p, s));
+  uint32_t* flags = ctx->flags.as<uint32_t>();
+  VLTR_HIP(launch_flag_contacts(p, flags, s));
+  VLTR_HIP(launch_scan_u32(flags, P, ctx->scan_tmp.as<uint32_t>(), s));
+  VLTR_HIP(launch_emit_contacts(p, flags, ctx->out_contacts.as<vltr_contact>(), s));
+  VLTR_HIP(hipMemcpyAsync(host_words + 3, flags + P, 4, hipMemcpyDeviceToHost, s));
+  if (mode == 0) {
+    VLTR_HIP(launch_flag_cells(p, flags, s));
+    VLTR_HIP(launch_scan_u32(flags, n, ctx->scan_tmp.as<uint32_t>(), s));
+    VLTR_HIP(launch_emit_cells(p, flags, ctx->out_cells.as<vltr_cell>(), s));
+    VLTR_HIP(hipMemcpyAsync(host_words + 4, flags + n, 4, hipMemcpyDeviceToHost, s));
+  }
+  VLTR_HIP(hipEventRecord(ctx->ev[5], s));
+  VLTR_HIP(hipStreamSynchronize(s));
+  *n_contacts = host_words[3];
+  *n_cells = mode == 0 ? host_words[4] : 0;
+  ctx->last_n = static_cast<size_t>(n);
+  ctx->last_nb_cap = p.nb_cap;
+  float ms = 0.0f;
+  for (int k = 0; k < 5; ++k) {
+    if (hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess) ctx->timings[k] = ms;
+  }
+  if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[5]) == hipSuccess) ctx->timings[5] = ms;
+  return VLTR_OK;
+}
+
+// tessellation.rs:527-601: among summaries touching a periodic image, keep the
+// first one of each unordered canonical pair; ids are reported modulo n.
+void periodic_dedup_full(std::vector<vltr_contact>& v, uint64_t n) {
+  std::unordered_map<uint64_t, size_t> first;
+  std::vector<vltr_contact> out;
+  out.reserve(v.size());
+  for (size_t i = 0; i < v.size(); ++i) {
+    vltr_contact c = v[i];
+    const bool boundary = c.id_a >= n || c.id_b >= n;
+    const uint64_t ca = c.id_a % n, cb = c.id_b % n;
+    if (boundary) {
+      const uint64_t key = std::min(ca, cb) * n + std::max(ca, cb);
+      if (first.emplace(key, i).second == false) continue;
+    }
+    c.id_a = ca;
+    c.id_b = cb;
+    out.push_back(c);
+  }
+  v.swap(out);
+}
+
+// tessellation.rs:172-188: first occurrence per canonical (min, max) pair,
+// ids canonical and ordered.
+void periodic_dedup_contacts_only(std::vector<vltr_contact>& v, uint64_t n) {
+  std::unordered_map<uint64_t, size_t> first;
+  std::vector<vltr_contact> out;
+  out.reserve(v.size());
+  for (const auto& c0 : v) {
+    vltr_contact c = c0;
+    uint64_t ca = c.id_a % n, cb = c.id_b % n;
+    if (ca > cb) std::swap(ca, cb);
+    if (!first.emplace(ca * n + cb, out.size()).second) continue;
+    c.id_a = ca;
+    c.id_b = cb;
+    out.push_back(c);
+  }
+  v.swap(out);
+}
+
+int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe, const vltr_periodic_box* pbox,
+               const int32_t* groups, size_t n_groups, int with_vertices, int mode, vltr_result** out) {
+  if (!ctx || !out || (n > 0 && !balls)) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  if (with_vertices) return fail(VLTR_ERR_INVALID_ARGUMENT, "with_cell_vertices is not supported by this build");
+  if (n > (size_t(1) << 31) / 27) return fail(VLTR_ERR_INVALID_ARGUMENT, "too many balls");
+  *out = nullptr;
+  auto* r = new (std::nothrow) vltr_result();
+  if (!r) return fail(VLTR_ERR_OUT_OF_MEMORY, "result allocation");
+  r->num_balls = mode == 0 ? n : n;
+  if (n == 0) {
+    *out = r;
+    return VLTR_OK;
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) {
+    delete r;
+    return fail(VLTR_ERR_DEVICE, "hipSetDevice failed");
+  }
+  hipStream_t s = ctx->stream;
+  int rc = VLTR_OK;
+  do {
+    if (ctx->balls.ensure(n * sizeof(vltr_ball)) != hipSuccess) {
+      rc = fail(VLTR_ERR_OUT_OF_MEMORY, "ball buffer");
+      break;
+    }
+    if (hipMemcpyAsync(ctx->balls.ptr, balls, n * sizeof(vltr_ball), hipMemcpyHostToDevice, s) != hipSuccess) {
+      rc = fail(VLTR_ERR_DEVICE, "ball upload");
+      break;
+    }
+    const int32_t* dg = nullptr;
+    if (groups && n_groups > 0) {
+      if (ctx->groups.ensure(n_groups * sizeof(int32_t)) != hipSuccess ||
+          hipMemcpyAsync(ctx->groups.ptr, groups, n_groups * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess) {
+        rc = fail(VLTR_ERR_DEVICE, "group upload");
+        break;
+      }
+      dg = ctx->groups.as<int32_t>();
+    }
+    size_t nc = 0, ncell = 0;
+    rc = run_pipeline(ctx, static_cast<int64_t>(n), probe, pbox, dg, dg ? static_cast<int64_t>(n_groups) : 0, mode, &nc,
+                      &ncell);
+    if (rc != VLTR_OK) break;
+    r->contacts.resize(nc);
+    r->cells.resize(ncell);
+    if (nc && hipMemcpyAsync(r->contacts.data(), ctx->out_contacts.ptr, nc * sizeof(vltr_contact),
+                             hipMemcpyDeviceToHost, s) != hipSuccess) {
+      rc = fail(VLTR_ERR_DEVICE, "contact download");
+      break;
+    }
+    if (ncell && hipMemcpyAsync(r->cells.data(), ctx->out_cells.ptr, ncell * sizeof(vltr_cell), hipMemcpyDeviceToHost,
+                                s) != hipSuccess) {
+      rc = fail(VLTR_ERR_DEVICE, "cell download");
+      break;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      rc = fail(VLTR_ERR_DEVICE, "stream synchronize");
+      break;
+    }
+    if (pbox) {
+      if (mode == 0)
+        periodic_dedup_full(r->contacts, n);
+      else
+        periodic_dedup_contacts_only(r->contacts, n);
+    }
+  } while (false);
+  if (rc != VLTR_OK) {
+    delete r;
+    return rc;
+  }
+  *out = r;
+  return VLTR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vltr_last_error(void) { return g_last_error.c_str(); }
+
+int vltr_context_create(int device, vltr_context** out) {
+  if (!out) return fail(VLTR_ERR_INVALID_ARGUMENT, "null out");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(VLTR_ERR_DEVICE, "no HIP device available (this library runs on the GPU only)");
+  if (device < 0 || device >= count) return fail(VLTR_ERR_INVALID_ARGUMENT, "device index out of range");
+  VLTR_HIP(hipSetDevice(device));
+  auto* ctx = new (std::nothrow) vltr_context();
+  if (!ctx) return fail(VLTR_ERR_OUT_OF_MEMORY, "context allocation");
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return fail(VLTR_ERR_DEVICE, "stream creation failed");
+  }
+  for (auto& e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      vltr_context_destroy(ctx);
+      return fail(VLTR_ERR_DEVICE, "event creation failed");
+    }
+  init_hexagon_constants();
+  if (upload_hexagon_constants(ctx->stream) != hipSuccess) {
+    vltr_context_destroy(ctx);
+    return fail(VLTR_ERR_DEVICE, "constant upload failed");
+  }
+  *out = ctx;
+  return VLTR_OK;
+}
+
+void vltr_context_destroy(vltr_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (DevBuf* b : {&ctx->balls, &ctx->groups, &ctx->sph, &ctx->seg_of, &ctx->segs, &ctx->bkey, &ctx->brank, &ctx->bcnt,
+                    &ctx->sorted_id, &ctx->sorted_sph, &ctx->small, &ctx->nb, &ctx->nb_cnt, &ctx->excluded,
+                    &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec, &ctx->cell, &ctx->overflow_list,
+                    &ctx->flags, &ctx->scan_tmp, &ctx->out_contacts, &ctx->out_cells, &ctx->slow_ws})
+    b->release();
+  ctx->pinned.release();
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int vltr_context_last_timings(const vltr_context* ctx, double* ms, int max_entries) {
+  if (!ctx || !ms) return 0;
+  const int k = std::min(max_entries, kPhases);
+  for (int i = 0; i < k; ++i) ms[i] = ctx->timings[i];
+  return k;
+}
+
+void vltr_periodic_box_from_corners(const double mn[3], const double mx[3], vltr_periodic_box* out) {
+  if (!out) return;
+  *out = vltr_periodic_box{{mx[0] - mn[0], 0.0, 0.0}, {0.0, mx[1] - mn[1], 0.0}, {0.0, 0.0, mx[2] - mn[2]}};
+}
+
+void vltr_periodic_box_from_vectors(const double a[3], const double b[3], const double c[3], vltr_periodic_box* out) {
+  if (!out) return;
+  *out = vltr_periodic_box{{a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {c[0], c[1], c[2]}};
+}
+
+int vltr_compute_tessellation(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe,
+                              const vltr_periodic_box* periodic_box, const int32_t* groups, size_t n_groups,
+                              int with_cell_vertices, vltr_result** out) {
+  return host_entry(ctx, balls, n, probe, periodic_box, groups, n_groups, with_cell_vertices, 0, out);
+}
+
+int vltr_compute_contacts_only(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe,
+                               const vltr_periodic_box* periodic_box, const int32_t* groups, size_t n_groups,
+                               vltr_result** out) {
+  return host_entry(ctx, balls, n, probe, periodic_box, groups, n_groups, 0, 1, out);
+}
+
+int vltr_compute_tessellation_device(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
+                                     const int32_t* d_groups, size_t n_groups, int contacts_only) {
+  if (!ctx || (n > 0 && !d_balls)) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  ctx->last_contacts = ctx->last_cells = 0;
+  if (n == 0) return VLTR_OK;
+  VLTR_HIP(hipSetDevice(ctx->device));
+  VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
+  VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, d_balls, n * sizeof(vltr_ball), hipMemcpyDeviceToDevice, ctx->stream));
+  size_t nc = 0, ncell = 0;
+  const int rc = run_pipeline(ctx, static_cast<int64_t>(n), probe, nullptr, d_groups,
+                              d_groups ? static_cast<int64_t>(n_groups) : 0, contacts_only ? 1 : 0, &nc, &ncell);
+  if (rc != VLTR_OK) return rc;
+  ctx->last_contacts = nc;
+  ctx->last_cells = ncell;
+  return VLTR_OK;
+}
+
+int vltr_context_device_outputs(const vltr_context* ctx, const vltr_contact** d_contacts, size_t* n_contacts,
+                                const vltr_cell** d_cells, size_t* n_cells) {
+  if (!ctx) return fail(VLTR_ERR_INVALID_ARGUMENT, "null context");
+  if (d_contacts) *d_contacts = ctx->out_contacts.as<const vltr_contact>();
+  if (n_contacts) *n_contacts = ctx->last_contacts;
+  if (d_cells) *d_cells = ctx->out_cells.as<const vltr_cell>();
+  if (n_cells) *n_cells = ctx->last_cells;
+  return VLTR_OK;
+}
+
+int64_t vltr_context_neighbor_lists(const vltr_context* ctx, size_t num_balls, int32_t* counts, int32_t* ids,
+                                    size_t max_ids) {
+  if (!ctx || !counts || ctx->last_n != num_balls || ctx->nb.ptr == nullptr) {
+    fail(VLTR_ERR_INVALID_ARGUMENT, "no pipeline run with that many balls on this context");
+    return -1;
+  }
+  if (num_balls == 0) return 0;
+  if (hipSetDevice(ctx->device) != hipSuccess) return -1;
+  std::vector<int32_t> all(num_balls * static_cast<size_t>(ctx->last_nb_cap));
+  if (hipMemcpy(counts, ctx->nb_cnt.ptr, num_balls * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(all.data(), ctx->nb.ptr, all.size() * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+    fail(VLTR_ERR_DEVICE, "neighbor list download");
+    return -1;
+  }
+  int64_t total = 0;
+  for (size_t i = 0; i < num_balls; ++i) {
+    for (int k = 0; k < counts[i]; ++k) {
+      if (ids && static_cast<size_t>(total) < max_ids) ids[total] = all[i * ctx->last_nb_cap + k];
+      ++total;
+    }
+  }
+  return total;
+}
+
+size_t vltr_result_num_balls(const vltr_result* r) { return r ? r->num_balls : 0; }
+size_t vltr_result_num_contacts(const vltr_result* r) { return r ? r->contacts.size() : 0; }
+const vltr_contact* vltr_result_contacts(const vltr_result* r) { return r ? r->contacts.data() : nullptr; }
+size_t vltr_result_num_cells(const vltr_result* r) { return r ? r->cells.size() : 0; }
+const vltr_cell* vltr_result_cells(const vltr_result* r) { return r ? r->cells.data() : nullptr; }
+int vltr_result_has_cell_vertices(const vltr_result* r) { return r && r->has_network ? 1 : 0; }
+size_t vltr_result_num_cell_vertices(const vltr_result* r) { return r ? r->vertices.size() : 0; }
+const vltr_cell_vertex* vltr_result_cell_vertices(const vltr_result* r) { return r ? r->vertices.data() : nullptr; }
+size_t vltr_result_num_cell_edges(const vltr_result* r) { return r ? r->edges.size() : 0; }
+const vltr_cell_edge* vltr_result_cell_edges(const vltr_result* r) { return r ? r->edges.data() : nullptr; }
+
+// types.rs:231-252: plain sums in vector order.
+double vltr_result_total_sas_area(const vltr_result* r) {
+  double s = 0.0;
+  if (r)
+    for (const auto& c : r->cells) s += c.sas_area;
+  return s;
+}
+double vltr_result_total_volume(const vltr_result* r) {
+  double s = 0.0;
+  if (r)
+    for (const auto& c : r->cells) s += c.volume;
+  return s;
+}
+double vltr_result_total_contact_area(const vltr_result* r) {
+  double s = 0.0;
+  if (r)
+    for (const auto& c : r->contacts) s += c.area;
+  return s;
+}
+// types.rs:206-228
+void vltr_result_sas_areas(const vltr_result* r, double* out) {
+  if (!r || !out) return;
+  for (size_t i = 0; i < r->num_balls; ++i) out[i] = std::nan("");
+  for (const auto& c : r->cells)
+    if (c.index < r->num_balls) out[c.index] = c.sas_area;
+}
+void vltr_result_volumes(const vltr_result* r, double* out) {
+  if (!r || !out) return;
+  for (size_t i = 0; i < r->num_balls; ++i) out[i] = std::nan("");
+  for (const auto& c : r->cells)
+    if (c.index < r->num_balls) out[c.index] = c.volume;
+}
+void vltr_result_free(vltr_result* r) { delete r; }
+
+}  // extern "C"

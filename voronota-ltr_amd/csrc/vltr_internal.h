@@ -1,0 +1,106 @@
+// Internal declarations shared by the kernel translation units and the host
+// orchestration (not part of the public C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "contour.h"
+#include "vltr_geom.h"
+#include "voronota_ltr.h"
+
+namespace vltr {
+
+// Per-ball cell accumulator result (types.rs:431-454): stage 0 = no cell,
+// 2 = SAS computed (cells are emitted for stage 2 only).
+struct CellRec {
+  double sas;
+  double volume;
+  int32_t stage;
+  int32_t pad;
+};
+
+// Geometry of the uniform cell hash: structure s owns buckets
+// [tab_off[s], tab_off[s] + 8^kbits[s]); a ball's bucket is the Morton code of
+// its cell coordinates modulo 2^kbits[s] per axis.
+struct GridSeg {
+  int64_t ball_off;
+  int64_t tab_off;
+  int32_t kbits;
+  int32_t n;
+};
+
+// Device-global status word (atomically OR-ed by kernels).
+constexpr uint32_t kStatNbOverflow = 1u;    // a neighbor list exceeded its capacity
+constexpr uint32_t kStatPairOverflow = 2u;  // more pairs than the pair buffers hold
+constexpr uint32_t kStatContourOverflow = 4u;
+
+// Launch parameters of the tessellation pipeline for one call (one or many
+// structures). All pointers are device pointers.
+struct Pipeline {
+  int64_t n;         // balls (canonical)
+  int64_t n_pop;     // spheres searched (n, or 27 n with periodic images)
+  int32_t n_segs;
+  int32_t periodic;  // 1: sph holds 27 n images, ids >= n are images
+  int32_t nb_cap;    // neighbor slots per ball
+  int32_t mode;      // 0 full tessellation, 1 contacts only
+  int64_t pair_cap;  // capacity of the pair buffers
+  int64_t n_pairs;   // pairs to clip (set once the pair counts are known)
+  double probe;
+  PBox pbox;
+
+  const double4* balls_in;  // n (x, y, z, r)
+  const int32_t* groups;    // may be null
+  int64_t n_groups;
+
+  double4* sph;          // n_pop (x, y, z, R)
+  int32_t* seg_of;       // n_pop (segment of each sphere)
+  const GridSeg* segs;   // n_segs
+  int64_t n_buckets;     // total buckets over all segments
+  uint32_t* bucket_key;  // n_pop
+  uint32_t* bucket_rank; // n_pop
+  uint32_t* bucket_cnt;  // n_buckets + 1 (scanned in place to starts)
+  int32_t* sorted_id;    // n_pop
+  double4* sorted_sph;   // n_pop
+  unsigned long long* box_bits;  // max over 2R + 0.25 as double bits
+
+  int32_t* nb;           // n * nb_cap neighbor ids (sorted by key, id)
+  int32_t* nb_cnt;       // n
+  uint8_t* excluded;     // n (hidden balls)
+  uint32_t* pair_cnt;    // n + 1 (scanned in place to bases)
+  int32_t* pair_a;       // pair_cap
+  int32_t* pair_b;       // pair_cap
+  PairRec* rec;          // pair_cap
+  CellRec* cell;         // n
+  uint32_t* status;      // 1 word
+  int32_t* overflow_list;  // pair_cap (pairs needing the slow kernel)
+  uint32_t* overflow_cnt;  // 1
+  double* slow_ws;         // workspace of the slow kernel
+  int64_t slow_ws_pairs;   // pairs the workspace holds at once
+};
+
+// Host-side launchers (k_grid.hip).
+hipError_t launch_prepare(const Pipeline& p, hipStream_t s);
+hipError_t launch_grid(const Pipeline& p, hipStream_t s);
+hipError_t launch_neighbors(const Pipeline& p, hipStream_t s);
+hipError_t launch_pairs(const Pipeline& p, hipStream_t s);
+// Exclusive scan of counts[0..n) in place into counts[0..n], counts[n] = total.
+hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream_t s);
+int64_t scan_tmp_words(int64_t n);
+
+// k_contact.hip
+hipError_t launch_cut(const Pipeline& p, hipStream_t s);
+hipError_t launch_cut_slow(const Pipeline& p, int64_t n_overflow, hipStream_t s);
+void set_hexagon_constants(const double* s_half5, const double* c_half5);
+hipError_t upload_hexagon_constants(hipStream_t s);
+
+// k_cells.hip
+hipError_t launch_cells(const Pipeline& p, hipStream_t s);
+hipError_t launch_flag_contacts(const Pipeline& p, uint32_t* flag, hipStream_t s);
+hipError_t launch_emit_contacts(const Pipeline& p, const uint32_t* pos, vltr_contact* out, hipStream_t s);
+hipError_t launch_flag_cells(const Pipeline& p, uint32_t* flag, hipStream_t s);
+hipError_t launch_emit_cells(const Pipeline& p, const uint32_t* pos, vltr_cell* out, hipStream_t s);
+
+}  // namespace vltr

@@ -1,0 +1,232 @@
+"""ctypes binding of the C ABI in include/voronota_ltr.h (libvltr_hip.so).
+
+The HIP library is the only compute path: if it is missing or no GPU is
+visible, every call raises. ctypes releases the GIL around each foreign call,
+as the reference's PyO3 module does around compute_tessellation
+(src/python.rs:253).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvltr_hip.so")
+
+VLTR_OK = 0
+NULL_ID = np.iinfo(np.uint64).max
+
+
+class VltrBall(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("r", C.c_double)]
+
+
+class VltrPeriodicBox(C.Structure):
+    _fields_ = [("shift_a", C.c_double * 3), ("shift_b", C.c_double * 3), ("shift_c", C.c_double * 3)]
+
+
+# numpy views of the ABI structs
+CONTACT_DTYPE = np.dtype([("id_a", "<u8"), ("id_b", "<u8"), ("area", "<f8"), ("arc_length", "<f8"),
+                          ("central", "u1"), ("reserved", "u1", (7,))])
+CELL_DTYPE = np.dtype([("index", "<u8"), ("sas_area", "<f8"), ("volume", "<f8")])
+VERTEX_DTYPE = np.dtype([("ball_indices", "<u8", (4,)), ("x", "<f8"), ("y", "<f8"), ("z", "<f8")])
+EDGE_DTYPE = np.dtype([("ball_indices", "<u8", (3,)), ("length", "<f8")])
+assert CONTACT_DTYPE.itemsize == 40 and CELL_DTYPE.itemsize == 24
+
+_vp = C.c_void_p
+_sz = C.c_size_t
+_f64p = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+
+# every symbol include/voronota_ltr.h declares, with (restype, argtypes)
+SIGNATURES = {
+    "vltr_context_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "vltr_context_destroy": (None, [_vp]),
+    "vltr_last_error": (C.c_char_p, []),
+    "vltr_context_last_timings": (C.c_int, [_vp, _f64p, C.c_int]),
+    "vltr_periodic_box_from_corners": (None, [_f64p, _f64p, C.POINTER(VltrPeriodicBox)]),
+    "vltr_periodic_box_from_vectors": (None, [_f64p, _f64p, _f64p, C.POINTER(VltrPeriodicBox)]),
+    "vltr_compute_tessellation": (C.c_int, [_vp, _vp, _sz, C.c_double, C.POINTER(VltrPeriodicBox), _i32p, _sz,
+                                            C.c_int, C.POINTER(_vp)]),
+    "vltr_compute_contacts_only": (C.c_int, [_vp, _vp, _sz, C.c_double, C.POINTER(VltrPeriodicBox), _i32p, _sz,
+                                             C.POINTER(_vp)]),
+    "vltr_compute_tessellation_device": (C.c_int, [_vp, _vp, _sz, C.c_double, _vp, _sz, C.c_int]),
+    "vltr_context_device_outputs": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_sz), C.POINTER(_vp), C.POINTER(_sz)]),
+    "vltr_context_neighbor_lists": (C.c_int64, [_vp, _sz, _i32p, _i32p, _sz]),
+    "vltr_result_num_balls": (_sz, [_vp]),
+    "vltr_result_num_contacts": (_sz, [_vp]),
+    "vltr_result_contacts": (_vp, [_vp]),
+    "vltr_result_num_cells": (_sz, [_vp]),
+    "vltr_result_cells": (_vp, [_vp]),
+    "vltr_result_has_cell_vertices": (C.c_int, [_vp]),
+    "vltr_result_num_cell_vertices": (_sz, [_vp]),
+    "vltr_result_cell_vertices": (_vp, [_vp]),
+    "vltr_result_num_cell_edges": (_sz, [_vp]),
+    "vltr_result_cell_edges": (_vp, [_vp]),
+    "vltr_result_total_sas_area": (C.c_double, [_vp]),
+    "vltr_result_total_volume": (C.c_double, [_vp]),
+    "vltr_result_total_contact_area": (C.c_double, [_vp]),
+    "vltr_result_sas_areas": (None, [_vp, _f64p]),
+    "vltr_result_volumes": (None, [_vp, _f64p]),
+    "vltr_result_free": (None, [_vp]),
+}
+
+_lock = threading.Lock()
+_lib = None
+_contexts: dict[int, int] = {}
+
+
+class VltrError(RuntimeError):
+    pass
+
+
+def load_library():
+    """Loads libvltr_hip.so and binds its signatures (no GPU needed)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise VltrError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+            lib = C.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    msg = load_library().vltr_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str):
+    if rc != VLTR_OK:
+        raise VltrError(f"{what} failed (status {rc}): {last_error()}")
+
+
+def context(device: int = 0) -> int:
+    """Per-device library context, created on first use."""
+    lib = load_library()
+    with _lock:
+        h = _contexts.get(device)
+        if h is None:
+            out = C.c_void_p()
+            check(lib.vltr_context_create(device, C.byref(out)), "vltr_context_create")
+            h = out.value
+            _contexts[device] = h
+    return h
+
+
+def last_timings(device: int = 0):
+    lib = load_library()
+    buf = (C.c_double * 6)()
+    k = lib.vltr_context_last_timings(context(device), buf, 6)
+    names = ["grid", "neighbors", "pairs", "cut", "cells", "total"]
+    return {names[i]: buf[i] for i in range(k)}
+
+
+def neighbor_lists(num_balls: int, device: int = 0):
+    """(counts, ids) of the last pipeline run on the device's context."""
+    lib = load_library()
+    ctx = context(device)
+    counts = np.zeros(num_balls, dtype=np.int32)
+    total = lib.vltr_context_neighbor_lists(ctx, num_balls, counts.ctypes.data_as(_i32p), None, 0)
+    if total < 0:
+        raise VltrError(last_error())
+    ids = np.zeros(max(total, 1), dtype=np.int32)
+    lib.vltr_context_neighbor_lists(ctx, num_balls, counts.ctypes.data_as(_i32p), ids.ctypes.data_as(_i32p), total)
+    return counts, ids[:total]
+
+
+def make_pbox(pbox) -> VltrPeriodicBox | None:
+    """Accepts None, a VltrPeriodicBox, or 9 numbers (shift_a | shift_b | shift_c)."""
+    if pbox is None:
+        return None
+    if isinstance(pbox, VltrPeriodicBox):
+        return pbox
+    v = np.asarray(pbox, dtype=np.float64).reshape(9)
+    b = VltrPeriodicBox()
+    for k in range(3):
+        b.shift_a[k] = v[k]
+        b.shift_b[k] = v[3 + k]
+        b.shift_c[k] = v[6 + k]
+    return b
+
+
+class Result:
+    """Host copy of a vltr_result: numpy structured arrays."""
+
+    def __init__(self, handle):
+        lib = load_library()
+        self.num_balls = lib.vltr_result_num_balls(handle)
+        nc = lib.vltr_result_num_contacts(handle)
+        ncell = lib.vltr_result_num_cells(handle)
+        self.contacts = _copy(lib.vltr_result_contacts(handle), nc, CONTACT_DTYPE)
+        self.cells = _copy(lib.vltr_result_cells(handle), ncell, CELL_DTYPE)
+        self.total_sas_area = lib.vltr_result_total_sas_area(handle)
+        self.total_volume = lib.vltr_result_total_volume(handle)
+        self.total_contact_area = lib.vltr_result_total_contact_area(handle)
+        self.cell_vertices = None
+        self.cell_edges = None
+        if lib.vltr_result_has_cell_vertices(handle):
+            self.cell_vertices = _copy(lib.vltr_result_cell_vertices(handle),
+                                       lib.vltr_result_num_cell_vertices(handle), VERTEX_DTYPE)
+            self.cell_edges = _copy(lib.vltr_result_cell_edges(handle), lib.vltr_result_num_cell_edges(handle),
+                                    EDGE_DTYPE)
+
+    def sas_areas(self):
+        """Per-ball SAS, NaN where the reference returns None (types.rs:206-213)."""
+        out = np.full(self.num_balls, np.nan)
+        out[self.cells["index"].astype(np.int64)] = self.cells["sas_area"]
+        return out
+
+    def volumes(self):
+        out = np.full(self.num_balls, np.nan)
+        out[self.cells["index"].astype(np.int64)] = self.cells["volume"]
+        return out
+
+
+def _copy(ptr, n, dtype):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    buf = (C.c_char * (n * dtype.itemsize)).from_address(ptr)
+    return np.frombuffer(bytes(buf), dtype=dtype).copy()
+
+
+def _balls_array(balls) -> np.ndarray:
+    b = np.ascontiguousarray(np.asarray(balls, dtype=np.float64))
+    if b.size == 0:
+        return np.zeros((0, 4), dtype=np.float64)
+    return b.reshape(-1, 4)
+
+
+def run(balls, probe, pbox=None, groups=None, with_cell_vertices=False, contacts_only=False, device=0) -> Result:
+    lib = load_library()
+    ctx = context(device)
+    b = _balls_array(balls)
+    box = make_pbox(pbox)
+    g = None
+    if groups is not None:
+        g = np.ascontiguousarray(np.asarray(groups, dtype=np.int32))
+    handle = C.c_void_p()
+    bp = b.ctypes.data_as(_vp) if len(b) else None
+    gp = g.ctypes.data_as(_i32p) if g is not None and len(g) else None
+    ng = len(g) if g is not None else 0
+    boxp = C.byref(box) if box is not None else None
+    if contacts_only:
+        rc = lib.vltr_compute_contacts_only(ctx, bp, len(b), float(probe), boxp, gp, ng, C.byref(handle))
+        check(rc, "vltr_compute_contacts_only")
+    else:
+        rc = lib.vltr_compute_tessellation(ctx, bp, len(b), float(probe), boxp, gp, ng, int(bool(with_cell_vertices)),
+                                           C.byref(handle))
+        check(rc, "vltr_compute_tessellation")
+    try:
+        return Result(handle)
+    finally:
+        lib.vltr_result_free(handle)

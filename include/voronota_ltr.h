@@ -93,6 +93,12 @@ const char* vltr_last_error(void);
  * neighbors, pairs, cut, cells+compaction, total. Returns the count written. */
 int vltr_context_last_timings(const vltr_context* ctx, double* ms, int max_entries);
 
+/* Work counts of the last pipeline run on ctx: out[0] balls, [1] searched
+ * spheres (27 n when periodic), [2] candidate pairs clipped, [3] contacts,
+ * [4] cells, [5] pairs redone by the large-contour kernel, [6] neighbor
+ * slots per ball. Returns the count written. */
+int vltr_context_last_counts(const vltr_context* ctx, uint64_t* out, int max_entries);
+
 /* ------------------------------------------------------- periodic box --- */
 
 /* PeriodicBox::from_corners (types.rs:302-310). */

@@ -107,6 +107,7 @@ struct vltr_context {
   size_t last_contacts = 0, last_cells = 0;
   size_t last_n = 0;
   int last_nb_cap = 0;
+  uint64_t counts[7] = {};
 };
 
 struct vltr_result {
@@ -292,6 +293,13 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   *n_cells = mode == 0 ? host_words[4] : 0;
   ctx->last_n = static_cast<size_t>(n);
   ctx->last_nb_cap = p.nb_cap;
+  ctx->counts[0] = static_cast<uint64_t>(n);
+  ctx->counts[1] = static_cast<uint64_t>(n_pop);
+  ctx->counts[2] = static_cast<uint64_t>(P);
+  ctx->counts[3] = *n_contacts;
+  ctx->counts[4] = *n_cells;
+  ctx->counts[5] = static_cast<uint64_t>(n_over);
+  ctx->counts[6] = static_cast<uint64_t>(p.nb_cap);
   float ms = 0.0f;
   for (int k = 0; k < 5; ++k) {
     if (hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess) ctx->timings[k] = ms;
@@ -466,6 +474,13 @@ int vltr_context_last_timings(const vltr_context* ctx, double* ms, int max_entri
   if (!ctx || !ms) return 0;
   const int k = std::min(max_entries, kPhases);
   for (int i = 0; i < k; ++i) ms[i] = ctx->timings[i];
+  return k;
+}
+
+int vltr_context_last_counts(const vltr_context* ctx, uint64_t* out, int max_entries) {
+  if (!ctx || !out) return 0;
+  const int k = std::min(max_entries, 7);
+  for (int i = 0; i < k; ++i) out[i] = ctx->counts[i];
   return k;
 }
 

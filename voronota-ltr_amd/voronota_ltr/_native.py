@@ -48,6 +48,7 @@ SIGNATURES = {
     "vltr_context_destroy": (None, [_vp]),
     "vltr_last_error": (C.c_char_p, []),
     "vltr_context_last_timings": (C.c_int, [_vp, _f64p, C.c_int]),
+    "vltr_context_last_counts": (C.c_int, [_vp, _u64p, C.c_int]),
     "vltr_periodic_box_from_corners": (None, [_f64p, _f64p, C.POINTER(VltrPeriodicBox)]),
     "vltr_periodic_box_from_vectors": (None, [_f64p, _f64p, _f64p, C.POINTER(VltrPeriodicBox)]),
     "vltr_compute_tessellation": (C.c_int, [_vp, _vp, _sz, C.c_double, C.POINTER(VltrPeriodicBox), _i32p, _sz,
@@ -129,6 +130,14 @@ def last_timings(device: int = 0):
     k = lib.vltr_context_last_timings(context(device), buf, 6)
     names = ["grid", "neighbors", "pairs", "cut", "cells", "total"]
     return {names[i]: buf[i] for i in range(k)}
+
+
+def last_counts(device: int = 0):
+    lib = load_library()
+    buf = (C.c_uint64 * 7)()
+    k = lib.vltr_context_last_counts(context(device), buf, 7)
+    names = ["balls", "spheres", "pairs", "contacts", "cells", "slow_pairs", "nb_cap"]
+    return {names[i]: int(buf[i]) for i in range(k)}
 
 
 def neighbor_lists(num_balls: int, device: int = 0):

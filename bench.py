@@ -63,13 +63,13 @@ def main():
         rc = lib.vltr_compute_tessellation_device(ctx, dbuf.ptr, args.n, args.probe, None, 0, contacts_only)
         _native.check(rc, "vltr_compute_tessellation_device")
 
-    phases = ["grid", "neighbors", "pairs", "cut", "cells", "total"]
+    phases = ["grid", "neighbors", "pairs", "prefilter", "cut", "cells", "total"]
 
     def timed(contacts_only):
         for _ in range(args.warmup):
             step(contacts_only)
         _hip.synchronize()
-        acc = np.zeros(6)
+        acc = np.zeros(len(phases))
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step(contacts_only)

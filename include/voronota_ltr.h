@@ -90,7 +90,8 @@ const char* vltr_last_error(void);
 
 /* Per-phase device time of the last pipeline run on ctx, in milliseconds,
  * measured with HIP events on the context stream. Order: prepare+grid,
- * neighbors, pairs, cut, cells+compaction, total. Returns the count written. */
+ * neighbors, pairs, prefilter, cut, cells+compaction, total. Returns the count
+ * written. */
 int vltr_context_last_timings(const vltr_context* ctx, double* ms, int max_entries);
 
 /* Work counts of the last pipeline run on ctx: out[0] balls, [1] searched

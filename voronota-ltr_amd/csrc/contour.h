@@ -329,70 +329,145 @@ VLTR_HD double contour_area(const A& ct, int m, const Sph& ic, V3* bary) {
   return area;
 }
 
+// Geometry of one candidate pair (contact.rs:202-218).
+struct PairGeom {
+  Sph a, b, ic;
+  V3 axis;    // unit a -> b
+  V3 ic_dir;  // unit a -> intersection-circle centre
+};
+
+// contact.rs:202-218: false when the pair cannot have a contact.
+VLTR_HD bool pair_setup(const Sph& a, const Sph& b, PairGeom* g) {
+  if (!intersects(a, b) || contains(a, b) || contains(b, a)) return false;
+  g->a = a;
+  g->b = b;
+  g->ic = intersection_circle(a, b);
+  g->axis = unit(b.c - a.c);
+  if (g->ic.r <= 0.0) return false;
+  g->ic_dir = unit(g->ic.c - a.c);
+  return true;
+}
+
+// Radical plane of a and neighbor c: a point, the unit normal a -> c, and the
+// unit direction a -> that point (contact.rs:246-250). Depends on (a, c) only.
+struct Plane {
+  V3 pc, pn, pdir;
+};
+VLTR_HD Plane plane_of(const Sph& a, const Sph& c) {
+  Plane p;
+  p.pc = circle_center(a, c);
+  p.pn = unit(c.c - a.c);
+  p.pdir = unit(p.pc - a.c);
+  return p;
+}
+
+// What neighbor c does to the contact of a pair (contact.rs:224-276). The
+// answer depends on (a, b, c) only, never on the contour built so far: the
+// pair has no contact as soon as one neighbor kills it, and otherwise its
+// contour is cut by the kCut neighbors in sorted order.
+enum NbClass : int { kSkip = 0, kCut = 1, kKill = 2 };
+VLTR_HD int classify(const PairGeom& g, const Sph& c, const Plane& pl, bool c_contains_a) {
+  if (!intersects(g.ic, c) || !intersects(g.b, c)) return kSkip;
+  if (c_contains_a || contains(c, g.b)) return kKill;
+  const double cosv = dot(g.ic_dir, pl.pdir);
+  if (fabs(cosv) >= 1.0) return halfspace(pl.pc, pl.pn, g.ic.c) > 0 ? kKill : kSkip;
+  const double l = fabs(dot(pl.pn, g.ic.c - pl.pc));
+  const double xl = l / sqrt(fma(cosv, -cosv, 1.0));
+  if (xl >= g.ic.r) return halfspace(pl.pc, pl.pn, g.ic.c) >= 0 ? kKill : kSkip;
+  return kCut;
+}
+
+// One cutting neighbor (contact.rs:278-297): the first one lays the hexagon
+// of circumradius 1.19 r around the circle (contact.rs:455-469), later ones
+// only cut when some contour point lies beyond their plane point. Returns
+// kContact to go on, kNone when the contour vanished, kOverflow.
+template <class S>
+VLTR_HD int apply_cut(S& ct, int& n, bool& initialized, const PairGeom& g, V3 pc, V3 pn, int32_t a_id, int32_t cid,
+                      const double* hex_s, const double* hex_c) {
+  if (!initialized) {
+    const V3 first = (any_normal(g.axis) * g.ic.r) * kHexagonScale;
+    ct.set(0, g.ic.c + first, a_id, a_id);
+    for (int h = 0; h < 5; ++h) ct.set(h + 1, g.ic.c + rotate(g.axis, hex_s[h], hex_c[h], first), a_id, a_id);
+    n = 6;
+    initialized = true;
+  } else if (!cuttable(ct, n, g.a.c, pc)) {
+    return kContact;
+  }
+  if (mark_and_cut(ct, n, pc, pn, unit(pn), cid) == kOverflow) return kOverflow;
+  return n == 0 ? kNone : kContact;
+}
+
+// contact.rs:303-340: uncut full circle, or restriction to the circle + area.
+template <class S, class A>
+VLTR_HD int finish_clip(S& ct, int n, bool initialized, const PairGeom& g, int32_t a_id, A& out, ClipOut* co) {
+  co->ic = g.ic;
+  co->axis = g.axis;
+  co->m = 0;
+  co->sum_arc = 0.0;
+  co->area = 0.0;
+  co->bary = V3{0.0, 0.0, 0.0};
+  if (!initialized) {
+    co->bary = g.ic.c;
+    co->sum_arc = kTau;
+    co->area = g.ic.r * g.ic.r * kPi;
+  } else {
+    int m = 0;
+    double s = 0.0;
+    if (restrict_to_circle(ct, n, out, m, g.ic, g.axis, a_id, &s) == kOverflow) return kOverflow;
+    co->sum_arc = s;
+    co->m = m;
+    if (m > 0) co->area = contour_area(out, m, g.ic, &co->bary);
+  }
+  return co->area <= 0.0 ? kNone : kContact;
+}
+
 // contact.rs:196-341 (clip_contact). `nbl` is a's sorted neighbor list.
 template <class S, class A>
 VLTR_HD int clip(const double* sph4, int32_t a_id, int32_t b_id, const int32_t* nbl, int nbn, const double* hex_s,
                  const double* hex_c, S& ct, A& out, ClipOut* co) {
-  const Sph a = load_sph(sph4, a_id);
-  const Sph b = load_sph(sph4, b_id);
-  if (!intersects(a, b) || contains(a, b) || contains(b, a)) return kNone;
-  const Sph ic = intersection_circle(a, b);
-  const V3 axis = unit(b.c - a.c);
-  if (ic.r <= 0.0) return kNone;
-  const V3 ic_dir = unit(ic.c - a.c);
+  PairGeom g;
+  if (!pair_setup(load_sph(sph4, a_id), load_sph(sph4, b_id), &g)) return kNone;
   int n = 0;
   bool initialized = false;
   for (int k = 0; k < nbn; ++k) {
     const int32_t cid = nbl[k];
     if (cid == b_id) continue;
     const Sph c = load_sph(sph4, cid);
-    if (!intersects(ic, c) || !intersects(b, c)) continue;
-    if (contains(c, a) || contains(c, b)) return kNone;
-    const V3 pc = circle_center(a, c);
-    const V3 pn = unit(c.c - a.c);
-    const double cosv = dot(ic_dir, unit(pc - a.c));
-    if (fabs(cosv) >= 1.0) {
-      if (halfspace(pc, pn, ic.c) > 0) return kNone;
-      continue;
-    }
-    const double l = fabs(dot(pn, ic.c - pc));
-    const double xl = l / sqrt(fma(cosv, -cosv, 1.0));
-    if (xl >= ic.r) {
-      if (halfspace(pc, pn, ic.c) >= 0) return kNone;
-      continue;
-    }
-    if (!initialized) {
-      // contact.rs:455-469: hexagon of circumradius 1.19 r around the circle
-      const V3 first = (any_normal(axis) * ic.r) * kHexagonScale;
-      ct.set(0, ic.c + first, a_id, a_id);
-      for (int h = 0; h < 5; ++h) ct.set(h + 1, ic.c + rotate(axis, hex_s[h], hex_c[h], first), a_id, a_id);
-      n = 6;
-      initialized = true;
-    } else if (!cuttable(ct, n, a.c, pc)) {
-      continue;
-    }
-    if (mark_and_cut(ct, n, pc, pn, unit(pn), cid) == kOverflow) return kOverflow;
-    if (n == 0) return kNone;
+    if (!intersects(g.ic, c) || !intersects(g.b, c)) continue;
+    const Plane pl = plane_of(g.a, c);
+    const int cls = classify(g, c, pl, contains(c, g.a));
+    if (cls == kKill) return kNone;
+    if (cls == kSkip) continue;
+    const int st = apply_cut(ct, n, initialized, g, pl.pc, pl.pn, a_id, cid, hex_s, hex_c);
+    if (st != kContact) return st;
   }
-  co->ic = ic;
-  co->axis = axis;
-  co->m = 0;
-  co->sum_arc = 0.0;
-  co->area = 0.0;
-  co->bary = V3{0.0, 0.0, 0.0};
-  if (!initialized) {
-    co->bary = ic.c;
-    co->sum_arc = kTau;
-    co->area = ic.r * ic.r * kPi;
-  } else {
-    int m = 0;
-    double s = 0.0;
-    if (restrict_to_circle(ct, n, out, m, ic, axis, a_id, &s) == kOverflow) return kOverflow;
-    co->sum_arc = s;
-    co->m = m;
-    if (m > 0) co->area = contour_area(out, m, ic, &co->bary);
+  return finish_clip(ct, n, initialized, g, a_id, out, co);
+}
+
+// Clip driven by a precomputed set of cutting neighbors: bit k of cut_mask
+// (128 bits, slot order) marks slot k of a's list as a kCut neighbor; the
+// pair is known to have no kKill neighbor.
+template <class S, class A>
+VLTR_HD int clip_masked(const double* sph4, int32_t a_id, int32_t b_id, const int32_t* nbl, uint64_t mlo, uint64_t mhi,
+                        const double* hex_s, const double* hex_c, S& ct, A& out, ClipOut* co) {
+  PairGeom g;
+  if (!pair_setup(load_sph(sph4, a_id), load_sph(sph4, b_id), &g)) return kNone;
+  int n = 0;
+  bool initialized = false;
+  for (int half = 0; half < 2; ++half) {
+    uint64_t m = half == 0 ? mlo : mhi;
+    while (m) {
+      const int k = half * 64 + ctz64(m);
+      m &= m - 1;
+      const int32_t cid = nbl[k];
+      const Sph c = load_sph(sph4, cid);
+      const V3 pc = circle_center(g.a, c);
+      const V3 pn = unit(c.c - g.a.c);
+      const int st = apply_cut(ct, n, initialized, g, pc, pn, a_id, cid, hex_s, hex_c);
+      if (st != kContact) return st;
+    }
   }
-  return co->area <= 0.0 ? kNone : kContact;
+  return finish_clip(ct, n, initialized, g, a_id, out, co);
 }
 
 // contact.rs:411-444
@@ -450,17 +525,20 @@ VLTR_HD double solid_angle(const Sph& a, const Sph& b, const Sph& ic, const A& c
   return solid;
 }
 
-// contact.rs:347-404 (descriptor summary / area-only) for one pair.
-template <class S, class A>
-VLTR_HD PairRec make_record(const double* sph4, int32_t a_id, int32_t b_id, const int32_t* nbl, int nbn, int mode,
-                            const double* hex_s, const double* hex_c, S& ct, A& out, int* status) {
+// Record of a pair without contact.
+VLTR_HD PairRec empty_record() {
   PairRec rec;
   rec.area = rec.arc_length = rec.solid_a = rec.solid_b = rec.pyr_a = rec.pyr_b = 0.0;
   rec.flags = 0u;
   rec.pad = 0u;
-  ClipOut co;
-  const int st = clip(sph4, a_id, b_id, nbl, nbn, hex_s, hex_c, ct, out, &co);
-  *status = st;
+  return rec;
+}
+
+// contact.rs:347-404: record of a clipped pair (descriptor summary, or area,
+// arc and centrality only when mode == 1).
+template <class A>
+VLTR_HD PairRec record_of(int st, const ClipOut& co, const Sph& a, const Sph& b, const A& out, int mode) {
+  PairRec rec = empty_record();
   if (st != kContact) {
     if (st == kOverflow) rec.flags = kRecOverflow;
     return rec;
@@ -470,8 +548,6 @@ VLTR_HD PairRec make_record(const double* sph4, int32_t a_id, int32_t b_id, cons
   if (is_central(co.ic.c, out, co.m, co.bary)) rec.flags |= kRecCentral;
   rec.flags |= kRecValid;
   if (mode == 0) {
-    const Sph a = load_sph(sph4, a_id);
-    const Sph b = load_sph(sph4, b_id);
     rec.solid_a = solid_angle(a, b, co.ic, out, co.m);
     rec.solid_b = solid_angle(b, a, co.ic, out, co.m);
     const double sa = rec.solid_a < 0.0 ? -1.0 : 1.0;
@@ -480,6 +556,41 @@ VLTR_HD PairRec make_record(const double* sph4, int32_t a_id, int32_t b_id, cons
     rec.pyr_b = norm(co.ic.c - b.c) * co.area / 3.0 * sb;
   }
   return rec;
+}
+
+template <class S, class A>
+VLTR_HD PairRec make_record(const double* sph4, int32_t a_id, int32_t b_id, const int32_t* nbl, int nbn, int mode,
+                            const double* hex_s, const double* hex_c, S& ct, A& out, int* status) {
+  ClipOut co;
+  const int st = clip(sph4, a_id, b_id, nbl, nbn, hex_s, hex_c, ct, out, &co);
+  *status = st;
+  return record_of(st, co, load_sph(sph4, a_id), load_sph(sph4, b_id), out, mode);
+}
+
+template <class S, class A>
+VLTR_HD PairRec make_record_masked(const double* sph4, int32_t a_id, int32_t b_id, const int32_t* nbl, uint64_t mlo,
+                                   uint64_t mhi, int mode, const double* hex_s, const double* hex_c, S& ct, A& out,
+                                   int* status) {
+  ClipOut co;
+  const int st = clip_masked(sph4, a_id, b_id, nbl, mlo, mhi, hex_s, hex_c, ct, out, &co);
+  *status = st;
+  return record_of(st, co, load_sph(sph4, a_id), load_sph(sph4, b_id), out, mode);
+}
+
+// Record of an uncut pair (no kCut and no kKill neighbor): the full circle.
+VLTR_HD PairRec record_uncut(const PairGeom& g, int mode) {
+  ClipOut co;
+  co.ic = g.ic;
+  co.axis = g.axis;
+  co.m = 0;
+  co.bary = g.ic.c;
+  co.sum_arc = kTau;
+  co.area = g.ic.r * g.ic.r * kPi;
+  struct NoContour {
+    double ang[1];
+    VLTR_HD V3 p(int) const { return V3{0.0, 0.0, 0.0}; }
+  } none{{0.0}};
+  return record_of(co.area <= 0.0 ? kNone : kContact, co, g.a, g.b, none, mode);
 }
 
 }  // namespace vltr

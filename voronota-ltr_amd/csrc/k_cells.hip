@@ -13,16 +13,6 @@
 namespace vltr {
 namespace {
 
-__device__ __forceinline__ bool same_group(const Pipeline& p, int64_t a, int64_t b) {
-  if (p.groups != nullptr && a < p.n_groups && b < p.n_groups) return p.groups[a] == p.groups[b];
-  return false;
-}
-__device__ __forceinline__ bool pair_qualifies(const Pipeline& p, int64_t a, int64_t c) {
-  const int64_t cc = p.periodic ? c % p.n : c;
-  if (same_group(p, a, cc)) return false;
-  return (p.periodic && c >= p.n) || a < c;
-}
-
 struct Acc {
   double area = 0.0, arc = 0.0, sa_pos = 0.0, sa_neg = 0.0, pv_pos = 0.0, pv_neg = 0.0;
   int added = 0;
@@ -41,39 +31,57 @@ struct Acc {
   }
 };
 
-__global__ void k_cells(Pipeline p) {
-  const int64_t b = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (b >= p.n) return;
+constexpr int kWave = 64;
+constexpr int kCellsBlock = 256;
+constexpr int kCellsWaves = kCellsBlock / kWave;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront per ball b. Its contributions arrive in the reference's order:
+// first the pairs (a, b) with a < b by ascending pair index (= ascending a),
+// then b's own pairs. Lanes fetch records in parallel into a 64-entry LDS
+// window in that order; lane 0 sums the window sequentially.
+__global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
+  __shared__ PairRec win[kCellsWaves][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const int64_t b = xcd_remap(blockIdx.x, gridDim.x) * kCellsWaves + w;
+  if (b >= p.n) return;  // wave-private LDS, no block barrier below
   const int cap = p.nb_cap;
-  const int32_t* lb = p.nb + b * static_cast<int64_t>(cap);
   const int nb = p.nb_cnt[b];
+  const int nin = min(static_cast<int>(p.in_cnt[b]), cap);
+  const int32_t* inc = p.incoming + b * static_cast<int64_t>(cap);
   Acc acc;
-  // b as second end: pairs (a, b), a < b canonical, ascending a.
-  int64_t prev = -1;
-  for (;;) {
-    int64_t best = INT64_MAX;
-    for (int k = 0; k < nb; ++k) {
-      const int64_t c = lb[k];
-      if (c < b && c > prev && c < best && pair_qualifies(p, c, b)) best = c;
+  for (int c0 = 0; c0 < nin; c0 += kWave) {
+    for (int e = lane; e < nin; e += kWave) {
+      const int32_t pe = inc[e];
+      int rank = 0;
+      for (int k = 0; k < nin; ++k) rank += inc[k] < pe ? 1 : 0;
+      if (rank >= c0 && rank < c0 + kWave) win[w][rank - c0] = p.rec[pe];
     }
-    if (best == INT64_MAX) break;
-    const int32_t* la = p.nb + best * static_cast<int64_t>(cap);
-    const int na = p.nb_cnt[best];
-    int64_t idx = p.pair_cnt[best];
-    for (int k = 0; k < na; ++k) {
-      const int64_t c = la[k];
-      if (!pair_qualifies(p, best, c)) continue;
-      if (c == b) {
-        acc.add(p.rec[idx], false);
-        break;
-      }
-      ++idx;
+    wave_sync();
+    if (lane == 0) {
+      const int m = min(kWave, nin - c0);
+      for (int k = 0; k < m; ++k) acc.add(win[w][k], false);
     }
-    prev = best;
+    wave_sync();
   }
-  // b as first end: its own pairs in neighbor order.
-  const int64_t p0 = p.pair_cnt[b], p1 = p.pair_cnt[b + 1];
-  for (int64_t t = p0; t < p1; ++t) acc.add(p.rec[t], true);
+  const int64_t own0 = p.pair_cnt[b];
+  const int nown = static_cast<int>(static_cast<int64_t>(p.pair_cnt[b + 1]) - own0);
+  for (int c0 = 0; c0 < nown; c0 += kWave) {
+    if (c0 + lane < nown) win[w][lane] = p.rec[own0 + c0 + lane];
+    wave_sync();
+    if (lane == 0) {
+      const int m = min(kWave, nown - c0);
+      for (int k = 0; k < m; ++k) acc.add(win[w][k], true);
+    }
+    wave_sync();
+  }
+  if (lane != 0) return;
 
   CellRec out;
   out.sas = 0.0;
@@ -160,7 +168,7 @@ inline int64_t blocks_for(int64_t n, int64_t per) { return (n + per - 1) / per; 
 }  // namespace
 
 hipError_t launch_cells(const Pipeline& p, hipStream_t s) {
-  if (p.n > 0) hipLaunchKernelGGL(k_cells, dim3(blocks_for(p.n, 128)), dim3(128), 0, s, p);
+  if (p.n > 0) hipLaunchKernelGGL(k_cells, dim3(blocks_for(p.n, kCellsWaves)), dim3(kCellsBlock), 0, s, p);
   return hipGetLastError();
 }
 

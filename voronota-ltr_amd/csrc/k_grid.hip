@@ -131,20 +131,6 @@ __global__ void k_scatter(Pipeline p) {
   p.sorted_sph[pos] = p.sph[i];
 }
 
-__device__ __forceinline__ bool same_group(const Pipeline& p, int64_t a, int64_t b) {
-  if (p.groups != nullptr && a < p.n_groups && b < p.n_groups) return p.groups[a] == p.groups[b];
-  return false;
-}
-
-// tessellation.rs:298-319 (and updateable.rs:516-561 without involvement):
-// (a, b) is a candidate pair when not in the same group and a < b, or b is a
-// periodic image.
-__device__ __forceinline__ bool pair_qualifies(const Pipeline& p, int64_t a, int64_t c) {
-  const int64_t cc = p.periodic ? c % p.n : c;
-  if (same_group(p, a, cc)) return false;
-  return (p.periodic && c >= p.n) || a < c;
-}
-
 struct NbEntry {
   double key;
   int32_t id;
@@ -260,7 +246,9 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
   }
 }
 
-// Pair list in reference order: by first ball, then neighbor order.
+// Pair list in reference order: by first ball, then neighbor order. Each pair
+// (a, c) whose second ball is canonical is also appended to c's incoming list
+// (tessellation.rs:624-631: an image end receives nothing).
 __global__ void k_pair_fill(Pipeline p) {
   const int64_t a = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (a >= p.n) return;
@@ -274,6 +262,13 @@ __global__ void k_pair_fill(Pipeline p) {
     if (base + j < p.pair_cap) {
       p.pair_a[base + j] = static_cast<int32_t>(a);
       p.pair_b[base + j] = c;
+      if (c < p.n) {
+        const uint32_t slot = atomicAdd(&p.in_cnt[c], 1u);
+        if (slot < static_cast<uint32_t>(p.nb_cap))
+          p.incoming[static_cast<int64_t>(c) * p.nb_cap + slot] = static_cast<int32_t>(base + j);
+        else
+          atomicOr(p.status, kStatNbOverflow);
+      }
     } else {
       atomicOr(p.status, kStatPairOverflow);
     }
@@ -393,7 +388,11 @@ hipError_t launch_neighbors(const Pipeline& p, hipStream_t s) {
 }
 
 hipError_t launch_pairs(const Pipeline& p, hipStream_t s) {
-  if (p.n > 0) hipLaunchKernelGGL(k_pair_fill, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p);
+  if (p.n > 0) {
+    const int64_t fb = blocks_for(p.n, 256) < 4096 ? blocks_for(p.n, 256) : 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, s, p.in_cnt, p.n, 0u);
+    hipLaunchKernelGGL(k_pair_fill, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p);
+  }
   return hipGetLastError();
 }
 

@@ -89,7 +89,7 @@ struct HostPinned {
   }
 };
 
-constexpr int kPhases = 6;
+constexpr int kPhases = 7;
 
 }  // namespace
 
@@ -101,7 +101,8 @@ struct vltr_context {
   int nb_cap = 96;
   // device buffers
   DevBuf balls, groups, sph, seg_of, segs, bkey, brank, bcnt, sorted_id, sorted_sph, small, nb, nb_cnt, excluded,
-      pair_cnt, pair_a, pair_b, rec, cell, overflow_list, flags, scan_tmp, out_contacts, out_cells, slow_ws;
+      pair_cnt, pair_a, pair_b, rec, cell, overflow_list, flags, scan_tmp, out_contacts, out_cells, slow_ws, alive, alive_all,
+      in_cnt, incoming;
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
@@ -177,6 +178,11 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.cell = ctx->cell.as<CellRec>();
   p.overflow_list = ctx->overflow_list.as<int32_t>();
   p.slow_ws = ctx->slow_ws.as<double>();
+  p.alive = ctx->alive.as<AliveRec>();
+  p.alive_all = ctx->alive_all.as<AliveRec>();
+  p.alive_flag = ctx->flags.as<uint32_t>();
+  p.in_cnt = ctx->in_cnt.as<uint32_t>();
+  p.incoming = ctx->incoming.as<int32_t>();
 }
 
 // Runs the device pipeline on ctx->balls (already on the device). On return
@@ -222,15 +228,18 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->excluded.ensure(n));
   VLTR_HIP(ctx->pair_cnt.ensure((n + 1) * sizeof(uint32_t)));
   VLTR_HIP(ctx->cell.ensure(n * sizeof(CellRec)));
+  VLTR_HIP(ctx->in_cnt.ensure(n * sizeof(uint32_t)));
   VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(p.n_buckets, 1) + n + 64) * sizeof(uint32_t) * 2));
   VLTR_HIP(ctx->pinned.ensure(64));
   VLTR_HIP(hipMemcpyAsync(ctx->segs.ptr, &seg, sizeof(seg), hipMemcpyHostToDevice, s));
 
   uint32_t* host_words = static_cast<uint32_t*>(ctx->pinned.ptr);
+ restart:
   VLTR_HIP(hipEventRecord(ctx->ev[0], s));
   for (;;) {
     p.nb_cap = ctx->nb_cap;
     VLTR_HIP(ctx->nb.ensure(static_cast<size_t>(n) * p.nb_cap * sizeof(int32_t)));
+    VLTR_HIP(ctx->incoming.ensure(static_cast<size_t>(n) * p.nb_cap * sizeof(int32_t)));
     fill_pointers(ctx, p);
     VLTR_HIP(launch_prepare(p, s));
     VLTR_HIP(launch_grid(p, s));
@@ -256,6 +265,8 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->pair_b.ensure(p.pair_cap * sizeof(int32_t)));
   VLTR_HIP(ctx->rec.ensure(p.pair_cap * sizeof(PairRec)));
   VLTR_HIP(ctx->overflow_list.ensure(p.pair_cap * sizeof(int32_t)));
+  VLTR_HIP(ctx->alive.ensure(p.pair_cap * sizeof(AliveRec)));
+  VLTR_HIP(ctx->alive_all.ensure(p.pair_cap * sizeof(AliveRec)));
   VLTR_HIP(ctx->flags.ensure((std::max<int64_t>(P, n) + 2) * sizeof(uint32_t)));
   VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(P, n) + 64) * sizeof(uint32_t) * 2));
   VLTR_HIP(ctx->out_contacts.ensure(p.pair_cap * sizeof(vltr_contact)));
@@ -263,7 +274,9 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   fill_pointers(ctx, p);
   VLTR_HIP(launch_pairs(p, s));
   VLTR_HIP(hipEventRecord(ctx->ev[3], s));
-  VLTR_HIP(launch_cut(p, s));
+  VLTR_HIP(launch_prefilter(p, ctx->scan_tmp.as<uint32_t>(), s));
+  VLTR_HIP(hipEventRecord(ctx->ev[4], s));
+  VLTR_HIP(launch_cut_alive(p, s));
   VLTR_HIP(hipMemcpyAsync(host_words + 2, p.overflow_cnt, 4, hipMemcpyDeviceToHost, s));
   VLTR_HIP(hipStreamSynchronize(s));
   const int64_t n_over = host_words[2];
@@ -274,7 +287,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     p.slow_ws = ctx->slow_ws.as<double>();
     VLTR_HIP(launch_cut_slow(p, n_over, s));
   }
-  VLTR_HIP(hipEventRecord(ctx->ev[4], s));
+  VLTR_HIP(hipEventRecord(ctx->ev[5], s));
   if (mode == 0) VLTR_HIP(launch_cells(p, s));
   uint32_t* flags = ctx->flags.as<uint32_t>();
   VLTR_HIP(launch_flag_contacts(p, flags, s));
@@ -287,8 +300,16 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     VLTR_HIP(launch_emit_cells(p, flags, ctx->out_cells.as<vltr_cell>(), s));
     VLTR_HIP(hipMemcpyAsync(host_words + 4, flags + n, 4, hipMemcpyDeviceToHost, s));
   }
-  VLTR_HIP(hipEventRecord(ctx->ev[5], s));
+  VLTR_HIP(hipMemcpyAsync(host_words + 5, p.status, 4, hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipEventRecord(ctx->ev[6], s));
   VLTR_HIP(hipStreamSynchronize(s));
+  if (host_words[5] & kStatNbOverflow) {
+    // an incoming-pair list outgrew nb_cap (a hidden ball with many
+    // neighbors): redo everything with longer lists
+    if (ctx->nb_cap >= 4096) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
+    ctx->nb_cap *= 2;
+    goto restart;
+  }
   *n_contacts = host_words[3];
   *n_cells = mode == 0 ? host_words[4] : 0;
   ctx->last_n = static_cast<size_t>(n);
@@ -301,10 +322,10 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   ctx->counts[5] = static_cast<uint64_t>(n_over);
   ctx->counts[6] = static_cast<uint64_t>(p.nb_cap);
   float ms = 0.0f;
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < kPhases - 1; ++k) {
     if (hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess) ctx->timings[k] = ms;
   }
-  if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[5]) == hipSuccess) ctx->timings[5] = ms;
+  if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[kPhases - 1]) == hipSuccess) ctx->timings[kPhases - 1] = ms;
   return VLTR_OK;
 }
 
@@ -461,7 +482,8 @@ void vltr_context_destroy(vltr_context* ctx) {
   for (DevBuf* b : {&ctx->balls, &ctx->groups, &ctx->sph, &ctx->seg_of, &ctx->segs, &ctx->bkey, &ctx->brank, &ctx->bcnt,
                     &ctx->sorted_id, &ctx->sorted_sph, &ctx->small, &ctx->nb, &ctx->nb_cnt, &ctx->excluded,
                     &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec, &ctx->cell, &ctx->overflow_list,
-                    &ctx->flags, &ctx->scan_tmp, &ctx->out_contacts, &ctx->out_cells, &ctx->slow_ws})
+                    &ctx->flags, &ctx->scan_tmp, &ctx->out_contacts, &ctx->out_cells, &ctx->slow_ws, &ctx->alive, &ctx->alive_all,
+                    &ctx->in_cnt, &ctx->incoming})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)

@@ -196,6 +196,8 @@ VLTR_HD double min_dihedral(V3 o, V3 a, V3 b1, V3 b2) {
   return acos(clamp1(dot(unit(d1), unit(d2))));
 }
 
+VLTR_HD int ctz64(uint64_t x) { return __builtin_ctzll(x); }
+
 // Workgroups are dealt to the 8 XCDs of an MI355X round-robin (block b runs
 // on XCD b % 8). xcd_remap turns the hardware block index into a logical one
 // so that each XCD walks one contiguous range of logical blocks (neighboring

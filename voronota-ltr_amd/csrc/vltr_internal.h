@@ -79,7 +79,38 @@ struct Pipeline {
   uint32_t* overflow_cnt;  // 1
   double* slow_ws;         // workspace of the slow kernel
   int64_t slow_ws_pairs;   // pairs the workspace holds at once
+
+  struct AliveRec* alive_all;  // pair_cap, by pair index (prefilter output)
+  uint32_t* alive_flag;        // pair_cap + 1: 1 where alive_all is live; scanned
+                               // in place to positions, [n_pairs] = count
+  struct AliveRec* alive;      // pair_cap: the live entries, in pair order
+  uint32_t* in_cnt;        // n: pairs (a, b), a < b, received by ball b
+  int32_t* incoming;       // n * nb_cap: their pair indices (any order)
 };
+
+// A pair that survived the prefilter: bit k of the 128-bit mask marks slot k
+// of the first ball's neighbor list as a cutting neighbor. kAliveFullScan:
+// the list is longer than the mask, so the cut kernel scans it itself.
+struct AliveRec {
+  int32_t pair;
+  uint32_t flags;
+  uint64_t mlo, mhi;
+};
+constexpr uint32_t kAliveFullScan = 1u;
+
+// tessellation.rs:323-328
+__device__ __forceinline__ bool same_group(const Pipeline& p, int64_t a, int64_t b) {
+  if (p.groups != nullptr && a < p.n_groups && b < p.n_groups) return p.groups[a] == p.groups[b];
+  return false;
+}
+// tessellation.rs:298-319 (and updateable.rs:516-561 without involvement):
+// (a, c) is a candidate pair when not in the same group and a < c, or c is a
+// periodic image.
+__device__ __forceinline__ bool pair_qualifies(const Pipeline& p, int64_t a, int64_t c) {
+  const int64_t cc = p.periodic ? c % p.n : c;
+  if (same_group(p, a, cc)) return false;
+  return (p.periodic && c >= p.n) || a < c;
+}
 
 // Host-side launchers (k_grid.hip).
 hipError_t launch_prepare(const Pipeline& p, hipStream_t s);
@@ -92,6 +123,8 @@ int64_t scan_tmp_words(int64_t n);
 
 // k_contact.hip
 hipError_t launch_cut(const Pipeline& p, hipStream_t s);
+hipError_t launch_prefilter(const Pipeline& p, uint32_t* scan_tmp, hipStream_t s);
+hipError_t launch_cut_alive(const Pipeline& p, hipStream_t s);
 hipError_t launch_cut_slow(const Pipeline& p, int64_t n_overflow, hipStream_t s);
 void set_hexagon_constants(const double* s_half5, const double* c_half5);
 hipError_t upload_hexagon_constants(hipStream_t s);

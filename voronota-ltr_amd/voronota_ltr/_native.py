@@ -126,9 +126,9 @@ def context(device: int = 0) -> int:
 
 def last_timings(device: int = 0):
     lib = load_library()
-    buf = (C.c_double * 6)()
-    k = lib.vltr_context_last_timings(context(device), buf, 6)
-    names = ["grid", "neighbors", "pairs", "cut", "cells", "total"]
+    buf = (C.c_double * 7)()
+    k = lib.vltr_context_last_timings(context(device), buf, 7)
+    names = ["grid", "neighbors", "pairs", "prefilter", "cut", "cells", "total"]
     return {names[i]: buf[i] for i in range(k)}
 
 

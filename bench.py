@@ -7,8 +7,9 @@ sequential addition in a sphere at 0.056 balls/A^3, 2.0 A minimum centre
 distance, radii from {1.52, 1.55, 1.70, 1.80} A, probe 1.4, f64.
 
 The JSON line carries the throughput, the roofline of the dominant kernel
-(k_cut: algorithmic FP64 work = FLOP per candidate pair x pairs per launch,
-over its HIP-event time on the library stream) and the CPU baseline (the
+(k_clip, which classifies and clips every candidate pair: algorithmic FP64
+work = FLOP per candidate pair x pairs per launch, over its HIP-event time on
+the library stream) and the CPU baseline (the
 oracle's C++ restatement of the reference on this host's cores, on the same
 input).
 """
@@ -63,7 +64,7 @@ def main():
         rc = lib.vltr_compute_tessellation_device(ctx, dbuf.ptr, args.n, args.probe, None, 0, contacts_only)
         _native.check(rc, "vltr_compute_tessellation_device")
 
-    phases = ["grid", "neighbors", "pairs", "prefilter", "cut", "cells", "total"]
+    phases = ["grid", "neighbors", "pairs", "clip", "slow", "cells", "total"]
 
     def timed(contacts_only):
         for _ in range(args.warmup):
@@ -82,7 +83,7 @@ def main():
     ms_co, ph_co, counts_co = timed(1)
 
     value = args.n / (ms_full / 1e3)
-    cut_ms = ph_full["cut"]
+    cut_ms = ph_full["clip"]
     flops = FLOP_PER_PAIR * counts["pairs"]
     achieved = flops / (cut_ms / 1e3) / 1e12 if cut_ms > 0 else 0.0
 
@@ -121,7 +122,7 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "k_cut",
+            "kernel": "k_clip",
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",

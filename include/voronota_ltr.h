@@ -90,14 +90,17 @@ const char* vltr_last_error(void);
 
 /* Per-phase device time of the last pipeline run on ctx, in milliseconds,
  * measured with HIP events on the context stream. Order: prepare+grid,
- * neighbors, pairs, prefilter, cut, cells+compaction, total. Returns the count
- * written. */
+ * neighbors, pairs, clip (classification and contour clipping of every
+ * candidate pair), slow (the pairs of balls with more than 64 neighbors and
+ * contours beyond 16 points),
+ * cells+compaction, total. Returns the count written. */
 int vltr_context_last_timings(const vltr_context* ctx, double* ms, int max_entries);
 
 /* Work counts of the last pipeline run on ctx: out[0] balls, [1] searched
  * spheres (27 n when periodic), [2] candidate pairs clipped, [3] contacts,
- * [4] cells, [5] pairs redone by the large-contour kernel, [6] neighbor
- * slots per ball. Returns the count written. */
+ * [4] cells, [5] pairs clipped outside the main clip kernel (first ball with
+ * more than 64 neighbors, or contour beyond 16 points), [6] neighbor slots
+ * per ball. Returns the count written. */
 int vltr_context_last_counts(const vltr_context* ctx, uint64_t* out, int max_entries);
 
 /* ------------------------------------------------------- periodic box --- */

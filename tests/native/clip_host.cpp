@@ -2,11 +2,15 @@
 // contour.h), so CPU tests can check it pair by pair against the oracle.
 // Test infrastructure: compiled by __graft_entry__.build() into
 // tests/native/libclip_host.so; the product library does not use it.
+#include <ucontext.h>
+
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "contour.h"
+#include "group_clip.h"
 
 namespace {
 void hexagon(double* s, double* c) {
@@ -16,6 +20,119 @@ void hexagon(double* s, double* c) {
     c[k] = std::cos(rot / 2.0);
     rot += vltr::kFracPi3;
   }
+}
+// The 16-lane group of group_clip.h on the host: one fiber per lane, run in
+// lock step. Every collective (ballot, shfl, sync) stores the lane's value
+// and yields; the scheduler resumes lanes 0..15 in turn, so when a lane
+// resumes, all 16 have contributed. Values alternate between two slot banks,
+// so a lane can write the next collective before the others read this one.
+struct FiberGroup {
+  static constexpr int W = vltr::kGroup;
+  ucontext_t sched;
+  ucontext_t fib[W];
+  std::vector<char> stack[W];
+  uint64_t slot[2][W];
+  int round[W];
+  bool done[W];
+  FiberGroup() {
+    for (int k = 0; k < W; ++k) stack[k].resize(1 << 18);
+  }
+};
+
+struct HostLane {
+  FiberGroup* f;
+  int ln;
+  int lane() const { return ln; }
+  void yield() { swapcontext(&f->fib[ln], &f->sched); }
+  uint64_t exchange(uint64_t v, int src) {
+    const int b = f->round[ln]++ & 1;
+    f->slot[b][ln] = v;
+    yield();
+    return f->slot[b][src & (FiberGroup::W - 1)];
+  }
+  uint32_t ballot(bool p) {
+    const int b = f->round[ln]++ & 1;
+    f->slot[b][ln] = p ? 1u : 0u;
+    yield();
+    uint32_t m = 0;
+    for (int k = 0; k < FiberGroup::W; ++k) m |= static_cast<uint32_t>(f->slot[b][k] & 1u) << k;
+    return m;
+  }
+  double shfl(double v, int src) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    u = exchange(u, src);
+    double r;
+    std::memcpy(&r, &u, 8);
+    return r;
+  }
+  int32_t shfl(int32_t v, int src) { return static_cast<int32_t>(exchange(static_cast<uint32_t>(v), src)); }
+  void sync() {
+    f->round[ln]++;
+    yield();
+  }
+};
+
+struct GroupTask {
+  const double* sph4;
+  const int32_t* nbl;
+  vltr::PairGeom g;
+  int32_t a_id;
+  uint64_t mlo, mhi;
+  int mode;
+  const double *hs, *hc;
+  vltr::GroupLds lds;
+  vltr::PairRec rec[FiberGroup::W];
+  int st[FiberGroup::W];
+  FiberGroup* f;
+};
+GroupTask* g_task = nullptr;
+
+void lane_main(int ln) {
+  GroupTask& t = *g_task;
+  HostLane lane{t.f, ln};
+  const vltr::Sph A = t.g.a;
+  auto fetch = [&](int k) {
+    const vltr::Sph C = vltr::load_sph(t.sph4, t.nbl[k]);
+    const vltr::Plane pl = vltr::plane_of(A, C);
+    return vltr::CutPlane{pl.pc, pl.pn, vltr::unit(pl.pn), t.nbl[k]};
+  };
+  t.rec[ln] = vltr::group_clip(lane, &t.lds, t.g, t.a_id, t.mlo, t.mhi, fetch, t.hs, t.hc, t.mode, &t.st[ln]);
+  t.f->done[ln] = true;
+}
+
+// Runs group_clip on 16 fibers; returns lane 0's record, checks all agree.
+bool run_group(GroupTask& t, vltr::PairRec* out, int* st) {
+  static FiberGroup fg;
+  t.f = &fg;
+  g_task = &t;
+  std::memset(&t.lds, 0xff, sizeof(t.lds));
+  for (int k = 0; k < FiberGroup::W; ++k) {
+    fg.round[k] = 0;
+    fg.done[k] = false;
+    getcontext(&fg.fib[k]);
+    fg.fib[k].uc_stack.ss_sp = fg.stack[k].data();
+    fg.fib[k].uc_stack.ss_size = fg.stack[k].size();
+    fg.fib[k].uc_link = &fg.sched;
+    makecontext(&fg.fib[k], reinterpret_cast<void (*)()>(lane_main), 1, k);
+  }
+  for (;;) {
+    int running = 0;
+    for (int k = 0; k < FiberGroup::W; ++k) {
+      if (fg.done[k]) continue;
+      swapcontext(&fg.sched, &fg.fib[k]);
+      ++running;
+    }
+    if (running == 0) break;
+  }
+  bool agree = true;
+  for (int k = 1; k < FiberGroup::W; ++k) {
+    if (t.st[k] != t.st[0] || std::memcmp(&t.rec[k], &t.rec[0], sizeof(vltr::PairRec)) != 0) agree = false;
+    if (fg.round[k] != fg.round[0]) agree = false;
+  }
+  *out = t.rec[0];
+  *st = t.st[0];
+  return agree;
 }
 }  // namespace
 
@@ -84,6 +201,77 @@ int clip_host_records_staged(const double* sph4, const int64_t* nb_off, const in
     out[6 * t + 5] = r.pyr_b;
     flags[t] = (r.flags & 3u) | extra;
   }
+  return 0;
+}
+
+// The fused kernel's path on the host: classification as in the staged path,
+// then the 16-lane group clip (group_clip.h) for pairs with cutting
+// neighbors; lists longer than 64 and group overflows take the one-lane code
+// with the large contour store, as on the device. flags: bit0 valid, bit1
+// central, bit3 decided by classification, bit4 clipped by the group, bit5
+// redone by the one-lane code, bit6 the 16 lanes disagreed (a bug).
+int clip_host_records_group(const double* sph4, const int64_t* nb_off, const int32_t* nb_idx, const int32_t* pairs,
+                            int64_t P, int mode, double* out, uint32_t* flags) {
+  double hs[5], hc[5];
+  hexagon(hs, hc);
+  std::vector<double> ws(5 * 1024 + 6 * 2048);
+  double* w2 = ws.data() + 5 * 1024;
+  vltr::GlobalCut gc{1024, ws.data(), ws.data() + 1024, ws.data() + 2048,
+                     reinterpret_cast<int32_t*>(ws.data() + 3072), reinterpret_cast<int32_t*>(ws.data() + 4096)};
+  vltr::GlobalArc ga{2048, w2, w2 + 2048, w2 + 4096, w2 + 6144, reinterpret_cast<int32_t*>(w2 + 8192),
+                     reinterpret_cast<int32_t*>(w2 + 10240)};
+  GroupTask* task = new GroupTask;
+  for (int64_t t = 0; t < P; ++t) {
+    const int32_t a = pairs[2 * t], b = pairs[2 * t + 1];
+    const int32_t* nbl = nb_idx + nb_off[a];
+    const int nbn = static_cast<int>(nb_off[a + 1] - nb_off[a]);
+    vltr::PairRec r = vltr::empty_record();
+    uint32_t extra = 8u;
+    vltr::PairGeom g;
+    const vltr::Sph A = vltr::load_sph(sph4, a);
+    int st = 0;
+    if (nbn > 64) {
+      r = vltr::make_record(sph4, a, b, nbl, nbn, mode, hs, hc, gc, ga, &st);
+      extra = 32u;
+    } else if (vltr::pair_setup(A, vltr::load_sph(sph4, b), &g)) {
+      bool kill = false;
+      uint64_t m = 0;
+      for (int k = 0; k < nbn; ++k) {
+        if (nbl[k] == b) continue;
+        const vltr::Sph C = vltr::load_sph(sph4, nbl[k]);
+        const int cl = vltr::classify(g, C, vltr::plane_of(A, C), vltr::contains(C, A));
+        if (cl == vltr::kKill) kill = true;
+        if (cl == vltr::kCut) m |= 1ull << k;
+      }
+      if (!kill && m == 0) {
+        r = vltr::record_uncut(g, mode);
+      } else if (!kill) {
+        task->sph4 = sph4;
+        task->nbl = nbl;
+        task->g = g;
+        task->a_id = a;
+        task->mlo = m;
+        task->mhi = 0;
+        task->mode = mode;
+        task->hs = hs;
+        task->hc = hc;
+        const bool agree = run_group(*task, &r, &st);
+        extra = 16u | (agree ? 0u : 64u);
+        if (st == vltr::kOverflow) {
+          r = vltr::make_record(sph4, a, b, nbl, nbn, mode, hs, hc, gc, ga, &st);
+          extra |= 32u;
+        }
+      }
+    }
+    out[6 * t + 0] = r.area;
+    out[6 * t + 1] = r.arc_length;
+    out[6 * t + 2] = r.solid_a;
+    out[6 * t + 3] = r.solid_b;
+    out[6 * t + 4] = r.pyr_a;
+    out[6 * t + 5] = r.pyr_b;
+    flags[t] = (r.flags & 3u) | extra;
+  }
+  delete task;
   return 0;
 }
 

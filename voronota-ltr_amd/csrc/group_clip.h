@@ -1,0 +1,454 @@
+// Contact clipping of one candidate pair by a group of 16 cooperating lanes
+// (same results as clip_masked + record_of in contour.h, which restate
+// src/contact.rs:196-848).
+//
+// Lane j holds point j of the contour while it is cut (at most 16 points);
+// cuts become lane permutations and the two plane/segment intersections of a
+// cut are computed side by side. The contour restricted to the circle (at
+// most 32 points) is compacted into a per-group shared buffer, and every
+// per-point term (arc angles, triangle areas, dihedral turns) is computed by
+// one lane each; sums are then taken in the reference's sequential order, so
+// the record is bit-identical to the one-lane code. Contours that would
+// outgrow the group, and the degenerate cuts that leave fewer than three
+// points, return kOverflow: the caller redoes them with the one-lane code.
+//
+// G is the lane group: lane(), ballot(bool) -> 16-bit mask, shfl(v, src)
+// for double and int32, sync() (shared-buffer writes visible to the group).
+// All control flow that reaches a ballot, shfl or sync is uniform over the
+// group. The kernels implement G with wavefront intrinsics; the CPU tests
+// run the same code on 16 lock-stepped host fibers.
+#pragma once
+
+#include "contour.h"
+
+namespace vltr {
+
+constexpr int kGroup = 16;
+constexpr int kGroupR = 2 * kGroup;
+
+// Restricted contour of one group: points, (left, right) ids, arc angles.
+struct GroupLds {
+  double x[kGroupR], y[kGroupR], z[kGroupR], ang[kGroupR];
+  int32_t l[kGroupR], r[kGroupR];
+};
+
+struct GPt {
+  V3 p;
+  int32_t l, r;
+};
+
+// Radical plane of a and a cutting neighbor (plane_of), plus unit(pn) and the
+// neighbor's id.
+struct CutPlane {
+  V3 pc, pn, nn;
+  int32_t cid;
+};
+
+VLTR_HD int popc32(uint32_t x) { return __builtin_popcount(x); }
+VLTR_HD int ctz32(uint32_t x) { return __builtin_ctz(x); }
+VLTR_HD int clz32(uint32_t x) { return __builtin_clz(x); }
+
+template <class G>
+VLTR_HD V3 gshfl3(G& g, V3 v, int src) {
+  return V3{g.shfl(v.x, src), g.shfl(v.y, src), g.shfl(v.z, src)};
+}
+template <class G>
+VLTR_HD GPt gshfl(G& g, const GPt& v, int src) {
+  return GPt{gshfl3(g, v.p, src), g.shfl(v.l, src), g.shfl(v.r, src)};
+}
+
+// One cut of the contour by a neighbor's radical plane (contact.rs:484-627;
+// one-lane form: mark_and_cut).
+template <class G>
+VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
+  const int j = g.lane();
+  const bool o = j < n && halfspace(cp.pc, cp.pn, c.p) >= 0;
+  if (o) {
+    c.l = cp.cid;
+    c.r = cp.cid;
+  }
+  const uint32_t M = g.ballot(o);
+  const int outs = popc32(M);
+  if (outs == 0) return kContact;
+  if (outs >= n) {
+    n = 0;
+    return kNone;
+  }
+  if (n < 3) return kContact;
+  int start = ctz32(M), end = 31 - clz32(M);
+  if (start == 0 && end == n - 1) {
+    // marked run wraps around point 0: it ends before the first unmarked
+    // point and starts after the last one
+    const uint32_t full = (1u << n) - 1u;
+    end = ctz32(~M) - 1;
+    start = 32 - clz32(~M & full);
+  }
+  int src = j, nn = n;
+  if (start == end) {
+    if (n + 1 > kGroup) return kOverflow;
+    src = j > start ? j - 1 : j;
+    nn = n + 1;
+    end = start + 1;
+  } else if (start < end) {
+    const int d = end - (start + 1);
+    src = j >= start + 1 ? j + d : j;
+    nn = n - d;
+    end = start + 1;
+  } else {
+    nn = start + 1 - end;
+    src = j + end;
+    start = nn - 1;
+    end = 0;
+  }
+  if (nn < 3) return kOverflow;
+  c = gshfl(g, c, src & (kGroup - 1));
+  n = nn;
+  const int il = start > 0 ? start - 1 : n - 1;
+  const int ir = end + 1 < n ? end + 1 : 0;
+  // with n >= 3, il != end and ir != start: both intersections use the
+  // points as they were before either is written
+  const GPt other = gshfl(g, c, j == start ? il : ir);
+  const V3 x = plane_segment(cp.pc, cp.nn, c.p, other.p);
+  if (j == start) {
+    c.r = c.l;
+    c.l = other.r;
+    c.p = x;
+  } else if (j == end) {
+    c.l = c.r;
+    c.r = other.l;
+    c.p = x;
+  }
+  const V3 ps = gshfl3(g, c.p, start);
+  const V3 pe = gshfl3(g, c.p, end);
+  if (!f_gt(norm2(pe - ps), 0.0)) {
+    const int32_t re = g.shfl(c.r, end);
+    if (j == start) c.r = re;
+    c = gshfl(g, c, (j >= end ? j + 1 : j) & (kGroup - 1));
+    --n;
+  }
+  return kContact;
+}
+
+template <class G>
+VLTR_HD void lds_put(GroupLds* s, int k, V3 p, int32_t l, int32_t r) {
+  s->x[k] = p.x;
+  s->y[k] = p.y;
+  s->z[k] = p.z;
+  s->l[k] = l;
+  s->r[k] = r;
+}
+VLTR_HD V3 lds_p(const GroupLds* s, int k) { return V3{s->x[k], s->y[k], s->z[k]}; }
+
+// Restriction to the circle (contact.rs:631-769; one-lane form:
+// restrict_to_circle). Lane j handles segment (j, j+1). On return the
+// restricted contour is s[0..m) with its arc angles; m = 0 means no contour.
+template <class G>
+VLTR_HD int group_restrict(G& g, GroupLds* s, const GPt& c, int n, const Sph& ic, V3 axis, int32_t a_id, int* m_out,
+                           double* sum) {
+  const int j = g.lane();
+  const double rr = ic.r * ic.r;
+  const bool valid = j < n;
+  const bool in = valid && norm2(c.p - ic.c) <= rr;
+  const uint32_t Min = g.ballot(in);
+  const int n_in = popc32(Min);
+  *sum = 0.0;
+  *m_out = 0;
+  if (n_in == n) {
+    if (valid) {
+      lds_put<G>(s, j, c.p, c.l, c.r);
+      s->ang[j] = 0.0;
+    }
+    g.sync();
+    *m_out = n;
+    return kContact;
+  }
+  const int ni = j + 1 < n ? j + 1 : 0;
+  const int pj = j > 0 ? j - 1 : n - 1;
+  const GPt nx = gshfl(g, c, ni & (kGroup - 1));
+  const bool o1 = !in;
+  const bool o2 = !((Min >> (ni & 31)) & 1u);
+  V3 ins1 = c.p, ins2 = c.p;
+  int32_t i1l = 0, i1r = 0, i2l = 0, i2r = 0;
+  int nins = 0;
+  bool mod_next = false, mod_self = false;
+  if (valid && (o1 || o2)) {
+    // o1 && o2: both crossings from the foot of the perpendicular (when it
+    // lies inside the circle); o1 only: from the next point (inside) towards
+    // this one; o2 only: from this point (inside) towards the next one.
+    bool go = true;
+    V3 from = o1 ? nx.p : c.p;
+    V3 to = o1 ? c.p : nx.p;
+    if (o1 && o2) {
+      V3 mp;
+      go = project_inside_segment(ic.c, c.p, nx.p, &mp) && norm2(mp - ic.c) <= rr;
+      from = mp;
+    }
+    V3 ip1 = c.p, ip2 = c.p;
+    bool ok1 = false, ok2 = false;
+    if (go) {
+#pragma unroll 1
+      for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !(o1 && o2 && ok1)) break;
+        V3 ip;
+        const bool ok = segment_circle(ic, from, h == 0 ? to : nx.p, &ip);
+        if (h == 0) {
+          ok1 = ok;
+          ip1 = ip;
+        } else {
+          ok2 = ok;
+          ip2 = ip;
+        }
+      }
+    }
+    if (o1 && o2) {
+      if (go && ok1 && ok2) {
+        ins1 = ip1;
+        i1l = a_id;
+        i1r = c.r;
+        ins2 = ip2;
+        i2l = nx.l;
+        i2r = a_id;
+        nins = 2;
+      }
+    } else if (ok1) {
+      ins1 = ip1;
+      i1l = o1 ? a_id : nx.l;
+      i1r = o1 ? c.r : a_id;
+      nins = 1;
+    } else if (o1) {
+      mod_next = true;  // point j+1 gets ids (a, r_j)
+    } else {
+      mod_self = true;  // point j gets ids (l_{j+1}, a)
+    }
+  }
+  // Final ids of an inside point: segment j-1 runs before segment j, except
+  // that segment n-1 runs after segment 0 and so wins for point 0.
+  const bool prev_mod = (g.ballot(mod_next) >> (pj & 31)) & 1u;
+  const int32_t prev_r = g.shfl(c.r, pj & (kGroup - 1));
+  int32_t fl = c.l, fr = c.r;
+  if (j == 0) {
+    if (mod_self) {
+      fl = nx.l;
+      fr = a_id;
+    }
+    if (prev_mod) {
+      fl = a_id;
+      fr = prev_r;
+    }
+  } else {
+    if (prev_mod) {
+      fl = a_id;
+      fr = prev_r;
+    }
+    if (mod_self) {
+      fl = nx.l;
+      fr = a_id;
+    }
+  }
+  // Output order: point j (if inside), then segment j's crossings.
+  const int e = valid ? (in ? 1 : 0) + nins : 0;
+  const uint32_t B0 = g.ballot((e & 1) != 0), B1 = g.ballot((e & 2) != 0);
+  const uint32_t below = (1u << j) - 1u;
+  const int off = popc32(B0 & below) + 2 * popc32(B1 & below);
+  const int m = popc32(B0) + 2 * popc32(B1);
+  if (m > kGroupR) return kOverflow;
+  if (m - n_in == 0 || m < 2) return kContact;
+  if (valid) {
+    int o = off;
+    if (in) lds_put<G>(s, o++, c.p, fl, fr);
+    if (nins >= 1) lds_put<G>(s, o++, ins1, i1l, i1r);
+    if (nins == 2) lds_put<G>(s, o, ins2, i2l, i2r);
+  }
+  g.sync();
+  const V3 ctrl = ic.c + axis;
+#pragma unroll 1
+  for (int h = 0; h < 2; ++h) {
+    const int t = j + h * kGroup;
+    if (t < m) {
+      const int nt = t + 1 < m ? t + 1 : 0;
+      double ang = 0.0;
+      if (s->r[t] == a_id && s->l[nt] == a_id) ang = directed_angle(ic.c, lds_p(s, t), lds_p(s, nt), ctrl);
+      s->ang[t] = ang;
+    }
+  }
+  g.sync();
+  // non-arc entries hold +0.0, which leaves the running sum unchanged
+  double sm = 0.0;
+  for (int k = 0; k < m; ++k) sm = sm + s->ang[k];
+  *sum = sm;
+  if (f_ge(sm, kTau) || (m > 2 && f_eq(sm, kTau))) {
+    *sum = kTau;
+    return kContact;
+  }
+  *m_out = m;
+  return kContact;
+}
+
+// Sequential sum of per-entry terms held by the lanes (entry t in lane t % 16,
+// slot t / 16), in entry order: acc = (acc + t1[k]) + t2[k].
+template <class G>
+VLTR_HD double group_sum2(G& g, double t1a, double t1b, double t2a, double t2b, int m) {
+  double acc = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const int src = k & (kGroup - 1);
+    const double v1 = k < kGroup ? g.shfl(t1a, src) : g.shfl(t1b, src);
+    const double v2 = k < kGroup ? g.shfl(t2a, src) : g.shfl(t2b, src);
+    acc = acc + v1;
+    acc = acc + v2;
+  }
+  return acc;
+}
+
+// Per-vertex turn of the Gauss-Bonnet sum (contact.rs:797-837): first and
+// second addend (second is +0.0 when absent). The three cases differ only in
+// the dihedral's two outer points, so min_dihedral is evaluated once.
+VLTR_HD void turn_terms(const Sph& a, const Sph& b, const Sph& ic, const GroupLds* s, int i, int m, double* t1,
+                        double* t2) {
+  const int pi = i > 0 ? i - 1 : m - 1;
+  const int ni = i + 1 < m ? i + 1 : 0;
+  const V3 p0 = lds_p(s, pi), p1 = lds_p(s, i), p2 = lds_p(s, ni);
+  const double a0 = s->ang[pi], a1 = s->ang[i];
+  *t2 = 0.0;
+  V3 b1 = p0, b2 = p2;
+  if (a0 > 0.0 || a1 > 0.0) {
+    const bool before = a0 > 0.0;
+    const double an = before ? a0 : a1;
+    V3 d = cross(b.c - a.c, p1 - ic.c);
+    const double t = dot(d, (before ? p0 : p2) - p1);
+    if ((an < kPi && t < 0.0) || (an > kPi && t > 0.0)) d = -d;
+    if (before) {
+      b1 = p1 + d;
+    } else {
+      b2 = p1 + d;
+      *t2 = a1 * (norm(ic.c - a.c) / a.r);
+    }
+  }
+  *t1 = kPi - min_dihedral(a.c, p1, b1, b2);
+}
+
+template <class G>
+VLTR_HD double group_solid_angle(G& g, const Sph& a, const Sph& b, const Sph& ic, const GroupLds* s, int m) {
+  const int j = g.lane();
+  double t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0};
+#pragma unroll 1
+  for (int h = 0; h < 2; ++h) {
+    const int t = j + h * kGroup;
+    if (t < m) turn_terms(a, b, ic, s, t, m, &t1[h], &t2[h]);
+  }
+  const double turn = group_sum2(g, t1[0], t1[1], t2[0], t2[1], m);
+  double solid = kTau - turn;
+  const V3 ic_a = ic.c - a.c;
+  const V3 ic_b = ic.c - b.c;
+  if (dot(ic_a, ic_b) > 0.0 && norm2(a.c - ic.c) < norm2(b.c - ic.c)) solid = -solid;
+  return solid;
+}
+
+// Area and centrality terms of restricted-contour entry t (contact.rs:773-793,
+// 411-444).
+VLTR_HD void area_terms(const GroupLds* s, int t, int m, const Sph& ic, V3 bary, double* tri, double* seg,
+                        bool* reflex, bool* fail) {
+  const int nt = t + 1 < m ? t + 1 : 0;
+  const V3 p = lds_p(s, t), q = lds_p(s, nt);
+  *tri = triangle_area(bary, p, q);
+  const double ang = s->ang[t];
+  *seg = ang > 0.0 ? ic.r * ic.r * (ang - sin(ang)) * 0.5 : 0.0;
+  const double fe = 1e-10;
+  *reflex = (ang - fe) > kPi;
+  const V3 edge = q - p;
+  const double len = norm(edge);
+  *fail = false;
+  if (!(len < fe)) {
+    const V3 u = edge / len;
+    const V3 foot = p + u * dot(u, bary - p);
+    const V3 nb = bary - foot;
+    *fail = dot(nb, ic.c - p) < 0.0;
+  }
+}
+
+// The whole clip of a pair known to have no killing neighbor and a non-empty
+// cutting set (bit k of (mlo, mhi) = slot k of a's sorted neighbor list).
+// fetch(k) returns slot k's CutPlane. Every lane returns the same record.
+template <class G, class Fetch>
+VLTR_HD PairRec group_clip(G& g, GroupLds* s, const PairGeom& pg, int32_t a_id, uint64_t mlo, uint64_t mhi,
+                           Fetch&& fetch, const double* hex_s, const double* hex_c, int mode, int* status) {
+  const int j = g.lane();
+  GPt c{pg.ic.c, a_id, a_id};
+  int n = 0;
+  bool init = false;
+  *status = kNone;
+  for (int half = 0; half < 2; ++half) {
+    uint64_t mm = half == 0 ? mlo : mhi;
+    while (mm) {
+      const int k = half * 64 + ctz64(mm);
+      mm &= mm - 1;
+      const CutPlane cp = fetch(k);
+      if (!init) {
+        // contact.rs:455-469: hexagon of circumradius 1.19 r
+        const V3 first = (any_normal(pg.axis) * pg.ic.r) * kHexagonScale;
+        if (j == 0) {
+          c.p = pg.ic.c + first;
+        } else if (j < 6) {
+          c.p = pg.ic.c + rotate(pg.axis, hex_s[j - 1], hex_c[j - 1], first);
+        }
+        c.l = a_id;
+        c.r = a_id;
+        n = 6;
+        init = true;
+      } else {
+        // contact.rs:472-481
+        const double thr = norm2(cp.pc - pg.a.c);
+        if (g.ballot(j < n && norm2(c.p - pg.a.c) >= thr) == 0u) continue;
+      }
+      const int st = group_cut(g, c, n, cp);
+      if (st != kContact) {
+        *status = st;
+        return empty_record();
+      }
+    }
+  }
+  int m = 0;
+  double sum_arc = 0.0;
+  if (group_restrict(g, s, c, n, pg.ic, pg.axis, a_id, &m, &sum_arc) == kOverflow) {
+    *status = kOverflow;
+    return empty_record();
+  }
+  if (m == 0) return empty_record();
+  // barycenter, area and centrality
+  V3 sum{0.0, 0.0, 0.0};
+  for (int k = 0; k < m; ++k) sum = sum + lds_p(s, k);
+  const V3 bary = sum / static_cast<double>(m);
+  double tri[2] = {0.0, 0.0}, seg[2] = {0.0, 0.0};
+  bool reflex = false, fail = false;
+#pragma unroll 1
+  for (int h = 0; h < 2; ++h) {
+    const int t = j + h * kGroup;
+    if (t < m) {
+      bool rf, fl;
+      area_terms(s, t, m, pg.ic, bary, &tri[h], &seg[h], &rf, &fl);
+      reflex = reflex || rf;
+      fail = fail || fl;
+    }
+  }
+  const double area = group_sum2(g, tri[0], tri[1], seg[0], seg[1], m);
+  const bool central = g.ballot(reflex) != 0u || g.ballot(fail) == 0u;
+  if (area <= 0.0) return empty_record();
+  *status = kContact;
+  PairRec rec = empty_record();
+  rec.area = area;
+  rec.arc_length = sum_arc * pg.ic.r;
+  rec.flags = kRecValid | (central ? kRecCentral : 0u);
+  if (mode == 0) {
+    double solid[2];
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) solid[h] = group_solid_angle(g, h == 0 ? pg.a : pg.b, h == 0 ? pg.b : pg.a, pg.ic, s, m);
+    rec.solid_a = solid[0];
+    rec.solid_b = solid[1];
+    const double sa = rec.solid_a < 0.0 ? -1.0 : 1.0;
+    const double sb = rec.solid_b < 0.0 ? -1.0 : 1.0;
+    rec.pyr_a = norm(pg.ic.c - pg.a.c) * area / 3.0 * sa;
+    rec.pyr_b = norm(pg.ic.c - pg.b.c) * area / 3.0 * sb;
+  }
+  return rec;
+}
+
+}  // namespace vltr

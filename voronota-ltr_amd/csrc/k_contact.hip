@@ -4,11 +4,24 @@
 // circle, and reduced to area, arc length, centrality, the two solid angles
 // and the two pyramid volumes.
 //
-// One lane per pair. The contour is a fixed-capacity array; a pair whose
-// contour would outgrow it is flagged and redone by k_cut_slow, whose contour
-// lives in a global workspace sized for the longest neighbor list. Both run
-// the same templated code, so they produce identical results.
+// k_clip: one wavefront per ball a, in two phases.
+//  1. Classification. Lane k holds neighbor k of a (sphere, radical plane
+//     with a) in registers and in the wavefront's shared-memory plane table.
+//     For each pair (a, b) of a every lane classifies its neighbor
+//     (contact.rs:224-276; the answer depends on (a, b, c) only, never on the
+//     contour). A killing neighbor means no contact; no cutting neighbor
+//     means the uncut full circle, finished one pair per lane afterwards;
+//     otherwise the pair and its 64-bit cutting mask go to the wavefront's
+//     list.
+//  2. Clipping. The wavefront splits into four groups of 16 lanes; each group
+//     clips one listed pair at a time with group_clip (contour points in
+//     lanes, planes read from the shared table), four pairs side by side.
+// Balls with more than 64 neighbors, and pairs whose contour outgrows a
+// group, go to k_cut_slow: one lane per pair, the one-lane code of contour.h
+// with the contour in a global workspace. All paths produce identical
+// records.
 #include "contour.h"
+#include "group_clip.h"
 #include "vltr_internal.h"
 
 namespace vltr {
@@ -22,151 +35,298 @@ namespace {
 
 double h_hex_s[5], h_hex_c[5];
 
-__global__ __launch_bounds__(256) void k_cut(Pipeline p) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < p.n_pairs; t += stride) {
-    const int32_t a = p.pair_a[t], b = p.pair_b[t];
-    const int32_t* nbl = p.nb + static_cast<int64_t>(a) * p.nb_cap;
-    LocalCut<kMaxC> ct;
-    LocalArc<kMaxR> out;
-    int st;
-    const PairRec rec = make_record(reinterpret_cast<const double*>(p.sph), a, b, nbl, p.nb_cnt[a], p.mode, c_hex_s, c_hex_c, ct, out, &st);
-    p.rec[t] = rec;
-    if (st == kOverflow) {
-      const uint32_t k = atomicAdd(p.overflow_cnt, 1u);
-      p.overflow_list[k] = static_cast<int32_t>(t);
-      atomicOr(p.status, kStatContourOverflow);
-    }
-  }
+constexpr int kWave = 64;
+constexpr int kClipSlots = 64;  // neighbor slots handled by k_clip
+constexpr int kClipWaves = 4;   // wavefronts per k_clip block
+// Occupancy target of k_clip (wavefronts per SIMD). Its shared memory allows
+// three 4-wavefront blocks per CU, so 3 caps registers at 168 per lane.
+#ifndef VLTR_CLIP_WAVES_PER_EU
+#define VLTR_CLIP_WAVES_PER_EU 3
+#endif
+
+// Shared memory of one k_clip wavefront.
+struct ClipLds {
+  double pcx[kClipSlots], pcy[kClipSlots], pcz[kClipSlots];
+  double pnx[kClipSlots], pny[kClipSlots], pnz[kClipSlots];
+  double nnx[kClipSlots], nny[kClipSlots], nnz[kClipSlots];
+  int32_t cid[kClipSlots];
+  // listed pairs: (pair offset within a << 8) | slot of b; pairs to clip
+  // from the front (with their cutting masks), uncut pairs from the back
+  int32_t item[kClipSlots];
+  uint64_t mask[kClipSlots];
+  GroupLds grp[kWave / kGroup];
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kWave = 64;
+// 16-lane group of a wavefront (the G of group_clip.h).
+struct WaveGroup {
+  int base, ln;
+  __device__ int lane() const { return ln; }
+  __device__ uint32_t ballot(bool p) const { return static_cast<uint32_t>((__ballot(p) >> base) & 0xFFFFull); }
+  __device__ double shfl(double v, int src) const { return __shfl(v, src, kGroup); }
+  __device__ int32_t shfl(int32_t v, int src) const { return __shfl(v, src, kGroup); }
+  __device__ void sync() const { wave_sync(); }
+};
 
-// Prefilter: one wavefront per ball a, one lane per neighbor slot (two slots
-// per lane for lists of up to 128). Each lane keeps its neighbor's sphere and
-// radical plane with a in registers and classifies it for every pair (a, b)
-// of a (contact.rs:224-276). A pair with a killing neighbor gets an empty
-// record; a pair with no cutting neighbor is the uncut full circle and is
-// finished here; the rest go to the alive list with their cutting mask.
-__global__ __launch_bounds__(256) void k_prefilter(Pipeline p) {
+// Value of v in lane k (k uniform over the wavefront).
+__device__ __forceinline__ double readlane_d(double v, int k) {
+  const uint64_t u = static_cast<uint64_t>(__double_as_longlong(v));
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u & 0xffffffffu), k));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), k));
+  return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+__device__ __forceinline__ void push_slow(const Pipeline& p, int64_t pidx) {
+  const uint32_t k = atomicAdd(p.overflow_cnt, 1u);
+  p.overflow_list[k] = static_cast<int32_t>(pidx);
+  atomicOr(p.status, kStatContourOverflow);
+}
+__device__ __forceinline__ void push_wide(const Pipeline& p, int64_t pidx) {
+  const uint32_t k = atomicAdd(p.wide_cnt, 1u);
+  p.wide_list[k] = static_cast<int32_t>(pidx);
+}
+
+__global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_eu(VLTR_CLIP_WAVES_PER_EU, VLTR_CLIP_WAVES_PER_EU))) void k_clip(Pipeline p) {
+  __shared__ ClipLds lds[kClipWaves];
+  const int wid = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t q = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  ClipLds& w = lds[wid];
+  const int64_t q = xcd_remap(blockIdx.x, gridDim.x) * kClipWaves + wid;
   if (q >= p.n) return;
   const int64_t a = p.periodic ? q : static_cast<int64_t>(p.sorted_id[q]);
   const int64_t base = p.pair_cnt[a];
-  const int64_t npairs = static_cast<int64_t>(p.pair_cnt[a + 1]) - base;
+  const int npairs = static_cast<int>(static_cast<int64_t>(p.pair_cnt[a + 1]) - base);
   if (npairs <= 0) return;
   const int cnt = p.nb_cnt[a];
   const int32_t* nbl = p.nb + a * static_cast<int64_t>(p.nb_cap);
   const double* sph4 = reinterpret_cast<const double*>(p.sph);
-  if (cnt > 2 * kWave) {
-    for (int64_t j = lane; j < npairs; j += kWave) {
-      p.alive_all[base + j] = AliveRec{static_cast<int32_t>(base + j), kAliveFullScan, 0ull, 0ull};
-      p.alive_flag[base + j] = 1u;
-    }
+  if (cnt > kClipSlots) {
+    for (int j = lane; j < npairs; j += kWave) push_wide(p, base + j);
     return;
   }
+
+  // ---- phase 1: classification
+  // Lane k holds neighbor k and the geometry of the pair (a, neighbor k)
+  // (contact.rs:202-218); each pair's geometry is broadcast from its lane.
   const Sph A = load_sph(sph4, a);
-  int32_t c0 = -1, c1 = -1;
-  Sph C0{}, C1{};
-  Plane P0{}, P1{};
-  bool k0 = false, k1 = false;
+  int32_t c0 = -1;
+  Sph C{};
+  Plane P{};
+  bool c_has_a = false;
+  PairGeom gk{};
+  bool gk_ok = false;
   if (lane < cnt) {
     c0 = nbl[lane];
-    C0 = load_sph(sph4, c0);
-    P0 = plane_of(A, C0);
-    k0 = contains(C0, A);
+    C = load_sph(sph4, c0);
+    P = plane_of(A, C);
+    c_has_a = contains(C, A);
+    gk_ok = pair_setup(A, C, &gk);
+    const V3 nn = unit(P.pn);
+    w.pcx[lane] = P.pc.x;
+    w.pcy[lane] = P.pc.y;
+    w.pcz[lane] = P.pc.z;
+    w.pnx[lane] = P.pn.x;
+    w.pny[lane] = P.pn.y;
+    w.pnz[lane] = P.pn.z;
+    w.nnx[lane] = nn.x;
+    w.nny[lane] = nn.y;
+    w.nnz[lane] = nn.z;
+    w.cid[lane] = c0;
   }
-  if (lane + kWave < cnt) {
-    c1 = nbl[lane + kWave];
-    C1 = load_sph(sph4, c1);
-    P1 = plane_of(A, C1);
-    k1 = contains(C1, A);
-  }
-  int64_t j = 0;
+  const uint64_t ok_mask = __ballot(gk_ok);
+  int n_clip = 0, n_uncut = 0, jp = 0;
   for (int k = 0; k < cnt; ++k) {
-    const int32_t b = nbl[k];
+    const int32_t b = __builtin_amdgcn_readlane(c0, k);
     if (!pair_qualifies(p, a, b)) continue;
-    const int64_t pidx = base + j;
-    ++j;
-    PairGeom g;
-    if (!pair_setup(A, load_sph(sph4, b), &g)) {
-      if (lane == 0) {
-        p.rec[pidx] = empty_record();
-        p.alive_flag[pidx] = 0u;
-      }
+    const int jj = jp++;
+    if (!((ok_mask >> k) & 1ull)) {
+      if (lane == 0) p.rec[base + jj] = empty_record();
       continue;
     }
-    const int cl0 = (c0 >= 0 && c0 != b) ? classify(g, C0, P0, k0) : kSkip;
-    const int cl1 = (c1 >= 0 && c1 != b) ? classify(g, C1, P1, k1) : kSkip;
-    const bool kill = __ballot(cl0 == kKill || cl1 == kKill) != 0ull;
-    const uint64_t mlo = __ballot(cl0 == kCut);
-    const uint64_t mhi = __ballot(cl1 == kCut);
-    if (lane != 0) continue;
-    const bool live = !kill && (mlo | mhi) != 0ull;
+    // classify reads the circle, its direction from a, and b
+    PairGeom g;
+    g.ic = Sph{V3{readlane_d(gk.ic.c.x, k), readlane_d(gk.ic.c.y, k), readlane_d(gk.ic.c.z, k)},
+               readlane_d(gk.ic.r, k)};
+    g.ic_dir = V3{readlane_d(gk.ic_dir.x, k), readlane_d(gk.ic_dir.y, k), readlane_d(gk.ic_dir.z, k)};
+    g.b = Sph{V3{readlane_d(C.c.x, k), readlane_d(C.c.y, k), readlane_d(C.c.z, k)}, readlane_d(C.r, k)};
+    const int cl = (lane < cnt && lane != k) ? classify(g, C, P, c_has_a) : kSkip;
+    const bool kill = __ballot(cl == kKill) != 0ull;
+    const uint64_t M = __ballot(cl == kCut);
     if (kill) {
-      p.rec[pidx] = empty_record();
-    } else if (!live) {
-      p.rec[pidx] = record_uncut(g, p.mode);
+      if (lane == 0) p.rec[base + jj] = empty_record();
+    } else if (M == 0ull) {
+      if (lane == 0) w.item[kClipSlots - 1 - n_uncut] = (jj << 8) | k;
+      ++n_uncut;
     } else {
-      p.alive_all[pidx] = AliveRec{static_cast<int32_t>(pidx), 0u, mlo, mhi};
+      if (lane == 0) {
+        w.item[n_clip] = (jj << 8) | k;
+        w.mask[n_clip] = M;
+      }
+      ++n_clip;
     }
-    p.alive_flag[pidx] = live ? 1u : 0u;
+  }
+  wave_sync();
+
+  // uncut pairs: the full circle, one pair per lane
+  if (lane < n_uncut) {
+    const int it = w.item[kClipSlots - 1 - lane];
+    PairGeom g;
+    pair_setup(A, load_sph(sph4, w.cid[it & 0xff]), &g);
+    p.rec[base + (it >> 8)] = record_uncut(g, p.mode);
+  }
+
+  // Pairs to clip in decreasing cutting-set size, so that the four pairs
+  // clipped side by side need about as many cuts (any order gives the same
+  // records).
+  if (n_clip > 1) {
+    int it = 0, rank = 0;
+    uint64_t mk = 0;
+    if (lane < n_clip) {
+      it = w.item[lane];
+      mk = w.mask[lane];
+      const int pc = __popcll(mk);
+      for (int i = 0; i < n_clip; ++i) {
+        const int pi = __popcll(w.mask[i]);
+        rank += (pi > pc || (pi == pc && i < lane)) ? 1 : 0;
+      }
+    }
+    wave_sync();
+    if (lane < n_clip) {
+      w.item[rank] = it;
+      w.mask[rank] = mk;
+    }
+    wave_sync();
+  }
+
+  // ---- phase 2: four pairs at a time, 16 lanes each
+  const int grp = lane / kGroup;
+  WaveGroup G{grp * kGroup, lane & (kGroup - 1)};
+  GroupLds* gs = &w.grp[grp];
+  auto fetch = [&w](int s) {
+    return CutPlane{V3{w.pcx[s], w.pcy[s], w.pcz[s]}, V3{w.pnx[s], w.pny[s], w.pnz[s]},
+                    V3{w.nnx[s], w.nny[s], w.nnz[s]}, w.cid[s]};
+  };
+  for (int r0 = 0; r0 < n_clip; r0 += kWave / kGroup) {
+    const int idx = r0 + grp;
+    if (idx < n_clip) {
+      const int it = w.item[idx];
+      const uint64_t M = w.mask[idx];
+      PairGeom g;
+      pair_setup(A, load_sph(sph4, w.cid[it & 0xff]), &g);
+      int st = kNone;
+      const PairRec rec = group_clip(G, gs, g, static_cast<int32_t>(a), M, 0ull, fetch, c_hex_s, c_hex_c, p.mode, &st);
+      if (G.lane() == 0) {
+        if (st == kOverflow) {
+          push_slow(p, base + (it >> 8));
+        } else {
+          p.rec[base + (it >> 8)] = rec;
+        }
+      }
+    }
   }
 }
 
-// Alive pairs in pair order (alive_flag holds the exclusive scan).
-__global__ void k_alive_compact(Pipeline p) {
-  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (i >= p.n_pairs) return;
-  const uint32_t k = p.alive_flag[i];
-  if (p.alive_flag[i + 1] != k) p.alive[k] = p.alive_all[i];
-}
+// Pairs of balls with more than 64 neighbors (listed by k_clip): one 16-lane
+// group per pair classifies the pair's neighbors itself (16 at a time, up to
+// 128) and clips with group_clip, computing each cutting plane when it is
+// needed. Longer lists and contour overflows go to k_cut_slow. A fixed grid
+// walks the device-side count.
+constexpr int kWideBlocks = 1024;
+constexpr int kWideSlots = 128;
 
-// Contour cutting of the alive pairs, one lane per pair.
-__global__ __launch_bounds__(256) void k_cut_alive(Pipeline p) {
-  const int64_t n_alive = p.alive_flag[p.n_pairs];
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+__global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
+  __shared__ GroupLds grp[256 / kGroup];
+  const int gl = threadIdx.x / kGroup;
+  const int ln = threadIdx.x & (kGroup - 1);
+  WaveGroup G{(gl % (kWave / kGroup)) * kGroup, ln};
+  GroupLds* gs = &grp[gl];
   const double* sph4 = reinterpret_cast<const double*>(p.sph);
-  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n_alive; t += stride) {
-    const AliveRec ar = p.alive[t];
-    const int32_t a = p.pair_a[ar.pair], b = p.pair_b[ar.pair];
+  const int64_t n_wide = *p.wide_cnt;
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * (256 / kGroup);
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * (256 / kGroup) + gl; t < n_wide; t += groups) {
+    const int64_t pidx = p.wide_list[t];
+    const int32_t a = p.pair_a[pidx], b = p.pair_b[pidx];
+    const int cnt = p.nb_cnt[a];
     const int32_t* nbl = p.nb + static_cast<int64_t>(a) * p.nb_cap;
-    LocalCut<kMaxC> ct;
-    LocalArc<kMaxR> out;
-    int st;
-    const PairRec rec = (ar.flags & kAliveFullScan)
-                            ? make_record(sph4, a, b, nbl, p.nb_cnt[a], p.mode, c_hex_s, c_hex_c, ct, out, &st)
-                            : make_record_masked(sph4, a, b, nbl, ar.mlo, ar.mhi, p.mode, c_hex_s, c_hex_c, ct, out, &st);
-    p.rec[ar.pair] = rec;
-    if (st == kOverflow) {
-      const uint32_t k = atomicAdd(p.overflow_cnt, 1u);
-      p.overflow_list[k] = ar.pair;
-      atomicOr(p.status, kStatContourOverflow);
+    if (cnt > kWideSlots) {
+      if (ln == 0) push_slow(p, pidx);
+      continue;
+    }
+    const Sph A = load_sph(sph4, a);
+    PairGeom g;
+    if (!pair_setup(A, load_sph(sph4, b), &g)) {
+      if (ln == 0) p.rec[pidx] = empty_record();
+      continue;
+    }
+    bool kill = false;
+    uint64_t mlo = 0, mhi = 0;
+    for (int s0 = 0; s0 < cnt; s0 += kGroup) {
+      const int sl = s0 + ln;
+      int cl = kSkip;
+      if (sl < cnt && nbl[sl] != b) {
+        const Sph C = load_sph(sph4, nbl[sl]);
+        cl = classify(g, C, plane_of(A, C), contains(C, A));
+      }
+      kill = kill || G.ballot(cl == kKill) != 0u;
+      const uint64_t bits = static_cast<uint64_t>(G.ballot(cl == kCut));
+      if (s0 < 64) {
+        mlo |= bits << s0;
+      } else {
+        mhi |= bits << (s0 - 64);
+      }
+    }
+    if (kill) {
+      if (ln == 0) p.rec[pidx] = empty_record();
+      continue;
+    }
+    if ((mlo | mhi) == 0ull) {
+      if (ln == 0) p.rec[pidx] = record_uncut(g, p.mode);
+      continue;
+    }
+    auto fetch = [&](int k) {
+      const Plane pl = plane_of(A, load_sph(sph4, nbl[k]));
+      return CutPlane{pl.pc, pl.pn, unit(pl.pn), nbl[k]};
+    };
+    int st = kNone;
+    const PairRec rec = group_clip(G, gs, g, a, mlo, mhi, fetch, c_hex_s, c_hex_c, p.mode, &st);
+    if (ln == 0) {
+      if (st == kOverflow) {
+        push_slow(p, pidx);
+      } else {
+        p.rec[pidx] = rec;
+      }
     }
   }
 }
 
-// Slow path: same code with the contour in a global workspace region per
-// lane, sized for the longest possible contour (6 + nb_cap points while
-// cutting, twice that after the restriction).
-__global__ __launch_bounds__(64) void k_cut_slow(Pipeline p, int64_t first, int64_t count) {
-  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (t >= count) return;
-  const int64_t pi = p.overflow_list[first + t];
+// One lane per pair, the one-lane clip (contour.h) with the contour in a
+// global workspace region per lane, sized for the longest possible contour
+// (6 + nb_cap points while cutting, twice that after the restriction). A
+// fixed grid of p.slow_ws_pairs lanes walks the device-side count.
+__global__ __launch_bounds__(64) void k_cut_slow(Pipeline p) {
+  const int64_t t0 = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  const int64_t count = *p.overflow_cnt;
   const int cc = 6 + p.nb_cap;
   const int cr = 2 * cc;
-  double* ws = p.slow_ws + t * static_cast<int64_t>(5 * cc + 6 * cr);
+  double* ws = p.slow_ws + t0 * static_cast<int64_t>(5 * cc + 6 * cr);
   GlobalCut ct{cc, ws, ws + cc, ws + 2 * cc, reinterpret_cast<int32_t*>(ws + 3 * cc),
                reinterpret_cast<int32_t*>(ws + 4 * cc)};
   double* w2 = ws + 5 * cc;
   GlobalArc out{cr, w2, w2 + cr, w2 + 2 * cr, w2 + 3 * cr, reinterpret_cast<int32_t*>(w2 + 4 * cr),
                 reinterpret_cast<int32_t*>(w2 + 5 * cr)};
-  const int32_t a = p.pair_a[pi], b = p.pair_b[pi];
-  const int32_t* nbl = p.nb + static_cast<int64_t>(a) * p.nb_cap;
-  int st;
-  PairRec rec = make_record(reinterpret_cast<const double*>(p.sph), a, b, nbl, p.nb_cnt[a], p.mode, c_hex_s, c_hex_c, ct, out, &st);
-  p.rec[pi] = rec;
+  for (int64_t t = t0; t < count; t += p.slow_ws_pairs) {
+    const int64_t pi = p.overflow_list[t];
+    const int32_t a = p.pair_a[pi], b = p.pair_b[pi];
+    const int32_t* nbl = p.nb + static_cast<int64_t>(a) * p.nb_cap;
+    int st;
+    p.rec[pi] = make_record(reinterpret_cast<const double*>(p.sph), a, b, nbl, p.nb_cnt[a], p.mode, c_hex_s, c_hex_c,
+                            ct, out, &st);
+  }
 }
 
 }  // namespace
@@ -186,39 +346,26 @@ hipError_t upload_hexagon_constants(hipStream_t s) {
   return hipStreamSynchronize(s);
 }
 
-hipError_t launch_cut(const Pipeline& p, hipStream_t s) {
-  if (p.n_pairs <= 0) return hipSuccess;
-  int64_t blocks = (p.n_pairs + 255) / 256;
-  if (blocks > 65535 * 8) blocks = 65535 * 8;
-  hipLaunchKernelGGL(k_cut, dim3(blocks), dim3(256), 0, s, p);
-  return hipGetLastError();
-}
-
-hipError_t launch_prefilter(const Pipeline& p, uint32_t* scan_tmp, hipStream_t s) {
+hipError_t launch_clip(const Pipeline& p, hipStream_t s) {
   if (p.n <= 0 || p.n_pairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_prefilter, dim3((p.n + 3) / 4), dim3(256), 0, s, p);
-  hipError_t e = launch_scan_u32(p.alive_flag, p.n_pairs, scan_tmp, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_alive_compact, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p);
+  const int64_t blocks = (p.n + kClipWaves - 1) / kClipWaves;
+  hipLaunchKernelGGL(k_clip, dim3(blocks), dim3(kClipWaves * kWave), 0, s, p);
   return hipGetLastError();
 }
 
-hipError_t launch_cut_alive(const Pipeline& p, hipStream_t s) {
-  if (p.n_pairs <= 0) return hipSuccess;
-  int64_t blocks = (p.n_pairs + 255) / 256;
-  if (blocks > 65535 * 8) blocks = 65535 * 8;
-  hipLaunchKernelGGL(k_cut_alive, dim3(blocks), dim3(256), 0, s, p);
-  return hipGetLastError();
+int64_t slow_lanes(int nb_cap) {
+  // workspace of at most 2^23 doubles (64 MiB), 256..4096 lanes
+  const int64_t per = 5 * (6 + nb_cap) + 6 * 2 * (6 + nb_cap);
+  int64_t lanes = (int64_t(1) << 23) / per;
+  lanes = lanes < 256 ? 256 : (lanes > 4096 ? 4096 : lanes);
+  return lanes / 64 * 64;
 }
 
-hipError_t launch_cut_slow(const Pipeline& p, int64_t n_overflow, hipStream_t s) {
-  for (int64_t first = 0; first < n_overflow; first += p.slow_ws_pairs) {
-    const int64_t count = (n_overflow - first) < p.slow_ws_pairs ? (n_overflow - first) : p.slow_ws_pairs;
-    hipLaunchKernelGGL(k_cut_slow, dim3((count + 63) / 64), dim3(64), 0, s, p, first, count);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+hipError_t launch_fallbacks(const Pipeline& p, hipStream_t s) {
+  if (p.n <= 0 || p.n_pairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_clip_wide, dim3(kWideBlocks), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(k_cut_slow, dim3(p.slow_ws_pairs / 64), dim3(64), 0, s, p);
+  return hipGetLastError();
 }
 
 }  // namespace vltr

@@ -71,6 +71,7 @@ __global__ void k_init(Pipeline p) {
     *p.box_bits = static_cast<unsigned long long>(__double_as_longlong(1.0));
     *p.status = 0u;
     *p.overflow_cnt = 0u;
+    *p.wide_cnt = 0u;
   }
 }
 

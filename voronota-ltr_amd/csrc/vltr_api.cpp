@@ -101,7 +101,7 @@ struct vltr_context {
   int nb_cap = 96;
   // device buffers
   DevBuf balls, groups, sph, seg_of, segs, bkey, brank, bcnt, sorted_id, sorted_sph, small, nb, nb_cnt, excluded,
-      pair_cnt, pair_a, pair_b, rec, cell, overflow_list, flags, scan_tmp, out_contacts, out_cells, slow_ws, alive, alive_all,
+      pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
       in_cnt, incoming;
   HostPinned pinned;
   // last device-resident outputs
@@ -168,6 +168,7 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.box_bits = reinterpret_cast<unsigned long long*>(small);
   p.status = reinterpret_cast<uint32_t*>(small + 8);
   p.overflow_cnt = reinterpret_cast<uint32_t*>(small + 12);
+  p.wide_cnt = reinterpret_cast<uint32_t*>(small + 16);
   p.nb = ctx->nb.as<int32_t>();
   p.nb_cnt = ctx->nb_cnt.as<int32_t>();
   p.excluded = ctx->excluded.as<uint8_t>();
@@ -177,10 +178,8 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.rec = ctx->rec.as<PairRec>();
   p.cell = ctx->cell.as<CellRec>();
   p.overflow_list = ctx->overflow_list.as<int32_t>();
+  p.wide_list = ctx->wide_list.as<int32_t>();
   p.slow_ws = ctx->slow_ws.as<double>();
-  p.alive = ctx->alive.as<AliveRec>();
-  p.alive_all = ctx->alive_all.as<AliveRec>();
-  p.alive_flag = ctx->flags.as<uint32_t>();
   p.in_cnt = ctx->in_cnt.as<uint32_t>();
   p.incoming = ctx->incoming.as<int32_t>();
 }
@@ -265,8 +264,12 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->pair_b.ensure(p.pair_cap * sizeof(int32_t)));
   VLTR_HIP(ctx->rec.ensure(p.pair_cap * sizeof(PairRec)));
   VLTR_HIP(ctx->overflow_list.ensure(p.pair_cap * sizeof(int32_t)));
-  VLTR_HIP(ctx->alive.ensure(p.pair_cap * sizeof(AliveRec)));
-  VLTR_HIP(ctx->alive_all.ensure(p.pair_cap * sizeof(AliveRec)));
+  VLTR_HIP(ctx->wide_list.ensure(p.pair_cap * sizeof(int32_t)));
+  p.slow_ws_pairs = slow_lanes(p.nb_cap);
+  {
+    const int64_t cc = 6 + p.nb_cap, cr = 2 * cc;
+    VLTR_HIP(ctx->slow_ws.ensure(static_cast<size_t>(p.slow_ws_pairs) * (5 * cc + 6 * cr) * sizeof(double)));
+  }
   VLTR_HIP(ctx->flags.ensure((std::max<int64_t>(P, n) + 2) * sizeof(uint32_t)));
   VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(P, n) + 64) * sizeof(uint32_t) * 2));
   VLTR_HIP(ctx->out_contacts.ensure(p.pair_cap * sizeof(vltr_contact)));
@@ -274,19 +277,9 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   fill_pointers(ctx, p);
   VLTR_HIP(launch_pairs(p, s));
   VLTR_HIP(hipEventRecord(ctx->ev[3], s));
-  VLTR_HIP(launch_prefilter(p, ctx->scan_tmp.as<uint32_t>(), s));
+  VLTR_HIP(launch_clip(p, s));
   VLTR_HIP(hipEventRecord(ctx->ev[4], s));
-  VLTR_HIP(launch_cut_alive(p, s));
-  VLTR_HIP(hipMemcpyAsync(host_words + 2, p.overflow_cnt, 4, hipMemcpyDeviceToHost, s));
-  VLTR_HIP(hipStreamSynchronize(s));
-  const int64_t n_over = host_words[2];
-  if (n_over > 0) {
-    const int cc = 6 + p.nb_cap, cr = 2 * cc;
-    p.slow_ws_pairs = std::min<int64_t>(n_over, 8192);
-    VLTR_HIP(ctx->slow_ws.ensure(static_cast<size_t>(p.slow_ws_pairs) * (5 * cc + 6 * cr) * sizeof(double)));
-    p.slow_ws = ctx->slow_ws.as<double>();
-    VLTR_HIP(launch_cut_slow(p, n_over, s));
-  }
+  VLTR_HIP(launch_fallbacks(p, s));
   VLTR_HIP(hipEventRecord(ctx->ev[5], s));
   if (mode == 0) VLTR_HIP(launch_cells(p, s));
   uint32_t* flags = ctx->flags.as<uint32_t>();
@@ -301,6 +294,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     VLTR_HIP(hipMemcpyAsync(host_words + 4, flags + n, 4, hipMemcpyDeviceToHost, s));
   }
   VLTR_HIP(hipMemcpyAsync(host_words + 5, p.status, 4, hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipMemcpyAsync(host_words + 6, p.overflow_cnt, 8, hipMemcpyDeviceToHost, s));
   VLTR_HIP(hipEventRecord(ctx->ev[6], s));
   VLTR_HIP(hipStreamSynchronize(s));
   if (host_words[5] & kStatNbOverflow) {
@@ -319,7 +313,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   ctx->counts[2] = static_cast<uint64_t>(P);
   ctx->counts[3] = *n_contacts;
   ctx->counts[4] = *n_cells;
-  ctx->counts[5] = static_cast<uint64_t>(n_over);
+  ctx->counts[5] = static_cast<uint64_t>(host_words[6]) + host_words[7];
   ctx->counts[6] = static_cast<uint64_t>(p.nb_cap);
   float ms = 0.0f;
   for (int k = 0; k < kPhases - 1; ++k) {
@@ -481,8 +475,8 @@ void vltr_context_destroy(vltr_context* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->balls, &ctx->groups, &ctx->sph, &ctx->seg_of, &ctx->segs, &ctx->bkey, &ctx->brank, &ctx->bcnt,
                     &ctx->sorted_id, &ctx->sorted_sph, &ctx->small, &ctx->nb, &ctx->nb_cnt, &ctx->excluded,
-                    &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec, &ctx->cell, &ctx->overflow_list,
-                    &ctx->flags, &ctx->scan_tmp, &ctx->out_contacts, &ctx->out_cells, &ctx->slow_ws, &ctx->alive, &ctx->alive_all,
+                    &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec, &ctx->cell, &ctx->overflow_list, &ctx->wide_list,
+                    &ctx->flags, &ctx->scan_tmp, &ctx->out_contacts, &ctx->out_cells, &ctx->slow_ws,
                     &ctx->in_cnt, &ctx->incoming})
     b->release();
   ctx->pinned.release();

@@ -77,26 +77,14 @@ struct Pipeline {
   uint32_t* status;      // 1 word
   int32_t* overflow_list;  // pair_cap (pairs needing the slow kernel)
   uint32_t* overflow_cnt;  // 1
+  int32_t* wide_list;      // pair_cap (pairs of balls with more than 64 neighbors)
+  uint32_t* wide_cnt;      // 1
   double* slow_ws;         // workspace of the slow kernel
-  int64_t slow_ws_pairs;   // pairs the workspace holds at once
+  int64_t slow_ws_pairs;   // its lanes (one workspace region each)
 
-  struct AliveRec* alive_all;  // pair_cap, by pair index (prefilter output)
-  uint32_t* alive_flag;        // pair_cap + 1: 1 where alive_all is live; scanned
-                               // in place to positions, [n_pairs] = count
-  struct AliveRec* alive;      // pair_cap: the live entries, in pair order
   uint32_t* in_cnt;        // n: pairs (a, b), a < b, received by ball b
   int32_t* incoming;       // n * nb_cap: their pair indices (any order)
 };
-
-// A pair that survived the prefilter: bit k of the 128-bit mask marks slot k
-// of the first ball's neighbor list as a cutting neighbor. kAliveFullScan:
-// the list is longer than the mask, so the cut kernel scans it itself.
-struct AliveRec {
-  int32_t pair;
-  uint32_t flags;
-  uint64_t mlo, mhi;
-};
-constexpr uint32_t kAliveFullScan = 1u;
 
 // tessellation.rs:323-328
 __device__ __forceinline__ bool same_group(const Pipeline& p, int64_t a, int64_t b) {
@@ -122,10 +110,12 @@ hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream
 int64_t scan_tmp_words(int64_t n);
 
 // k_contact.hip
-hipError_t launch_cut(const Pipeline& p, hipStream_t s);
-hipError_t launch_prefilter(const Pipeline& p, uint32_t* scan_tmp, hipStream_t s);
-hipError_t launch_cut_alive(const Pipeline& p, hipStream_t s);
-hipError_t launch_cut_slow(const Pipeline& p, int64_t n_overflow, hipStream_t s);
+hipError_t launch_clip(const Pipeline& p, hipStream_t s);
+// k_clip_wide and k_cut_slow over the pairs k_clip listed (device-side
+// counts; no host synchronisation). p.slow_ws must hold slow_lanes(nb_cap)
+// regions.
+hipError_t launch_fallbacks(const Pipeline& p, hipStream_t s);
+int64_t slow_lanes(int nb_cap);
 void set_hexagon_constants(const double* s_half5, const double* c_half5);
 hipError_t upload_hexagon_constants(hipStream_t s);
 

@@ -128,7 +128,7 @@ def last_timings(device: int = 0):
     lib = load_library()
     buf = (C.c_double * 7)()
     k = lib.vltr_context_last_timings(context(device), buf, 7)
-    names = ["grid", "neighbors", "pairs", "prefilter", "cut", "cells", "total"]
+    names = ["grid", "neighbors", "pairs", "clip", "slow", "cells", "total"]
     return {names[i]: buf[i] for i in range(k)}
 
 

@@ -92,11 +92,15 @@ void lane_main(int ln) {
   GroupTask& t = *g_task;
   HostLane lane{t.f, ln};
   const vltr::Sph A = t.g.a;
-  auto fetch = [&](int k) {
-    const vltr::Sph C = vltr::load_sph(t.sph4, t.nbl[k]);
-    const vltr::Plane pl = vltr::plane_of(A, C);
-    return vltr::CutPlane{pl.pc, pl.pn, vltr::unit(pl.pn), t.nbl[k]};
-  };
+  struct {
+    GroupTask& t;
+    vltr::Sph A;
+    double thr(int k) const { return vltr::norm2(vltr::plane_of(A, vltr::load_sph(t.sph4, t.nbl[k])).pc - A.c); }
+    vltr::CutPlane plane(int k) const {
+      const vltr::Plane pl = vltr::plane_of(A, vltr::load_sph(t.sph4, t.nbl[k]));
+      return vltr::CutPlane{pl.pc, pl.pn, vltr::unit(pl.pn), t.nbl[k]};
+    }
+  } fetch{t, A};
   t.rec[ln] = vltr::group_clip(lane, &t.lds, t.g, t.a_id, t.mlo, t.mhi, fetch, t.hs, t.hc, t.mode, &t.st[ln]);
   t.f->done[ln] = true;
 }

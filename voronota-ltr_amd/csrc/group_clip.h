@@ -118,9 +118,8 @@ VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
     c.r = other.l;
     c.p = x;
   }
-  const V3 ps = gshfl3(g, c.p, start);
   const V3 pe = gshfl3(g, c.p, end);
-  if (!f_gt(norm2(pe - ps), 0.0)) {
+  if (g.ballot(j == start && !f_gt(norm2(pe - c.p), 0.0)) != 0u) {
     const int32_t re = g.shfl(c.r, end);
     if (j == start) c.r = re;
     c = gshfl(g, c, (j >= end ? j + 1 : j) & (kGroup - 1));
@@ -367,21 +366,28 @@ VLTR_HD void area_terms(const GroupLds* s, int t, int m, const Sph& ic, V3 bary,
 
 // The whole clip of a pair known to have no killing neighbor and a non-empty
 // cutting set (bit k of (mlo, mhi) = slot k of a's sorted neighbor list).
-// fetch(k) returns slot k's CutPlane. Every lane returns the same record.
+// f.plane(k) returns slot k's CutPlane and f.thr(k) the squared distance from
+// a's centre to its plane point. Every lane returns the same record.
 template <class G, class Fetch>
 VLTR_HD PairRec group_clip(G& g, GroupLds* s, const PairGeom& pg, int32_t a_id, uint64_t mlo, uint64_t mhi,
-                           Fetch&& fetch, const double* hex_s, const double* hex_c, int mode, int* status) {
+                           Fetch&& f, const double* hex_s, const double* hex_c, int mode, int* status) {
   const int j = g.lane();
   GPt c{pg.ic.c, a_id, a_id};
   int n = 0;
   bool init = false;
+  double d2 = 0.0;  // squared distance of this lane's point from a's centre
   *status = kNone;
   for (int half = 0; half < 2; ++half) {
     uint64_t mm = half == 0 ? mlo : mhi;
     while (mm) {
       const int k = half * 64 + ctz64(mm);
       mm &= mm - 1;
-      const CutPlane cp = fetch(k);
+      if (init) {
+        // contact.rs:472-481: some point at least as far from a's centre as
+        // the plane point
+        if (g.ballot(j < n && d2 >= f.thr(k)) == 0u) continue;
+      }
+      const CutPlane cp = f.plane(k);
       if (!init) {
         // contact.rs:455-469: hexagon of circumradius 1.19 r
         const V3 first = (any_normal(pg.axis) * pg.ic.r) * kHexagonScale;
@@ -394,16 +400,13 @@ VLTR_HD PairRec group_clip(G& g, GroupLds* s, const PairGeom& pg, int32_t a_id, 
         c.r = a_id;
         n = 6;
         init = true;
-      } else {
-        // contact.rs:472-481
-        const double thr = norm2(cp.pc - pg.a.c);
-        if (g.ballot(j < n && norm2(c.p - pg.a.c) >= thr) == 0u) continue;
       }
       const int st = group_cut(g, c, n, cp);
       if (st != kContact) {
         *status = st;
         return empty_record();
       }
+      d2 = norm2(c.p - pg.a.c);
     }
   }
   int m = 0;

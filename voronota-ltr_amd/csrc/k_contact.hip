@@ -49,6 +49,7 @@ struct ClipLds {
   double pcx[kClipSlots], pcy[kClipSlots], pcz[kClipSlots];
   double pnx[kClipSlots], pny[kClipSlots], pnz[kClipSlots];
   double nnx[kClipSlots], nny[kClipSlots], nnz[kClipSlots];
+  double thr[kClipSlots];  // squared distance from a's centre to the plane point
   int32_t cid[kClipSlots];
   // listed pairs: (pair offset within a << 8) | slot of b; pairs to clip
   // from the front (with their cutting masks), uncut pairs from the back
@@ -136,6 +137,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     w.nnx[lane] = nn.x;
     w.nny[lane] = nn.y;
     w.nnz[lane] = nn.z;
+    w.thr[lane] = norm2(P.pc - A.c);
     w.cid[lane] = c0;
   }
   const uint64_t ok_mask = __ballot(gk_ok);
@@ -207,10 +209,14 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
   const int grp = lane / kGroup;
   WaveGroup G{grp * kGroup, lane & (kGroup - 1)};
   GroupLds* gs = &w.grp[grp];
-  auto fetch = [&w](int s) {
-    return CutPlane{V3{w.pcx[s], w.pcy[s], w.pcz[s]}, V3{w.pnx[s], w.pny[s], w.pnz[s]},
-                    V3{w.nnx[s], w.nny[s], w.nnz[s]}, w.cid[s]};
-  };
+  struct {
+    const ClipLds& w;
+    __device__ double thr(int s) const { return w.thr[s]; }
+    __device__ CutPlane plane(int s) const {
+      return CutPlane{V3{w.pcx[s], w.pcy[s], w.pcz[s]}, V3{w.pnx[s], w.pny[s], w.pnz[s]},
+                      V3{w.nnx[s], w.nny[s], w.nnz[s]}, w.cid[s]};
+    }
+  } fetch{w};
   for (int r0 = 0; r0 < n_clip; r0 += kWave / kGroup) {
     const int idx = r0 + grp;
     if (idx < n_clip) {
@@ -288,10 +294,16 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       if (ln == 0) p.rec[pidx] = record_uncut(g, p.mode);
       continue;
     }
-    auto fetch = [&](int k) {
-      const Plane pl = plane_of(A, load_sph(sph4, nbl[k]));
-      return CutPlane{pl.pc, pl.pn, unit(pl.pn), nbl[k]};
-    };
+    struct {
+      const double* sph4;
+      const int32_t* nbl;
+      Sph A;
+      __device__ double thr(int k) const { return norm2(circle_center(A, load_sph(sph4, nbl[k])) - A.c); }
+      __device__ CutPlane plane(int k) const {
+        const Plane pl = plane_of(A, load_sph(sph4, nbl[k]));
+        return CutPlane{pl.pc, pl.pn, unit(pl.pn), nbl[k]};
+      }
+    } fetch{sph4, nbl, A};
     int st = kNone;
     const PairRec rec = group_clip(G, gs, g, a, mlo, mhi, fetch, c_hex_s, c_hex_c, p.mode, &st);
     if (ln == 0) {

@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--probe", type=float, default=1.4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--batch-structs", type=int, default=4096,
+                    help="structures in the batched leg (0 skips it); sizes log-uniform in [200, 20000]")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -86,6 +88,8 @@ def main():
     cut_ms = ph_full["clip"]
     flops = FLOP_PER_PAIR * counts["pairs"]
     achieved = flops / (cut_ms / 1e3) / 1e12 if cut_ms > 0 else 0.0
+
+    batched = batched_leg(args, lib, ctx) if args.batch_structs > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -137,9 +141,68 @@ def main():
         "contacts_only": {"balls_per_s": round(args.n / (ms_co / 1e3), 1), "ms_per_step": round(ms_co, 4),
                           "phases_ms": ph_co, "contacts": counts_co["contacts"]},
     }
+    if batched:
+        out["batched"] = batched
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
     print(json.dumps(out), flush=True)
+
+
+def batched_leg(args, lib, ctx):
+    """Many independent structures: one batched pass vs one call per
+    structure, both on device-resident balls (structure k: RSA cluster of
+    n_k balls, n_k log-uniform in [200, 20000], seed 1000000 + k)."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from voronota_ltr import _hip, _native, workload
+
+    sizes = workload.batch_sizes(args.batch_structs, seed=4096)
+    with ThreadPoolExecutor(max_workers=min(os.cpu_count() or 1, 16)) as ex:
+        structs = list(ex.map(lambda k: workload.rsa_sphere(int(sizes[k]), seed=1000000 + k)[0], range(len(sizes))))
+    off = _native.offsets_of(structs)
+    allb = np.ascontiguousarray(np.concatenate(structs))
+    del structs
+    total = int(off[-1])
+    dbuf = _hip.DeviceBuffer(allb.nbytes)
+    dbuf.upload(allb)
+    offp = off.ctypes.data_as(C.POINTER(C.c_uint64))
+
+    def batched_once():
+        _native.check(lib.vltr_compute_tessellation_batched_device(ctx, dbuf.ptr, offp, len(sizes), args.probe, 0),
+                      "vltr_compute_tessellation_batched_device")
+
+    batched_once()
+    _hip.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batched_once()
+    _hip.synchronize()
+    t_batched = (time.perf_counter() - t0) / reps
+    counts = _native.last_counts()
+
+    base = dbuf.ptr.value
+    for k in range(min(8, len(sizes))):  # warm the per-call path
+        lib.vltr_compute_tessellation_device(ctx, C.c_void_p(base + int(off[k]) * 32), int(sizes[k]), args.probe,
+                                             None, 0, 0)
+    _hip.synchronize()
+    t0 = time.perf_counter()
+    for k in range(len(sizes)):
+        _native.check(lib.vltr_compute_tessellation_device(ctx, C.c_void_p(base + int(off[k]) * 32), int(sizes[k]),
+                                                           args.probe, None, 0, 0), "vltr_compute_tessellation_device")
+    _hip.synchronize()
+    t_plain = time.perf_counter() - t0
+    dbuf.free()
+    return {"structures": len(sizes), "balls": total, "pairs": counts["pairs"], "contacts": counts["contacts"],
+            "batched_s": round(t_batched, 4), "batched_balls_per_s": round(total / t_batched, 1),
+            "plain_s": round(t_plain, 4), "plain_balls_per_s": round(total / t_plain, 1),
+            "batched_over_plain": round(t_plain / t_batched, 3),
+            "note": "full compute_tessellation of every structure, balls resident in HBM; plain = one "
+                    "vltr_compute_tessellation_device call per structure, batched = one "
+                    "vltr_compute_tessellation_batched_device call"}
 
 
 if __name__ == "__main__":

@@ -75,6 +75,7 @@ typedef struct vltr_cell_edge {
 
 typedef struct vltr_context vltr_context;
 typedef struct vltr_result vltr_result;
+typedef struct vltr_batch_result vltr_batch_result;
 
 /* ------------------------------------------------------------ context --- */
 
@@ -147,6 +148,31 @@ int vltr_context_device_outputs(const vltr_context* ctx, const vltr_contact** d_
  * Returns the total number of entries. */
 int64_t vltr_context_neighbor_lists(const vltr_context* ctx, size_t num_balls, int32_t* counts, int32_t* ids,
                                     size_t max_ids);
+
+/* ------------------------------------------------------------ batched --- */
+
+/* Many independent structures in one pipeline pass. Structure k is
+ * balls[offsets[k] .. offsets[k+1]) (offsets[0] = 0, n_structs + 1 entries,
+ * non-decreasing); its result equals compute_tessellation (or, with
+ * contacts_only, compute_contacts_only) of those balls alone
+ * (tessellation.rs:37-48, 125-166), with structure-local indices.
+ * Non-periodic, no groups, no cell vertices. No reference counterpart: the
+ * reference runs structures one call at a time. */
+int vltr_compute_tessellation_batched(vltr_context* ctx, const vltr_ball* balls, const uint64_t* offsets,
+                                      size_t n_structs, double probe, int contacts_only, vltr_batch_result** out);
+size_t vltr_batch_num_structs(const vltr_batch_result* r);
+/* Structure k's contacts / cells (count written to *count). */
+const vltr_contact* vltr_batch_contacts(const vltr_batch_result* r, size_t k, size_t* count);
+const vltr_cell* vltr_batch_cells(const vltr_batch_result* r, size_t k, size_t* count);
+void vltr_batch_free(vltr_batch_result* r);
+
+/* Device-resident batched run: d_balls is a device pointer, offsets a host
+ * array. Outputs stay in ctx (vltr_context_device_outputs: all structures
+ * back to back); vltr_context_batch_offsets copies the n_structs + 1
+ * per-structure start positions of contacts and cells (NULL to skip). */
+int vltr_compute_tessellation_batched_device(vltr_context* ctx, const vltr_ball* d_balls, const uint64_t* offsets,
+                                             size_t n_structs, double probe, int contacts_only);
+int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64_t* contact_off, uint64_t* cell_off);
 
 /* ------------------------------------------------------------ results --- */
 

@@ -1,0 +1,87 @@
+"""Batched runs (many structures in one pipeline pass) through the C ABI.
+
+Every structure of a batch must give exactly what a call on that structure
+alone gives: the same contact pair sequence, flags and cell indices, and
+bit-identical values (the same arithmetic runs on the same inputs), and so
+the same parity with the oracle as single calls. The batch mixes the
+reference's fixtures, seeded RSA clusters of different sizes, one ball and an
+empty structure.
+"""
+import numpy as np
+import pytest
+
+from conftest import CENTRAL8, RING17, load_xyzr
+from parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _structures():
+    from voronota_ltr import workload
+
+    return [
+        ("cs_1x1", load_xyzr("balls_cs_1x1.xyzr"), None),
+        ("ring17", RING17, None),
+        ("empty", np.zeros((0, 4)), None),
+        ("central8", CENTRAL8, None),
+        ("one ball", np.array([[1.0, 2.0, 3.0, 1.5]]), None),
+        ("1ctf", load_xyzr("1ctf_pdb.xyzr"), None),
+        ("rsa 2500", workload.rsa_sphere(2500, seed=2500)[0], None),
+        ("rsa 700", workload.rsa_sphere(700, seed=7)[0], None),
+        ("2zsk", load_xyzr("balls_2zsk.xyzr"), None),
+    ]
+
+
+def _same(part, single, check_cells=True):
+    assert np.array_equal(part.contacts, single.contacts)
+    if check_cells:
+        assert np.array_equal(part.cells, single.cells)
+
+
+@pytest.mark.parametrize("contacts_only", [False, True])
+def test_batched_equals_single_calls(oracle, contacts_only):
+    from voronota_ltr import _native
+
+    probe = 1.4
+    structs = _structures()
+    print(f"\n[gpu] batched: {len(structs)} structures, contacts_only={contacts_only}", flush=True)
+    parts = _native.run_batched([s for _, s, _ in structs], probe, contacts_only=contacts_only)
+    assert len(parts) == len(structs)
+    for (name, balls, _), part in zip(structs, parts):
+        print(f"[gpu] batched part {name}: n={len(balls)}", flush=True)
+        single = _native.run(balls, probe, contacts_only=contacts_only)
+        _same(part, single, check_cells=not contacts_only)
+        if len(balls) == 0:
+            assert len(part.contacts) == 0 and len(part.cells) == 0
+            continue
+        if contacts_only:
+            ref = oracle.compute_contacts_only(balls, probe)
+            compare(part, ref, balls, probe, check_cells=False)
+        else:
+            ref = oracle.compute_tessellation(balls, probe)
+            compare(part, ref, balls, probe)
+
+
+def test_batched_device_offsets():
+    """Device-resident batched entry: the per-structure offsets partition the
+    outputs as the host entry's parts do."""
+    import ctypes as C
+
+    from voronota_ltr import _hip, _native, workload
+
+    structs = [workload.rsa_sphere(n, seed=n)[0] for n in (300, 1200, 50, 2000)]
+    parts = _native.run_batched(structs, 1.4)
+    lib = _native.load_library()
+    ctx = _native.context(0)
+    allb = np.ascontiguousarray(np.concatenate(structs))
+    off = _native.offsets_of(structs)
+    buf = _hip.DeviceBuffer(allb.nbytes)
+    buf.upload(allb)
+    _native.check(lib.vltr_compute_tessellation_batched_device(ctx, buf.ptr, off.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                               len(structs), 1.4, 0), "batched device")
+    coff = np.zeros(len(structs) + 1, np.uint64)
+    loff = np.zeros(len(structs) + 1, np.uint64)
+    _native.check(lib.vltr_context_batch_offsets(ctx, len(structs), coff.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                 loff.ctypes.data_as(C.POINTER(C.c_uint64))), "batch offsets")
+    assert list(np.diff(coff)) == [len(p.contacts) for p in parts]
+    assert list(np.diff(loff)) == [len(p.cells) for p in parts]

@@ -135,9 +135,12 @@ __global__ void k_emit_contacts(Pipeline p, const uint32_t* pos, vltr_contact* o
   const uint32_t k = pos[i];
   if (pos[i + 1] == k) return;
   const PairRec r = p.rec[i];
+  // ids local to the ball's structure (batched runs)
+  const int32_t a = p.pair_a[i];
+  const int64_t off = p.n_segs > 1 ? p.segs[p.seg_of[a]].ball_off : 0;
   vltr_contact c;
-  c.id_a = static_cast<uint64_t>(p.pair_a[i]);
-  c.id_b = static_cast<uint64_t>(p.pair_b[i]);
+  c.id_a = static_cast<uint64_t>(a - off);
+  c.id_b = static_cast<uint64_t>(p.pair_b[i] - off);
   c.area = r.area;
   c.arc_length = r.arc_length;
   c.central = (r.flags & kRecCentral) ? 1 : 0;
@@ -151,16 +154,25 @@ __global__ void k_flag_cells(const CellRec* cell, int64_t n, uint32_t* flag) {
   flag[i] = cell[i].stage == 2 ? 1u : 0u;
 }
 
-__global__ void k_emit_cells(const CellRec* cell, int64_t n, const uint32_t* pos, vltr_cell* out) {
+__global__ void k_emit_cells(Pipeline p, const uint32_t* pos, vltr_cell* out) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
+  if (i >= p.n) return;
   const uint32_t k = pos[i];
   if (pos[i + 1] == k) return;
+  const CellRec* cell = p.cell;
   vltr_cell c;
-  c.index = static_cast<uint64_t>(i);
+  c.index = static_cast<uint64_t>(i - (p.n_segs > 1 ? p.segs[p.seg_of[i]].ball_off : 0));
   c.sas_area = cell[i].sas;
   c.volume = cell[i].volume;
   out[k] = c;
+}
+
+__global__ void k_seg_offsets(Pipeline p, const uint32_t* pos, int what, uint32_t* out) {
+  const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (k > p.n_segs) return;
+  const int64_t ball = k < p.n_segs ? p.segs[k].ball_off : p.n;
+  // contacts: the first pair of the structure's first ball
+  out[k] = what == 0 ? pos[p.pair_cnt[ball]] : pos[ball];
 }
 
 inline int64_t blocks_for(int64_t n, int64_t per) { return (n + per - 1) / per; }
@@ -186,7 +198,12 @@ hipError_t launch_flag_cells(const Pipeline& p, uint32_t* flag, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_emit_cells(const Pipeline& p, const uint32_t* pos, vltr_cell* out, hipStream_t s) {
-  if (p.n > 0) hipLaunchKernelGGL(k_emit_cells, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p.cell, p.n, pos, out);
+  if (p.n > 0) hipLaunchKernelGGL(k_emit_cells, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p, pos, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_offsets(const Pipeline& p, const uint32_t* pos, int what, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_offsets, dim3(blocks_for(p.n_segs + 1, 256)), dim3(256), 0, s, p, pos, what, out);
   return hipGetLastError();
 }
 

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <string>
 #include <unordered_map>
+#include <memory>
 #include <vector>
 
 #include "vltr_internal.h"
@@ -102,13 +103,21 @@ struct vltr_context {
   // device buffers
   DevBuf balls, groups, sph, seg_of, segs, bkey, brank, bcnt, sorted_id, sorted_sph, small, nb, nb_cnt, excluded,
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
-      in_cnt, incoming;
+      in_cnt, incoming, seg_out;
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
   size_t last_n = 0;
+  size_t last_structs = 0;  // structures of the last batched run (0: not batched)
   int last_nb_cap = 0;
   uint64_t counts[7] = {};
+};
+
+struct vltr_batch_result {
+  size_t n_structs = 0;
+  std::vector<uint64_t> ball_off, contact_off, cell_off;  // n_structs + 1 each
+  std::vector<vltr_contact> contacts;
+  std::vector<vltr_cell> cells;
 };
 
 struct vltr_result {
@@ -188,14 +197,16 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
 // the compacted contacts/cells are in ctx->out_contacts / out_cells and their
 // counts in *n_contacts / *n_cells.
 int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic_box* pbox, const int32_t* d_groups,
-                 int64_t n_groups, int mode, size_t* n_contacts, size_t* n_cells) {
+                 int64_t n_groups, int mode, size_t* n_contacts, size_t* n_cells, const uint64_t* seg_offsets = nullptr,
+                 int64_t n_structs = 0) {
   hipStream_t s = ctx->stream;
   const bool periodic = pbox != nullptr;
   const int64_t n_pop = periodic ? 27 * n : n;
   Pipeline p{};
   p.n = n;
   p.n_pop = n_pop;
-  p.n_segs = 1;
+  const bool batched = seg_offsets != nullptr && n_structs > 0;
+  p.n_segs = batched ? static_cast<int32_t>(n_structs) : 1;
   p.periodic = periodic ? 1 : 0;
   p.mode = mode;
   p.probe = probe;
@@ -207,16 +218,25 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   p.groups = d_groups;
   p.n_groups = n_groups;
 
-  GridSeg seg{};
-  seg.ball_off = 0;
-  seg.tab_off = 0;
-  seg.kbits = kbits_for(n_pop);
-  seg.n = static_cast<int32_t>(n_pop);
-  p.n_buckets = int64_t(1) << (3 * seg.kbits);
+  // One cell-hash segment per structure: its own bucket range, sized for it.
+  std::vector<GridSeg> segs(static_cast<size_t>(p.n_segs));
+  int64_t tab = 0;
+  for (int32_t k = 0; k < p.n_segs; ++k) {
+    GridSeg& sg = segs[k];
+    const int64_t nk = batched ? static_cast<int64_t>(seg_offsets[k + 1] - seg_offsets[k]) : n_pop;
+    sg.ball_off = batched ? static_cast<int64_t>(seg_offsets[k]) : 0;
+    sg.tab_off = tab;
+    sg.kbits = kbits_for(nk);
+    sg.n = static_cast<int32_t>(nk);
+    tab += int64_t(1) << (3 * sg.kbits);
+  }
+  if (tab >= (int64_t(1) << 32)) return fail(VLTR_ERR_INVALID_ARGUMENT, "too many cells");
+  p.n_buckets = tab;
 
   VLTR_HIP(ctx->sph.ensure(n_pop * sizeof(double4)));
   VLTR_HIP(ctx->seg_of.ensure(n_pop * sizeof(int32_t)));
-  VLTR_HIP(ctx->segs.ensure(sizeof(GridSeg)));
+  VLTR_HIP(ctx->segs.ensure(segs.size() * sizeof(GridSeg)));
+  VLTR_HIP(ctx->seg_out.ensure(2 * (segs.size() + 1) * sizeof(uint32_t)));
   VLTR_HIP(ctx->bkey.ensure(n_pop * sizeof(uint32_t)));
   VLTR_HIP(ctx->brank.ensure(n_pop * sizeof(uint32_t)));
   VLTR_HIP(ctx->bcnt.ensure((p.n_buckets + 1) * sizeof(uint32_t)));
@@ -230,7 +250,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->in_cnt.ensure(n * sizeof(uint32_t)));
   VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(p.n_buckets, 1) + n + 64) * sizeof(uint32_t) * 2));
   VLTR_HIP(ctx->pinned.ensure(64));
-  VLTR_HIP(hipMemcpyAsync(ctx->segs.ptr, &seg, sizeof(seg), hipMemcpyHostToDevice, s));
+  VLTR_HIP(hipMemcpyAsync(ctx->segs.ptr, segs.data(), segs.size() * sizeof(GridSeg), hipMemcpyHostToDevice, s));
 
   uint32_t* host_words = static_cast<uint32_t*>(ctx->pinned.ptr);
  restart:
@@ -287,11 +307,14 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(launch_scan_u32(flags, P, ctx->scan_tmp.as<uint32_t>(), s));
   VLTR_HIP(launch_emit_contacts(p, flags, ctx->out_contacts.as<vltr_contact>(), s));
   VLTR_HIP(hipMemcpyAsync(host_words + 3, flags + P, 4, hipMemcpyDeviceToHost, s));
+  uint32_t* seg_out = ctx->seg_out.as<uint32_t>();
+  if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 0, seg_out, s));
   if (mode == 0) {
     VLTR_HIP(launch_flag_cells(p, flags, s));
     VLTR_HIP(launch_scan_u32(flags, n, ctx->scan_tmp.as<uint32_t>(), s));
     VLTR_HIP(launch_emit_cells(p, flags, ctx->out_cells.as<vltr_cell>(), s));
     VLTR_HIP(hipMemcpyAsync(host_words + 4, flags + n, 4, hipMemcpyDeviceToHost, s));
+    if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 1, seg_out + p.n_segs + 1, s));
   }
   VLTR_HIP(hipMemcpyAsync(host_words + 5, p.status, 4, hipMemcpyDeviceToHost, s));
   VLTR_HIP(hipMemcpyAsync(host_words + 6, p.overflow_cnt, 8, hipMemcpyDeviceToHost, s));
@@ -307,6 +330,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   *n_contacts = host_words[3];
   *n_cells = mode == 0 ? host_words[4] : 0;
   ctx->last_n = static_cast<size_t>(n);
+  ctx->last_structs = batched ? static_cast<size_t>(n_structs) : 0;
   ctx->last_nb_cap = p.nb_cap;
   ctx->counts[0] = static_cast<uint64_t>(n);
   ctx->counts[1] = static_cast<uint64_t>(n_pop);
@@ -434,9 +458,132 @@ int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe
   return VLTR_OK;
 }
 
+int check_offsets(const uint64_t* offsets, size_t n_structs) {
+  if (!offsets || n_structs == 0) return fail(VLTR_ERR_INVALID_ARGUMENT, "no structures");
+  if (offsets[0] != 0) return fail(VLTR_ERR_INVALID_ARGUMENT, "offsets[0] must be 0");
+  for (size_t k = 0; k < n_structs; ++k)
+    if (offsets[k + 1] < offsets[k]) return fail(VLTR_ERR_INVALID_ARGUMENT, "offsets must not decrease");
+  if (offsets[n_structs] > (uint64_t(1) << 31) - 1) return fail(VLTR_ERR_INVALID_ARGUMENT, "too many balls");
+  if (n_structs > (size_t(1) << 24)) return fail(VLTR_ERR_INVALID_ARGUMENT, "too many structures");
+  return VLTR_OK;
+}
+
+// Batched run on ctx->balls (already on the device); per-structure offsets of
+// the outputs are left in ctx->seg_out.
+int run_batched(vltr_context* ctx, const uint64_t* offsets, size_t n_structs, double probe, int mode, size_t* nc,
+                size_t* ncell) {
+  const int64_t n = static_cast<int64_t>(offsets[n_structs]);
+  *nc = *ncell = 0;
+  if (n == 0) {
+    ctx->last_structs = n_structs;
+    ctx->last_n = 0;
+    ctx->last_contacts = ctx->last_cells = 0;
+    return VLTR_OK;
+  }
+  return run_pipeline(ctx, n, probe, nullptr, nullptr, 0, mode, nc, ncell, offsets, static_cast<int64_t>(n_structs));
+}
+
 }  // namespace
 
 extern "C" {
+
+int vltr_compute_tessellation_batched(vltr_context* ctx, const vltr_ball* balls, const uint64_t* offsets,
+                                      size_t n_structs, double probe, int contacts_only, vltr_batch_result** out) {
+  if (!ctx || !out) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  const int chk = check_offsets(offsets, n_structs);
+  if (chk != VLTR_OK) return chk;
+  const size_t n = static_cast<size_t>(offsets[n_structs]);
+  if (n > 0 && !balls) return fail(VLTR_ERR_INVALID_ARGUMENT, "null balls");
+  auto r = std::make_unique<vltr_batch_result>();
+  r->n_structs = n_structs;
+  r->ball_off.assign(offsets, offsets + n_structs + 1);
+  r->contact_off.assign(n_structs + 1, 0);
+  r->cell_off.assign(n_structs + 1, 0);
+  const int mode = contacts_only ? 1 : 0;
+  if (n > 0) {
+    VLTR_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
+    VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, balls, n * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
+    size_t nc = 0, ncell = 0;
+    const int rc = run_batched(ctx, offsets, n_structs, probe, mode, &nc, &ncell);
+    if (rc != VLTR_OK) return rc;
+    r->contacts.resize(nc);
+    r->cells.resize(ncell);
+    std::vector<uint32_t> so(2 * (n_structs + 1), 0u);
+    if (nc) VLTR_HIP(hipMemcpyAsync(r->contacts.data(), ctx->out_contacts.ptr, nc * sizeof(vltr_contact),
+                                    hipMemcpyDeviceToHost, s));
+    if (ncell)
+      VLTR_HIP(hipMemcpyAsync(r->cells.data(), ctx->out_cells.ptr, ncell * sizeof(vltr_cell), hipMemcpyDeviceToHost, s));
+    VLTR_HIP(hipMemcpyAsync(so.data(), ctx->seg_out.ptr, so.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    VLTR_HIP(hipStreamSynchronize(s));
+    for (size_t k = 0; k <= n_structs; ++k) {
+      r->contact_off[k] = so[k];
+      r->cell_off[k] = mode == 0 ? so[n_structs + 1 + k] : 0;
+    }
+  }
+  *out = r.release();
+  return VLTR_OK;
+}
+
+int vltr_compute_tessellation_batched_device(vltr_context* ctx, const vltr_ball* d_balls, const uint64_t* offsets,
+                                             size_t n_structs, double probe, int contacts_only) {
+  if (!ctx) return fail(VLTR_ERR_INVALID_ARGUMENT, "null context");
+  const int chk = check_offsets(offsets, n_structs);
+  if (chk != VLTR_OK) return chk;
+  const size_t n = static_cast<size_t>(offsets[n_structs]);
+  if (n > 0 && !d_balls) return fail(VLTR_ERR_INVALID_ARGUMENT, "null balls");
+  ctx->last_contacts = ctx->last_cells = 0;
+  if (n == 0) {
+    ctx->last_structs = n_structs;
+    ctx->last_n = 0;
+    return VLTR_OK;
+  }
+  VLTR_HIP(hipSetDevice(ctx->device));
+  VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
+  VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, d_balls, n * sizeof(vltr_ball), hipMemcpyDeviceToDevice, ctx->stream));
+  size_t nc = 0, ncell = 0;
+  const int rc = run_batched(ctx, offsets, n_structs, probe, contacts_only ? 1 : 0, &nc, &ncell);
+  if (rc != VLTR_OK) return rc;
+  ctx->last_contacts = nc;
+  ctx->last_cells = ncell;
+  return VLTR_OK;
+}
+
+int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64_t* contact_off, uint64_t* cell_off) {
+  if (!ctx || ctx->last_structs != n_structs || n_structs == 0)
+    return fail(VLTR_ERR_INVALID_ARGUMENT, "no batched run with that many structures on this context");
+  std::vector<uint32_t> so(2 * (n_structs + 1), 0u);
+  if (ctx->last_n > 0) {
+    VLTR_HIP(hipSetDevice(ctx->device));
+    VLTR_HIP(hipMemcpy(so.data(), ctx->seg_out.ptr, so.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  }
+  for (size_t k = 0; k <= n_structs; ++k) {
+    if (contact_off) contact_off[k] = so[k];
+    if (cell_off) cell_off[k] = ctx->last_cells > 0 ? so[n_structs + 1 + k] : 0;
+  }
+  return VLTR_OK;
+}
+
+size_t vltr_batch_num_structs(const vltr_batch_result* r) { return r ? r->n_structs : 0; }
+const vltr_contact* vltr_batch_contacts(const vltr_batch_result* r, size_t k, size_t* count) {
+  if (!r || k >= r->n_structs) {
+    if (count) *count = 0;
+    return nullptr;
+  }
+  if (count) *count = static_cast<size_t>(r->contact_off[k + 1] - r->contact_off[k]);
+  return r->contacts.data() + r->contact_off[k];
+}
+const vltr_cell* vltr_batch_cells(const vltr_batch_result* r, size_t k, size_t* count) {
+  if (!r || k >= r->n_structs) {
+    if (count) *count = 0;
+    return nullptr;
+  }
+  if (count) *count = static_cast<size_t>(r->cell_off[k + 1] - r->cell_off[k]);
+  return r->cells.data() + r->cell_off[k];
+}
+void vltr_batch_free(vltr_batch_result* r) { delete r; }
 
 const char* vltr_last_error(void) { return g_last_error.c_str(); }
 
@@ -477,7 +624,7 @@ void vltr_context_destroy(vltr_context* ctx) {
                     &ctx->sorted_id, &ctx->sorted_sph, &ctx->small, &ctx->nb, &ctx->nb_cnt, &ctx->excluded,
                     &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec, &ctx->cell, &ctx->overflow_list, &ctx->wide_list,
                     &ctx->flags, &ctx->scan_tmp, &ctx->out_contacts, &ctx->out_cells, &ctx->slow_ws,
-                    &ctx->in_cnt, &ctx->incoming})
+                    &ctx->in_cnt, &ctx->incoming, &ctx->seg_out})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)

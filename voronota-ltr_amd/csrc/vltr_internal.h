@@ -125,5 +125,9 @@ hipError_t launch_flag_contacts(const Pipeline& p, uint32_t* flag, hipStream_t s
 hipError_t launch_emit_contacts(const Pipeline& p, const uint32_t* pos, vltr_contact* out, hipStream_t s);
 hipError_t launch_flag_cells(const Pipeline& p, uint32_t* flag, hipStream_t s);
 hipError_t launch_emit_cells(const Pipeline& p, const uint32_t* pos, vltr_cell* out, hipStream_t s);
+// Batched runs: out[k] = position of structure k's first contact (what = 0,
+// pos = scanned contact flags) or first cell (what = 1, pos = scanned cell
+// flags) in the compacted output, k = 0..n_segs (out[n_segs] = total).
+hipError_t launch_seg_offsets(const Pipeline& p, const uint32_t* pos, int what, uint32_t* out, hipStream_t s);
 
 }  // namespace vltr

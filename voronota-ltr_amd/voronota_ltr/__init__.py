@@ -135,3 +135,18 @@ def compute_contacts_only(balls, probe, periodic_box=None, groups=None):
     g = None if groups is None else [int(x) for x in groups]
     r = _native.run(b, float(probe), pb, g, False, True)
     return _contacts_to_list(r.contacts)
+
+
+def compute_tessellation_batched(structures, probe, contacts_only=False):
+    """compute_tessellation (or compute_contacts_only) of many independent
+    structures in one pass on the GPU. Each element of `structures` is any
+    ball collection parse_balls accepts; returns one dict per structure in
+    the layout of compute_tessellation / compute_contacts_only."""
+    parts = _native.run_batched([parse_balls(s) for s in structures], probe, contacts_only=contacts_only)
+    out = []
+    for part in parts:
+        if contacts_only:
+            out.append(_contacts_to_list(part.contacts))
+        else:
+            out.append({"contacts": _contacts_to_list(part.contacts), "cells": _cells_to_list(part.cells)})
+    return out

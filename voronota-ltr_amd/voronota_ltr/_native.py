@@ -74,6 +74,13 @@ SIGNATURES = {
     "vltr_result_sas_areas": (None, [_vp, _f64p]),
     "vltr_result_volumes": (None, [_vp, _f64p]),
     "vltr_result_free": (None, [_vp]),
+    "vltr_compute_tessellation_batched": (C.c_int, [_vp, _vp, _u64p, _sz, C.c_double, C.c_int, C.POINTER(_vp)]),
+    "vltr_batch_num_structs": (_sz, [_vp]),
+    "vltr_batch_contacts": (_vp, [_vp, _sz, C.POINTER(_sz)]),
+    "vltr_batch_cells": (_vp, [_vp, _sz, C.POINTER(_sz)]),
+    "vltr_batch_free": (None, [_vp]),
+    "vltr_compute_tessellation_batched_device": (C.c_int, [_vp, _vp, _u64p, _sz, C.c_double, C.c_int]),
+    "vltr_context_batch_offsets": (C.c_int, [_vp, _sz, _u64p, _u64p]),
 }
 
 _lock = threading.Lock()
@@ -239,3 +246,56 @@ def run(balls, probe, pbox=None, groups=None, with_cell_vertices=False, contacts
         return Result(handle)
     finally:
         lib.vltr_result_free(handle)
+
+
+class BatchPart:
+    """One structure of a batched run: contacts and cells with local indices."""
+
+    def __init__(self, num_balls, contacts, cells):
+        self.num_balls = num_balls
+        self.contacts = contacts
+        self.cells = cells
+
+    def sas_areas(self):
+        out = np.full(self.num_balls, np.nan)
+        out[self.cells["index"].astype(np.int64)] = self.cells["sas_area"]
+        return out
+
+    def volumes(self):
+        out = np.full(self.num_balls, np.nan)
+        out[self.cells["index"].astype(np.int64)] = self.cells["volume"]
+        return out
+
+
+def offsets_of(structures):
+    sizes = [len(s) for s in structures]
+    off = np.zeros(len(sizes) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(sizes)
+    return off
+
+
+def run_batched(structures, probe, contacts_only=False, device=0):
+    """Tessellation of many independent structures in one pipeline pass; one
+    BatchPart per structure (equal to running each structure alone)."""
+    lib = load_library()
+    ctx = context(device)
+    arrays = [_balls_array(s) for s in structures]
+    if not arrays:
+        return []
+    off = offsets_of(arrays)
+    allb = np.ascontiguousarray(np.concatenate(arrays)) if int(off[-1]) else np.zeros((0, 4))
+    handle = C.c_void_p()
+    rc = lib.vltr_compute_tessellation_batched(ctx, allb.ctypes.data_as(_vp) if len(allb) else None,
+                                               off.ctypes.data_as(_u64p), len(arrays), float(probe),
+                                               int(bool(contacts_only)), C.byref(handle))
+    check(rc, "vltr_compute_tessellation_batched")
+    try:
+        parts = []
+        for k, arr in enumerate(arrays):
+            nc, ncell = C.c_size_t(), C.c_size_t()
+            cp = lib.vltr_batch_contacts(handle, k, C.byref(nc))
+            clp = lib.vltr_batch_cells(handle, k, C.byref(ncell))
+            parts.append(BatchPart(len(arr), _copy(cp, nc.value, CONTACT_DTYPE), _copy(clp, ncell.value, CELL_DTYPE)))
+        return parts
+    finally:
+        lib.vltr_batch_free(handle)

@@ -76,6 +76,7 @@ typedef struct vltr_cell_edge {
 typedef struct vltr_context vltr_context;
 typedef struct vltr_result vltr_result;
 typedef struct vltr_batch_result vltr_batch_result;
+typedef struct vltr_updateable vltr_updateable;
 
 /* ------------------------------------------------------------ context --- */
 
@@ -173,6 +174,44 @@ void vltr_batch_free(vltr_batch_result* r);
 int vltr_compute_tessellation_batched_device(vltr_context* ctx, const vltr_ball* d_balls, const uint64_t* offsets,
                                              size_t n_structs, double probe, int contacts_only);
 int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64_t* contact_off, uint64_t* cell_off);
+
+/* --------------------------------------------------------- updateable --- */
+
+/* UpdateableTessellation (updateable.rs:154-742) on the device: a
+ * tessellation kept up to date as balls move. Each handle owns its own
+ * context (device buffers, stream); calls on one handle must be serialized.
+ * Functions returning an int status report the reference's bool in *ok. */
+/* UpdateableTessellation::new / with_backup (updateable.rs:164-182) */
+int vltr_updateable_create(int device, int backup_enabled, vltr_updateable** out);
+void vltr_updateable_destroy(vltr_updateable* u);
+/* init (updateable.rs:187-202); *ok = a non-empty result was produced */
+int vltr_updateable_init(vltr_updateable* u, const vltr_ball* balls, size_t n, double probe,
+                         const vltr_periodic_box* periodic_box, int* ok);
+/* update (updateable.rs:208-213): changed balls found by comparison
+ * (sphere_equals_sphere); *ok = something changed */
+int vltr_updateable_update(vltr_updateable* u, const vltr_ball* balls, size_t n, int* ok);
+/* update_with_changed (updateable.rs:215-220): only the listed ids are
+ * compared; an empty list is "no update" */
+int vltr_updateable_update_with_changed(vltr_updateable* u, const vltr_ball* balls, size_t n, const uint64_t* changed,
+                                        size_t n_changed, int* ok);
+/* set_exclusion (updateable.rs:225-272) */
+int vltr_updateable_set_exclusion(vltr_updateable* u, size_t id, int excluded, int* ok);
+/* restore (updateable.rs:277-322): back to the state before the last
+ * init / update / set_exclusion; *ok = 0 without backup */
+int vltr_updateable_restore(vltr_updateable* u, int* ok);
+/* summary (updateable.rs:333-376) as a result handle (free with
+ * vltr_result_free); contacts listed per first ball, periodic ids canonical
+ * and ordered */
+int vltr_updateable_summary(const vltr_updateable* u, vltr_result** out);
+/* changed_ids / affected_ids (updateable.rs:378-392): total count returned,
+ * at most max_ids written */
+size_t vltr_updateable_changed_ids(const vltr_updateable* u, uint64_t* out, size_t max_ids);
+size_t vltr_updateable_affected_ids(const vltr_updateable* u, uint64_t* out, size_t max_ids);
+int vltr_updateable_last_update_was_full_reinit(const vltr_updateable* u);
+int vltr_updateable_backup_enabled(const vltr_updateable* u);
+/* Phase timings of the handle's last pipeline run (as
+ * vltr_context_last_timings). */
+int vltr_updateable_last_timings(const vltr_updateable* u, double* ms, int max_entries);
 
 /* ------------------------------------------------------------ results --- */
 

@@ -1,0 +1,193 @@
+"""The device UpdateableTessellation against the oracle's restatement of
+updateable.rs / spheres_container.rs, operation by operation: the returned
+booleans, changed and affected ids and the full-reinit flag exactly; the
+summary's contact set and cell set exactly, their values to the parity bar
+(1e-9 relative with the full-sphere / full-disk floors). Scenarios follow the
+reference's tests/updateable.rs:63-321 and updateable.rs:754-816, plus seeded
+random moves of dense and periodic clusters.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import RING17, corners_box, load_xyzr
+
+pytestmark = pytest.mark.gpu
+
+LINE3 = np.array([[0, 0, 0, 1.0], [2, 0, 0, 1.0], [4, 0, 0, 1.0]])
+
+
+def _close(a, b, scale, what):
+    tol = 1e-9 * abs(b) + 1e-12 * scale
+    assert abs(a - b) <= tol, f"{what}: gpu {a!r} oracle {b!r}"
+
+
+def _same_summary(gpu, ref, balls, probe):
+    n = len(balls)
+    R = np.asarray(balls, float)[:, 3] + probe if n else np.zeros(0)
+    g = {(int(c["id_a"]), int(c["id_b"])): c for c in gpu.contacts}
+    r = {(int(a), int(b)): k for k, (a, b) in enumerate(ref.contact_ids)}
+    assert g.keys() == r.keys(), f"contact sets differ: {sorted(set(g) ^ set(r))[:5]}"
+    for key, k in r.items():
+        rm = max(R[key[0]], R[key[1]])
+        _close(g[key]["area"], ref.contact_area[k], math.pi * rm * rm, f"area {key}")
+        _close(g[key]["arc_length"], ref.contact_arc[k], 2 * math.pi * rm, f"arc {key}")
+        assert bool(g[key]["central"]) == bool(ref.contact_central[k]), f"central {key}"
+    assert [int(i) for i in gpu.cells["index"]] == [int(i) for i in ref.cell_index]
+    for c, sas, vol in zip(gpu.cells, ref.cell_sas, ref.cell_volume):
+        Ri = R[int(c["index"])]
+        _close(c["sas_area"], sas, 4 * math.pi * Ri * Ri, f"sas {int(c['index'])}")
+        _close(c["volume"], vol, 4.0 / 3.0 * math.pi * Ri ** 3, f"volume {int(c['index'])}")
+
+
+class Pair:
+    """Runs every operation on the device updateable and on the oracle's."""
+
+    def __init__(self, oracle, with_backup=False):
+        from voronota_ltr import _native
+
+        self.g = _native.Updateable(with_backup=with_backup)
+        self.o = oracle.Updateable(with_backup=with_backup)
+        self.balls = None
+        self.probe = None
+
+    def check(self):
+        assert [int(i) for i in self.g.changed_ids()] == [int(i) for i in self.o.changed_ids()]
+        assert [int(i) for i in self.g.affected_ids()] == [int(i) for i in self.o.affected_ids()]
+        assert self.g.last_update_was_full_reinit() == self.o.last_update_was_full_reinit()
+        _same_summary(self.g.summary(), self.o.summary(), self.balls, self.probe)
+
+    def op(self, name, *args):
+        a = getattr(self.g, name)(*args)
+        b = getattr(self.o, name)(*args)
+        assert a == b, f"{name} returned {a} on the GPU, {b} in the oracle"
+        self.check()
+        return a
+
+    def init(self, balls, probe, pbox=None):
+        self.balls = np.array(balls, float)
+        self.probe = probe
+        return self.op("init", self.balls, probe, pbox)
+
+    def update(self, balls, changed=None):
+        self.balls = np.array(balls, float)
+        if changed is None:
+            return self.op("update", self.balls)
+        return self.op("update_with_changed", self.balls, list(changed))
+
+
+def _moves(p, balls, changed, iters):
+    balls = balls.copy()
+    for it in range(iters):
+        for i in changed:
+            balls[i, 0] += 0.05 * (it + 1.0)
+        print(f"[gpu] updateable step {it}: {len(changed)} changed of {len(balls)}", flush=True)
+        assert p.update(balls, changed)
+    return balls
+
+
+def test_ring17_and_restore(oracle):
+    p = Pair(oracle, with_backup=True)
+    assert p.init(RING17, 1.0)
+    _moves(p, RING17, [0, 8], 3)
+    assert p.op("restore")
+    assert not p.op("restore")
+
+
+def test_three_line(oracle):
+    p = Pair(oracle)
+    assert p.init(LINE3, 0.5)
+    _moves(p, LINE3, [1], 5)
+
+
+def test_ring17_periodic(oracle):
+    p = Pair(oracle, with_backup=True)
+    box = corners_box((-1.6, -1.6, -0.6), (1.6, 1.6, 3.1))
+    assert p.init(RING17, 1.0, box)
+    balls = RING17.copy()
+    for _ in range(5):
+        balls[[0, 1], 0] += 0.1
+        assert p.update(balls, [0, 1])
+    assert p.op("restore")
+
+
+def test_exclusion_and_restore(oracle):
+    p = Pair(oracle, with_backup=True)
+    assert p.init(LINE3, 0.5)
+    assert p.op("set_exclusion", 1, True)
+    assert not p.op("set_exclusion", 1, True)
+    assert p.op("restore")
+    assert p.op("set_exclusion", 0, True)
+    assert p.op("set_exclusion", 0, False)
+    assert not p.op("set_exclusion", 7, True)
+
+
+def test_auto_detect_no_change_and_reinit(oracle):
+    p = Pair(oracle)
+    b = np.array([[0, 0, 0, 1.0], [2, 0, 0, 1.0]])
+    assert p.init(b, 0.5)
+    b2 = b.copy()
+    b2[0, 0] += 0.1
+    assert p.update(b2)
+    assert not p.update(b2)
+    assert not p.update(b2, [])
+    # below the comparison tolerance: not a change
+    b3 = b2.copy()
+    b3[1, 1] += 1e-12
+    assert not p.update(b3, [1])
+    # more than half the balls changed: full reinit
+    n = 20
+    line = np.array([[2.0 * i, 0, 0, 1.0] for i in range(n)])
+    q = Pair(oracle)
+    assert q.init(line, 0.5)
+    m = line.copy()
+    m[: n // 2 + 1, 0] += 0.1
+    assert q.update(m, list(range(n // 2 + 1)))
+    assert q.g.last_update_was_full_reinit()
+    # a different ball count: full reinit
+    assert q.update(line[:7])
+
+
+def test_2zsk_incremental(oracle):
+    p = Pair(oracle)
+    balls = load_xyzr("balls_2zsk.xyzr")
+    assert p.init(balls, 1.4)
+    b = balls.copy()
+    b[[0, 100, 500], 0] += 0.1
+    assert p.update(b, [0, 100, 500])
+    assert not p.g.last_update_was_full_reinit()
+
+
+def test_rsa_random_moves(oracle):
+    from voronota_ltr import workload
+
+    balls, _ = workload.rsa_sphere(3000, seed=77)
+    rng = np.random.default_rng(1)
+    p = Pair(oracle, with_backup=True)
+    assert p.init(balls, 1.4)
+    for step in range(4):
+        ch = np.sort(rng.choice(len(balls), 15, replace=False))
+        balls[ch, :3] += rng.normal(scale=0.05, size=(15, 3))
+        print(f"[gpu] rsa step {step}", flush=True)
+        assert p.update(balls, ch)
+    assert p.op("restore")
+    assert p.op("set_exclusion", 5, True)
+
+
+def test_periodic_box_moves(oracle):
+    """A small version of the periodic dynamics workload: RSA in a cube,
+    balls displaced by N(0, 0.05^2) per axis and wrapped back into the box."""
+    from voronota_ltr import workload
+
+    L = 32.0
+    balls, _ = workload.rsa_box(1500, L, seed=11)
+    box = corners_box((0, 0, 0), (L, L, L))
+    rng = np.random.default_rng(4)
+    p = Pair(oracle)
+    assert p.init(balls, 1.4, box)
+    for step in range(3):
+        ch = np.sort(rng.choice(len(balls), 40, replace=False))
+        balls[ch, :3] = np.mod(balls[ch, :3] + rng.normal(scale=0.05, size=(40, 3)), L)
+        print(f"[gpu] periodic step {step}", flush=True)
+        assert p.update(balls, ch)

@@ -175,6 +175,58 @@ __global__ void k_seg_offsets(Pipeline p, const uint32_t* pos, int what, uint32_
   out[k] = what == 0 ? pos[p.pair_cnt[ball]] : pos[ball];
 }
 
+// updateable.rs:633-727 with the container's exclusion statuses excl
+// (spheres_container.rs: hidden balls, and balls excluded by the caller).
+// After init / full reinit (full != 0) a ball without contacts is detached
+// only if it is not excluded and has no colliding neighbor; after an
+// incremental update an affected ball without contacts is detached whenever
+// it is not excluded, and unaffected balls keep their cells. refresh != 0
+// (init and updates, not set_exclusion) resets the exclusion status of every
+// recomputed ball to its hidden status, as the collision update does.
+__global__ void k_updateable_cells(Pipeline p, CellRec* state, uint8_t* excl, const uint8_t* affected, int full,
+                                   int refresh) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= p.n) return;
+  if (!full && !affected[i]) {
+    p.cell[i] = state[i];
+    return;
+  }
+  if (refresh) excl[i] = p.excluded[i];
+  const CellRec rec = p.cell[i];
+  const bool from_contacts = rec.stage == 2 && p.nb_cnt[i] > 0;
+  CellRec out = rec;
+  if (!from_contacts) {
+    out.sas = 0.0;
+    out.volume = 0.0;
+    out.stage = 0;
+    if (!excl[i] && (!full || p.nb_cnt[i] == 0)) {
+      // types.rs:666-675
+      const double r = p.sph[i].w;
+      out.sas = 4.0 * kPi * r * r;
+      out.volume = out.sas * r / 3.0;
+      out.stage = 2;
+    }
+  }
+  p.cell[i] = out;
+  state[i] = out;
+}
+
+// spheres_container.rs:150-170: the changed balls' new colliding neighbors
+// (canonical ids) join the affected set.
+__global__ void k_mark_more(Pipeline p, const int32_t* changed, int64_t k, uint8_t* affected) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= k * p.nb_cap) return;
+  const int64_t r = t / p.nb_cap, j = t - r * p.nb_cap;
+  const int32_t id = changed[r];
+  if (id < 0 || id >= p.n || j >= p.nb_cnt[id]) return;
+  affected[p.nb[static_cast<int64_t>(id) * p.nb_cap + j] % p.n] = 1;
+}
+
+__global__ void k_mask_clear(uint8_t* mask, const uint8_t* where, int64_t n) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n && where[i]) mask[i] = 0;
+}
+
 inline int64_t blocks_for(int64_t n, int64_t per) { return (n + per - 1) / per; }
 
 }  // namespace
@@ -199,6 +251,23 @@ hipError_t launch_flag_cells(const Pipeline& p, uint32_t* flag, hipStream_t s) {
 }
 hipError_t launch_emit_cells(const Pipeline& p, const uint32_t* pos, vltr_cell* out, hipStream_t s) {
   if (p.n > 0) hipLaunchKernelGGL(k_emit_cells, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p, pos, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_updateable_cells(const Pipeline& p, CellRec* state, uint8_t* excl, const uint8_t* affected, int full,
+                                   int refresh, hipStream_t s) {
+  if (p.n > 0)
+    hipLaunchKernelGGL(k_updateable_cells, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p, state, excl, affected, full,
+                       refresh);
+  return hipGetLastError();
+}
+hipError_t launch_mark_more(const Pipeline& p, const int32_t* changed, int64_t k, uint8_t* affected, hipStream_t s) {
+  if (k > 0 && p.n > 0)
+    hipLaunchKernelGGL(k_mark_more, dim3(blocks_for(k * p.nb_cap, 256)), dim3(256), 0, s, p, changed, k, affected);
+  return hipGetLastError();
+}
+hipError_t launch_mask_clear(uint8_t* mask, const uint8_t* where, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_mask_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, mask, where, n);
   return hipGetLastError();
 }
 

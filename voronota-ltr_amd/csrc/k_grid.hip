@@ -277,6 +277,20 @@ __global__ void k_pair_fill(Pipeline p) {
   }
 }
 
+__global__ void k_gather_rows(Pipeline p, const int32_t* ids, int64_t k, int32_t* cnt, int32_t* rows) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= k * p.nb_cap) return;
+  const int64_t r = t / p.nb_cap, j = t - r * p.nb_cap;
+  const int32_t id = ids[r];
+  if (id < 0 || id >= p.n) {
+    if (j == 0) cnt[r] = 0;
+    return;
+  }
+  const int c = p.nb_cnt[id];
+  if (j == 0) cnt[r] = c;
+  if (j < c) rows[t] = p.nb[static_cast<int64_t>(id) * p.nb_cap + j];
+}
+
 // ------------------------------------------------------------------ scan ----
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
@@ -371,6 +385,13 @@ hipError_t launch_prepare(const Pipeline& p, hipStream_t s) {
   hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, s, p);
   if (p.n > 0) hipLaunchKernelGGL(k_prepare, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p);
   if (p.periodic && p.n > 0) hipLaunchKernelGGL(k_populate, dim3(blocks_for(p.n * 26, 256)), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const Pipeline& p, const int32_t* ids, int64_t k, int32_t* cnt, int32_t* rows,
+                              hipStream_t s) {
+  if (k > 0)
+    hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(k * p.nb_cap, 256)), dim3(256), 0, s, p, ids, k, cnt, rows);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <unordered_map>
 #include <memory>
@@ -196,9 +197,16 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
 // Runs the device pipeline on ctx->balls (already on the device). On return
 // the compacted contacts/cells are in ctx->out_contacts / out_cells and their
 // counts in *n_contacts / *n_cells.
+// Extra inputs of the updateable path: balls excluded by the caller, and a
+// step run on the cell records before they are compacted.
+struct RunExtras {
+  const uint8_t* user_excl = nullptr;
+  std::function<hipError_t(const Pipeline&, hipStream_t)> post_cells;
+};
+
 int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic_box* pbox, const int32_t* d_groups,
                  int64_t n_groups, int mode, size_t* n_contacts, size_t* n_cells, const uint64_t* seg_offsets = nullptr,
-                 int64_t n_structs = 0) {
+                 int64_t n_structs = 0, const RunExtras* extras = nullptr) {
   hipStream_t s = ctx->stream;
   const bool periodic = pbox != nullptr;
   const int64_t n_pop = periodic ? 27 * n : n;
@@ -217,6 +225,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   }
   p.groups = d_groups;
   p.n_groups = n_groups;
+  p.user_excl = extras ? extras->user_excl : nullptr;
 
   // One cell-hash segment per structure: its own bucket range, sized for it.
   std::vector<GridSeg> segs(static_cast<size_t>(p.n_segs));
@@ -302,6 +311,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(launch_fallbacks(p, s));
   VLTR_HIP(hipEventRecord(ctx->ev[5], s));
   if (mode == 0) VLTR_HIP(launch_cells(p, s));
+  if (mode == 0 && extras && extras->post_cells) VLTR_HIP(extras->post_cells(p, s));
   uint32_t* flags = ctx->flags.as<uint32_t>();
   VLTR_HIP(launch_flag_contacts(p, flags, s));
   VLTR_HIP(launch_scan_u32(flags, P, ctx->scan_tmp.as<uint32_t>(), s));
@@ -764,5 +774,425 @@ void vltr_result_volumes(const vltr_result* r, double* out) {
     if (c.index < r->num_balls) out[c.index] = c.volume;
 }
 void vltr_result_free(vltr_result* r) { delete r; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------ updateable ----
+// UpdateableTessellation (updateable.rs:154-742, spheres_container.rs:28-405)
+// on the device. Every state transition (changed and affected sets, the
+// full-reinit thresholds, exclusion statuses, backup and restore) follows the
+// reference; the tessellation itself is recomputed for all pairs on each
+// step. That is exact: a pair whose ends are both unaffected has the same
+// inputs as before, so its record is bit-identical to the summary the
+// reference keeps, while pairs with two affected ends are re-clipped in both.
+// Cells follow the reference's incremental rules (k_updateable_cells): only
+// affected balls take new records.
+struct vltr_updateable {
+  vltr_context* ctx = nullptr;
+  bool backup_enabled = false;
+  struct State {
+    double probe = 0.0;
+    bool periodic = false;
+    vltr_periodic_box pbox{};
+    std::vector<vltr_ball> balls;  // the container's balls
+    std::vector<uint64_t> changed, affected;
+    bool was_full_reinit = true;
+    bool empty = true;  // updateable.rs result_empty(): no balls or no cells
+    size_t n_contacts = 0, n_cells = 0;
+  };
+  State st, bk;
+  bool has_backup = false;
+  // cells and exclusion statuses of the current state and of the backup
+  DevBuf cells_state, cells_backup, excl, excl_backup, affected_mask, ids, rows, rows_cnt;
+};
+
+namespace {
+
+using Upd = vltr_updateable;
+
+// sphere_equals_sphere (geometry.rs:69-72) on Sphere::from_ball (types.rs:58-63)
+bool same_sphere(const vltr_ball& a, const vltr_ball& b, double probe) {
+  return equals(Sph{V3{a.x, a.y, a.z}, a.r + probe}, Sph{V3{b.x, b.y, b.z}, b.r + probe});
+}
+
+// Canonical colliding neighbors of the balls ids, from the last pipeline run.
+int upd_rows(Upd* u, const std::vector<uint64_t>& ids, std::vector<std::vector<uint64_t>>* out) {
+  vltr_context* ctx = u->ctx;
+  const size_t n = u->st.balls.size();
+  out->assign(ids.size(), {});
+  if (ids.empty() || n == 0 || ctx->last_n != n) return VLTR_OK;
+  const int cap = ctx->last_nb_cap;
+  std::vector<int32_t> id32(ids.begin(), ids.end());
+  VLTR_HIP(u->ids.ensure(id32.size() * sizeof(int32_t)));
+  VLTR_HIP(u->rows.ensure(id32.size() * cap * sizeof(int32_t)));
+  VLTR_HIP(u->rows_cnt.ensure(id32.size() * sizeof(int32_t)));
+  hipStream_t s = ctx->stream;
+  VLTR_HIP(hipMemcpyAsync(u->ids.ptr, id32.data(), id32.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  Pipeline p{};
+  p.n = static_cast<int64_t>(n);
+  p.nb = ctx->nb.as<int32_t>();
+  p.nb_cnt = ctx->nb_cnt.as<int32_t>();
+  p.nb_cap = cap;
+  VLTR_HIP(launch_gather_rows(p, u->ids.as<int32_t>(), static_cast<int64_t>(id32.size()), u->rows_cnt.as<int32_t>(),
+                              u->rows.as<int32_t>(), s));
+  std::vector<int32_t> cnt(id32.size()), rows(id32.size() * cap);
+  VLTR_HIP(hipMemcpyAsync(cnt.data(), u->rows_cnt.ptr, cnt.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipMemcpyAsync(rows.data(), u->rows.ptr, rows.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipStreamSynchronize(s));
+  for (size_t k = 0; k < ids.size(); ++k)
+    for (int j = 0; j < cnt[k]; ++j) (*out)[k].push_back(static_cast<uint64_t>(rows[k * cap + j]) % n);
+  return VLTR_OK;
+}
+
+// One tessellation of the current balls. full: init / full reinit rules;
+// otherwise only balls in aff take new cells. refresh: affected balls get
+// their hidden status back as exclusion status (collision update). more:
+// balls whose new neighbors join the affected set.
+int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, const std::vector<uint64_t>& more) {
+  vltr_context* ctx = u->ctx;
+  auto& st = u->st;
+  const size_t n = st.balls.size();
+  st.n_contacts = st.n_cells = 0;
+  if (n == 0) {
+    st.empty = true;
+    ctx->last_n = 0;
+    return VLTR_OK;
+  }
+  hipStream_t s = ctx->stream;
+  VLTR_HIP(hipSetDevice(ctx->device));
+  VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
+  VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, st.balls.data(), n * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
+  VLTR_HIP(u->cells_state.ensure(n * sizeof(CellRec)));
+  VLTR_HIP(u->excl.ensure(n));
+  VLTR_HIP(u->affected_mask.ensure(n));
+  std::vector<uint8_t> mask;
+  if (full) {
+    VLTR_HIP(hipMemsetAsync(u->excl.ptr, 0, n, s));
+  } else {
+    mask.assign(n, 0);
+    for (uint64_t id : aff)
+      if (id < n) mask[id] = 1;
+    VLTR_HIP(hipMemcpyAsync(u->affected_mask.ptr, mask.data(), n, hipMemcpyHostToDevice, s));
+    if (refresh) VLTR_HIP(launch_mask_clear(u->excl.as<uint8_t>(), u->affected_mask.as<uint8_t>(),
+                                            static_cast<int64_t>(n), s));
+  }
+  std::vector<int32_t> more32(more.begin(), more.end());
+  if (!more32.empty()) {
+    VLTR_HIP(u->ids.ensure(more32.size() * sizeof(int32_t)));
+    VLTR_HIP(hipMemcpyAsync(u->ids.ptr, more32.data(), more32.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  }
+  RunExtras ex;
+  ex.user_excl = full ? nullptr : u->excl.as<uint8_t>();
+  CellRec* state = u->cells_state.as<CellRec>();
+  uint8_t* excl = u->excl.as<uint8_t>();
+  uint8_t* am = u->affected_mask.as<uint8_t>();
+  const int32_t* more_ids = u->ids.as<int32_t>();
+  const int64_t n_more = static_cast<int64_t>(more32.size());
+  ex.post_cells = [=](const Pipeline& p, hipStream_t st2) -> hipError_t {
+    if (n_more > 0) {
+      const hipError_t e = launch_mark_more(p, more_ids, n_more, am, st2);
+      if (e != hipSuccess) return e;
+    }
+    return launch_updateable_cells(p, state, excl, full ? nullptr : am, full ? 1 : 0, refresh ? 1 : 0, st2);
+  };
+  size_t nc = 0, ncell = 0;
+  const int rc = run_pipeline(ctx, static_cast<int64_t>(n), st.probe, st.periodic ? &st.pbox : nullptr, nullptr, 0, 0,
+                              &nc, &ncell, nullptr, 0, &ex);
+  if (rc != VLTR_OK) return rc;
+  st.n_contacts = nc;
+  st.n_cells = ncell;
+  st.empty = ncell == 0;
+  return VLTR_OK;
+}
+
+// updateable.rs:404-420
+int upd_prepare(Upd* u) {
+  if (u->backup_enabled && !u->st.empty) {
+    u->bk = u->st;
+    const size_t n = u->st.balls.size();
+    hipStream_t s = u->ctx->stream;
+    VLTR_HIP(u->cells_backup.ensure(n * sizeof(CellRec)));
+    VLTR_HIP(u->excl_backup.ensure(n));
+    VLTR_HIP(hipMemcpyAsync(u->cells_backup.ptr, u->cells_state.ptr, n * sizeof(CellRec), hipMemcpyDeviceToDevice, s));
+    VLTR_HIP(hipMemcpyAsync(u->excl_backup.ptr, u->excl.ptr, n, hipMemcpyDeviceToDevice, s));
+    u->has_backup = true;
+  }
+  u->st.changed.clear();
+  u->st.affected.clear();
+  u->st.was_full_reinit = false;
+  return VLTR_OK;
+}
+
+int upd_full(Upd* u, const vltr_ball* balls, size_t n) {
+  u->st.balls.assign(balls, balls + n);
+  u->st.changed.clear();
+  u->st.affected.clear();
+  u->st.was_full_reinit = true;
+  return upd_run(u, true, true, {}, {});
+}
+
+// updateable.rs:422-447 with spheres_container.rs:71-182
+int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* changed_ids, size_t n_changed, int* ok) {
+  *ok = 0;
+  int rc = upd_prepare(u);
+  if (rc != VLTR_OK) return rc;
+  auto& st = u->st;
+  if (nb != st.balls.size()) {
+    rc = upd_full(u, balls, nb);
+    if (rc == VLTR_OK) *ok = 1;
+    return rc;
+  }
+  const size_t n = nb;
+  const size_t threshold = std::max<size_t>(n / 2, 10);
+  std::vector<uint64_t> changed;
+  if (changed_ids) {
+    for (size_t k = 0; k < n_changed; ++k) {
+      const uint64_t id = changed_ids[k];
+      if (id < n && !same_sphere(balls[id], st.balls[id], st.probe)) changed.push_back(id);
+    }
+  } else {
+    for (size_t i = 0; i < n && changed.size() < threshold + 1; ++i)
+      if (!same_sphere(balls[i], st.balls[i], st.probe)) changed.push_back(i);
+  }
+  if (changed.empty()) return VLTR_OK;
+  if (changed.size() > threshold) {
+    rc = upd_full(u, balls, nb);
+    if (rc == VLTR_OK) *ok = 1;
+    return rc;
+  }
+  std::vector<uint64_t> affected = changed;
+  std::sort(affected.begin(), affected.end());
+  std::vector<std::vector<uint64_t>> old_rows;
+  rc = upd_rows(u, changed, &old_rows);
+  if (rc != VLTR_OK) return rc;
+  for (const auto& row : old_rows)
+    for (uint64_t cid : row) {
+      auto it = std::lower_bound(affected.begin(), affected.end(), cid);
+      if (it == affected.end() || *it != cid) {
+        if (affected.size() < threshold) {
+          affected.insert(it, cid);
+        } else {
+          rc = upd_full(u, balls, nb);
+          if (rc == VLTR_OK) *ok = 1;
+          return rc;
+        }
+      }
+    }
+  for (uint64_t id : changed) st.balls[id] = balls[id];
+  rc = upd_run(u, false, true, affected, changed);
+  if (rc != VLTR_OK) return rc;
+  std::vector<std::vector<uint64_t>> new_rows;
+  rc = upd_rows(u, changed, &new_rows);
+  if (rc != VLTR_OK) return rc;
+  std::vector<uint64_t> more;
+  for (const auto& row : new_rows)
+    for (uint64_t cid : row)
+      if (!std::binary_search(affected.begin(), affected.end(), cid)) {
+        auto it = std::lower_bound(more.begin(), more.end(), cid);
+        if (it == more.end() || *it != cid) more.insert(it, cid);
+      }
+  if (!more.empty()) {
+    affected.insert(affected.end(), more.begin(), more.end());
+    std::sort(affected.begin(), affected.end());
+    affected.erase(std::unique(affected.begin(), affected.end()), affected.end());
+  }
+  st.changed = changed;
+  st.affected = affected;
+  st.was_full_reinit = false;
+  *ok = 1;
+  return VLTR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vltr_updateable_create(int device, int backup_enabled, vltr_updateable** out) {
+  if (!out) return fail(VLTR_ERR_INVALID_ARGUMENT, "null out");
+  *out = nullptr;
+  auto u = std::make_unique<vltr_updateable>();
+  const int rc = vltr_context_create(device, &u->ctx);
+  if (rc != VLTR_OK) return rc;
+  u->backup_enabled = backup_enabled != 0;
+  *out = u.release();
+  return VLTR_OK;
+}
+
+void vltr_updateable_destroy(vltr_updateable* u) {
+  if (!u) return;
+  if (u->ctx) {
+    (void)hipSetDevice(u->ctx->device);
+    (void)hipStreamSynchronize(u->ctx->stream);
+    for (DevBuf* b : {&u->cells_state, &u->cells_backup, &u->excl, &u->excl_backup, &u->affected_mask, &u->ids, &u->rows,
+                      &u->rows_cnt})
+      b->release();
+    vltr_context_destroy(u->ctx);
+  }
+  delete u;
+}
+
+// updateable.rs:187-202
+int vltr_updateable_init(vltr_updateable* u, const vltr_ball* balls, size_t n, double probe,
+                         const vltr_periodic_box* periodic_box, int* ok) {
+  if (!u || !ok || (n > 0 && !balls)) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  if (n > (size_t(1) << 31) / 27) return fail(VLTR_ERR_INVALID_ARGUMENT, "too many balls");
+  *ok = 0;
+  int rc = upd_prepare(u);
+  if (rc != VLTR_OK) return rc;
+  u->st.probe = probe;
+  u->st.periodic = periodic_box != nullptr;
+  if (periodic_box) u->st.pbox = *periodic_box;
+  rc = upd_full(u, balls, n);
+  if (rc != VLTR_OK) return rc;
+  *ok = u->st.empty ? 0 : 1;
+  return VLTR_OK;
+}
+
+// updateable.rs:208-213
+int vltr_updateable_update(vltr_updateable* u, const vltr_ball* balls, size_t n, int* ok) {
+  if (!u || !ok || (n > 0 && !balls)) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  return upd_update(u, balls, n, nullptr, 0, ok);
+}
+
+// updateable.rs:215-220
+int vltr_updateable_update_with_changed(vltr_updateable* u, const vltr_ball* balls, size_t n, const uint64_t* changed,
+                                        size_t n_changed, int* ok) {
+  if (!u || !ok || (n > 0 && !balls) || (n_changed > 0 && !changed))
+    return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  *ok = 0;
+  if (n_changed == 0) return VLTR_OK;
+  return upd_update(u, balls, n, changed, n_changed, ok);
+}
+
+// updateable.rs:225-272 with spheres_container.rs:186-208
+int vltr_updateable_set_exclusion(vltr_updateable* u, size_t id, int excluded, int* ok) {
+  if (!u || !ok) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  *ok = 0;
+  auto& st = u->st;
+  const size_t n = st.balls.size();
+  if (st.empty || id >= n) return VLTR_OK;
+  vltr_context* ctx = u->ctx;
+  hipStream_t s = ctx->stream;
+  VLTR_HIP(hipSetDevice(ctx->device));
+  uint8_t cur = 0;
+  VLTR_HIP(hipMemcpyAsync(&cur, u->excl.as<uint8_t>() + id, 1, hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipStreamSynchronize(s));
+  if ((cur != 0) == (excluded != 0)) return VLTR_OK;
+  // affected: the ball and its contact partners (exclusion) or its colliding
+  // neighbors (inclusion), read before the state changes
+  std::vector<uint64_t> affected{static_cast<uint64_t>(id)};
+  std::vector<uint64_t> partners;
+  if (excluded) {
+    std::vector<vltr_contact> cs(st.n_contacts);
+    if (!cs.empty()) {
+      VLTR_HIP(hipMemcpyAsync(cs.data(), ctx->out_contacts.ptr, cs.size() * sizeof(vltr_contact),
+                              hipMemcpyDeviceToHost, s));
+      VLTR_HIP(hipStreamSynchronize(s));
+    }
+    for (const auto& c : cs) {
+      const uint64_t a = c.id_a % n, b = c.id_b % n;
+      if (a == id) partners.push_back(b);
+      if (b == id) partners.push_back(a);
+    }
+  } else {
+    std::vector<std::vector<uint64_t>> rows;
+    const int rc = upd_rows(u, {static_cast<uint64_t>(id)}, &rows);
+    if (rc != VLTR_OK) return rc;
+    partners = rows[0];
+  }
+  for (uint64_t nb : partners) {
+    auto it = std::lower_bound(affected.begin(), affected.end(), nb);
+    if (it == affected.end() || *it != nb) affected.insert(it, nb);
+  }
+  int rc = upd_prepare(u);
+  if (rc != VLTR_OK) return rc;
+  const uint8_t v = excluded ? 1 : 0;
+  VLTR_HIP(hipMemcpyAsync(u->excl.as<uint8_t>() + id, &v, 1, hipMemcpyHostToDevice, s));
+  st.changed = {static_cast<uint64_t>(id)};
+  st.affected = affected;
+  st.was_full_reinit = false;
+  rc = upd_run(u, false, false, affected, {});
+  if (rc != VLTR_OK) return rc;
+  *ok = 1;
+  return VLTR_OK;
+}
+
+// updateable.rs:277-322: the state before the last update, exclusion change
+// or init comes back whole.
+int vltr_updateable_restore(vltr_updateable* u, int* ok) {
+  if (!u || !ok) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  *ok = 0;
+  if (!u->backup_enabled || !u->has_backup) return VLTR_OK;
+  u->st = u->bk;
+  u->has_backup = false;
+  const size_t n = u->st.balls.size();
+  hipStream_t s = u->ctx->stream;
+  VLTR_HIP(hipSetDevice(u->ctx->device));
+  VLTR_HIP(u->cells_state.ensure(n * sizeof(CellRec)));
+  VLTR_HIP(u->excl.ensure(n));
+  VLTR_HIP(hipMemcpyAsync(u->cells_state.ptr, u->cells_backup.ptr, n * sizeof(CellRec), hipMemcpyDeviceToDevice, s));
+  VLTR_HIP(hipMemcpyAsync(u->excl.ptr, u->excl_backup.ptr, n, hipMemcpyDeviceToDevice, s));
+  const auto kept = u->st;
+  // contacts are recomputed from the restored balls and exclusions; every
+  // cell keeps its restored record
+  const int rc = upd_run(u, false, false, {}, {});
+  if (rc != VLTR_OK) return rc;
+  u->st.changed = kept.changed;
+  u->st.affected = kept.affected;
+  u->st.was_full_reinit = kept.was_full_reinit;
+  *ok = 1;
+  return VLTR_OK;
+}
+
+// updateable.rs:333-376: contacts listed from each ball's view with
+// id_a == ball (periodic: the first of each canonical pair, ids ordered).
+int vltr_updateable_summary(const vltr_updateable* u, vltr_result** out) {
+  if (!u || !out) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  auto r = std::make_unique<vltr_result>();
+  const size_t n = u->st.balls.size();
+  r->num_balls = n;
+  if (n > 0) {
+    vltr_context* ctx = u->ctx;
+    hipStream_t s = ctx->stream;
+    VLTR_HIP(hipSetDevice(ctx->device));
+    r->contacts.resize(u->st.n_contacts);
+    r->cells.resize(u->st.n_cells);
+    if (!r->contacts.empty())
+      VLTR_HIP(hipMemcpyAsync(r->contacts.data(), ctx->out_contacts.ptr, r->contacts.size() * sizeof(vltr_contact),
+                              hipMemcpyDeviceToHost, s));
+    if (!r->cells.empty())
+      VLTR_HIP(hipMemcpyAsync(r->cells.data(), ctx->out_cells.ptr, r->cells.size() * sizeof(vltr_cell),
+                              hipMemcpyDeviceToHost, s));
+    VLTR_HIP(hipStreamSynchronize(s));
+    if (u->st.periodic) {
+      periodic_dedup_full(r->contacts, n);
+      for (auto& c : r->contacts)
+        if (c.id_a > c.id_b) std::swap(c.id_a, c.id_b);
+      std::stable_sort(r->contacts.begin(), r->contacts.end(),
+                       [](const vltr_contact& x, const vltr_contact& y) { return x.id_a < y.id_a; });
+    }
+  }
+  *out = r.release();
+  return VLTR_OK;
+}
+
+size_t vltr_updateable_changed_ids(const vltr_updateable* u, uint64_t* out, size_t max_ids) {
+  if (!u) return 0;
+  const auto& v = u->st.changed;
+  for (size_t k = 0; out && k < v.size() && k < max_ids; ++k) out[k] = v[k];
+  return v.size();
+}
+size_t vltr_updateable_affected_ids(const vltr_updateable* u, uint64_t* out, size_t max_ids) {
+  if (!u) return 0;
+  const auto& v = u->st.affected;
+  for (size_t k = 0; out && k < v.size() && k < max_ids; ++k) out[k] = v[k];
+  return v.size();
+}
+int vltr_updateable_last_update_was_full_reinit(const vltr_updateable* u) { return u && u->st.was_full_reinit ? 1 : 0; }
+int vltr_updateable_backup_enabled(const vltr_updateable* u) { return u && u->backup_enabled ? 1 : 0; }
+int vltr_updateable_last_timings(const vltr_updateable* u, double* ms, int max_entries) {
+  return u ? vltr_context_last_timings(u->ctx, ms, max_entries) : 0;
+}
 
 }  // extern "C"

@@ -54,6 +54,7 @@ struct Pipeline {
   const double4* balls_in;  // n (x, y, z, r)
   const int32_t* groups;    // may be null
   int64_t n_groups;
+  const uint8_t* user_excl;  // n, may be null: balls excluded by the caller (updateable.rs:225-272)
 
   double4* sph;          // n_pop (x, y, z, R)
   int32_t* seg_of;       // n_pop (segment of each sphere)
@@ -97,6 +98,7 @@ __device__ __forceinline__ bool same_group(const Pipeline& p, int64_t a, int64_t
 __device__ __forceinline__ bool pair_qualifies(const Pipeline& p, int64_t a, int64_t c) {
   const int64_t cc = p.periodic ? c % p.n : c;
   if (same_group(p, a, cc)) return false;
+  if (p.user_excl != nullptr && (p.user_excl[a] || p.user_excl[cc])) return false;
   return (p.periodic && c >= p.n) || a < c;
 }
 
@@ -129,5 +131,19 @@ hipError_t launch_emit_cells(const Pipeline& p, const uint32_t* pos, vltr_cell* 
 // pos = scanned contact flags) or first cell (what = 1, pos = scanned cell
 // flags) in the compacted output, k = 0..n_segs (out[n_segs] = total).
 hipError_t launch_seg_offsets(const Pipeline& p, const uint32_t* pos, int what, uint32_t* out, hipStream_t s);
+// Cells under the updateable's rules (updateable.rs:633-727), in place in
+// p.cell and copied to state: full = 1 after init or a full reinit, else only
+// balls with affected[i] != 0 take the new record (the rest keep state[i]);
+// excl holds the exclusion statuses (reset to the hidden status of each
+// recomputed ball when refresh != 0).
+hipError_t launch_updateable_cells(const Pipeline& p, CellRec* state, uint8_t* excl, const uint8_t* affected, int full,
+                                   int refresh, hipStream_t s);
+// affected[nb % n] = 1 for every neighbor nb of the balls changed[0..k).
+hipError_t launch_mark_more(const Pipeline& p, const int32_t* changed, int64_t k, uint8_t* affected, hipStream_t s);
+// mask[i] = 0 where where[i] != 0.
+hipError_t launch_mask_clear(uint8_t* mask, const uint8_t* where, int64_t n, hipStream_t s);
+// rows[k * cap + j] = neighbor j of ball ids[k] (cnt[k] entries).
+hipError_t launch_gather_rows(const Pipeline& p, const int32_t* ids, int64_t k, int32_t* cnt, int32_t* rows,
+                              hipStream_t s);
 
 }  // namespace vltr

@@ -10,6 +10,7 @@ import numpy as np
 
 from . import _native
 from ._native import Result, VltrError, last_timings
+from ._native import Updateable as _Updateable
 
 __all__ = [
     "compute_tessellation",
@@ -150,3 +151,46 @@ def compute_tessellation_batched(structures, probe, contacts_only=False):
         else:
             out.append({"contacts": _contacts_to_list(part.contacts), "cells": _cells_to_list(part.cells)})
     return out
+
+
+class UpdateableTessellation:
+    """updateable.rs:154-742 from Python: balls in any form parse_balls
+    accepts, periodic_box as in compute_tessellation; summary() returns the
+    dict layout of compute_tessellation."""
+
+    def __init__(self, with_backup=False):
+        self._u = _Updateable(with_backup=with_backup)
+
+    def init(self, balls, probe, periodic_box=None):
+        pb = parse_periodic_box(periodic_box) if periodic_box is not None else None
+        return self._u.init(parse_balls(balls), probe, pb)
+
+    def update(self, balls):
+        return self._u.update(parse_balls(balls))
+
+    def update_with_changed(self, balls, changed_ids):
+        return self._u.update_with_changed(parse_balls(balls), [int(i) for i in changed_ids])
+
+    def set_exclusion(self, ball_id, excluded):
+        return self._u.set_exclusion(ball_id, excluded)
+
+    def restore(self):
+        return self._u.restore()
+
+    def summary(self):
+        r = self._u.summary()
+        return {"num_balls": int(r.num_balls), "contacts": _contacts_to_list(r.contacts),
+                "cells": _cells_to_list(r.cells), "total_sas_area": float(r.total_sas_area),
+                "total_volume": float(r.total_volume), "total_contact_area": float(r.total_contact_area)}
+
+    def changed_ids(self):
+        return self._u.changed_ids()
+
+    def affected_ids(self):
+        return self._u.affected_ids()
+
+    def last_update_was_full_reinit(self):
+        return self._u.last_update_was_full_reinit()
+
+    def backup_enabled(self):
+        return self._u.backup_enabled()

@@ -81,6 +81,19 @@ SIGNATURES = {
     "vltr_batch_free": (None, [_vp]),
     "vltr_compute_tessellation_batched_device": (C.c_int, [_vp, _vp, _u64p, _sz, C.c_double, C.c_int]),
     "vltr_context_batch_offsets": (C.c_int, [_vp, _sz, _u64p, _u64p]),
+    "vltr_updateable_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
+    "vltr_updateable_destroy": (None, [_vp]),
+    "vltr_updateable_init": (C.c_int, [_vp, _vp, _sz, C.c_double, C.POINTER(VltrPeriodicBox), C.POINTER(C.c_int)]),
+    "vltr_updateable_update": (C.c_int, [_vp, _vp, _sz, C.POINTER(C.c_int)]),
+    "vltr_updateable_update_with_changed": (C.c_int, [_vp, _vp, _sz, _u64p, _sz, C.POINTER(C.c_int)]),
+    "vltr_updateable_set_exclusion": (C.c_int, [_vp, _sz, C.c_int, C.POINTER(C.c_int)]),
+    "vltr_updateable_restore": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "vltr_updateable_summary": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "vltr_updateable_changed_ids": (_sz, [_vp, _u64p, _sz]),
+    "vltr_updateable_affected_ids": (_sz, [_vp, _u64p, _sz]),
+    "vltr_updateable_last_update_was_full_reinit": (C.c_int, [_vp]),
+    "vltr_updateable_backup_enabled": (C.c_int, [_vp]),
+    "vltr_updateable_last_timings": (C.c_int, [_vp, _f64p, C.c_int]),
 }
 
 _lock = threading.Lock()
@@ -299,3 +312,86 @@ def run_batched(structures, probe, contacts_only=False, device=0):
         return parts
     finally:
         lib.vltr_batch_free(handle)
+
+
+class Updateable:
+    """UpdateableTessellation (updateable.rs:154-742) on the GPU; the methods
+    return the reference's booleans."""
+
+    def __init__(self, with_backup=False, device=0):
+        lib = load_library()
+        h = C.c_void_p()
+        check(lib.vltr_updateable_create(device, int(bool(with_backup)), C.byref(h)), "vltr_updateable_create")
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if self._h:
+            self._lib.vltr_updateable_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        ok = C.c_int(0)
+        check(getattr(self._lib, name)(self._h, *args, C.byref(ok)), name)
+        return bool(ok.value)
+
+    def init(self, balls, probe, pbox=None):
+        b = _balls_array(balls)
+        box = make_pbox(pbox)
+        return self._call("vltr_updateable_init", b.ctypes.data_as(_vp) if len(b) else None, len(b), float(probe),
+                          C.byref(box) if box is not None else None)
+
+    def update(self, balls):
+        b = _balls_array(balls)
+        return self._call("vltr_updateable_update", b.ctypes.data_as(_vp) if len(b) else None, len(b))
+
+    def update_with_changed(self, balls, changed):
+        b = _balls_array(balls)
+        ch = np.ascontiguousarray(np.asarray(changed, dtype=np.uint64).reshape(-1))
+        return self._call("vltr_updateable_update_with_changed", b.ctypes.data_as(_vp) if len(b) else None, len(b),
+                          ch.ctypes.data_as(_u64p) if len(ch) else None, len(ch))
+
+    def set_exclusion(self, ball_id, excluded):
+        return self._call("vltr_updateable_set_exclusion", int(ball_id), int(bool(excluded)))
+
+    def restore(self):
+        return self._call("vltr_updateable_restore")
+
+    def summary(self) -> Result:
+        h = C.c_void_p()
+        check(self._lib.vltr_updateable_summary(self._h, C.byref(h)), "vltr_updateable_summary")
+        try:
+            return Result(h)
+        finally:
+            self._lib.vltr_result_free(h)
+
+    def _ids(self, name):
+        fn = getattr(self._lib, name)
+        k = fn(self._h, None, 0)
+        out = np.zeros(max(k, 1), dtype=np.uint64)
+        fn(self._h, out.ctypes.data_as(_u64p), k)
+        return [int(x) for x in out[:k]]
+
+    def changed_ids(self):
+        return self._ids("vltr_updateable_changed_ids")
+
+    def affected_ids(self):
+        return self._ids("vltr_updateable_affected_ids")
+
+    def last_update_was_full_reinit(self):
+        return bool(self._lib.vltr_updateable_last_update_was_full_reinit(self._h))
+
+    def backup_enabled(self):
+        return bool(self._lib.vltr_updateable_backup_enabled(self._h))
+
+    def last_timings(self):
+        buf = (C.c_double * 7)()
+        k = self._lib.vltr_updateable_last_timings(self._h, buf, 7)
+        names = ["grid", "neighbors", "pairs", "clip", "slow", "cells", "total"]
+        return {names[i]: buf[i] for i in range(k)}

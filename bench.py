@@ -29,6 +29,10 @@ METRIC = "balls/s full tessellation f64 @100k: plain vs batched; fwd+adjoint/fwd
 FLOP_PER_PAIR = 2914.39
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes one full-mode k_clip launch moves on the default workload (100,000
+# balls, seed 100000): FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB, factors from a
+# 256 MiB device copy; profiles/k_clip_100k_hbm_bytes.txt.
+K_CLIP_TRAFFIC_BYTES_100K = 449.9e6
 
 
 def main():
@@ -43,6 +47,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--batch-structs", type=int, default=4096,
                     help="structures in the batched leg (0 skips it); sizes log-uniform in [200, 20000]")
+    ap.add_argument("--dyn-balls", type=int, default=50000,
+                    help="balls of the periodic dynamics leg (0 skips it)")
+    ap.add_argument("--dyn-steps", type=int, default=20)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (reference kernel for the "
+                         "HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -55,6 +65,14 @@ def main():
     import numpy as np
 
     from voronota_ltr import _hip, _native, workload
+
+    if args.copy_mb > 0:
+        nbytes = args.copy_mb << 20
+        src, dst = _hip.DeviceBuffer(nbytes), _hip.DeviceBuffer(nbytes)
+        _hip.copy_d2d(dst, src, nbytes)
+        _hip.synchronize()
+        src.free()
+        dst.free()
 
     balls, _ = workload.rsa_sphere(args.n, seed=args.seed)
     lib = _native.load_library()
@@ -90,6 +108,7 @@ def main():
     achieved = flops / (cut_ms / 1e3) / 1e12 if cut_ms > 0 else 0.0
 
     batched = batched_leg(args, lib, ctx) if args.batch_structs > 0 else None
+    dynamics = dynamics_leg(args) if args.dyn_balls > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -131,7 +150,8 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": K_CLIP_TRAFFIC_BYTES_100K if (args.n, args.seed, args.probe) == (100000, 100000, 1.4) else None,
+            "traffic_unit": "bytes per launch",
             "note": "FP64 vector work (no matrix op on this path); flops = "
                     f"{FLOP_PER_PAIR} per candidate pair x {counts['pairs']} pairs",
         },
@@ -143,6 +163,8 @@ def main():
     }
     if batched:
         out["batched"] = batched
+    if dynamics:
+        out["periodic_dynamics"] = dynamics
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
     print(json.dumps(out), flush=True)
@@ -203,6 +225,60 @@ def batched_leg(args, lib, ctx):
             "note": "full compute_tessellation of every structure, balls resident in HBM; plain = one "
                     "vltr_compute_tessellation_device call per structure, batched = one "
                     "vltr_compute_tessellation_batched_device call"}
+
+
+def dynamics_leg(args):
+    """Periodic dynamics through the updateable tessellation: N balls placed
+    by RSA in a periodic cube at 0.056 balls/A^3; each step moves 500 seeded
+    balls by N(0, 0.05^2) A per axis, wraps them into the box and updates
+    (update_with_changed). The oracle's updateable (C++, host cores) runs the
+    first steps of the same sequence for comparison."""
+    import numpy as np
+
+    from voronota_ltr import _native, workload
+
+    n = args.dyn_balls
+    L = (n / 0.056) ** (1.0 / 3.0)
+    balls, _ = workload.rsa_box(n, L, seed=50000)
+    box = [L, 0, 0, 0, L, 0, 0, 0, L]
+    rng = np.random.default_rng(50000)
+    moves = []
+    cur = balls.copy()
+    for _ in range(args.dyn_steps):
+        ch = np.sort(rng.choice(n, 500, replace=False))
+        cur[ch, :3] = np.mod(cur[ch, :3] + rng.normal(scale=0.05, size=(500, 3)), L)
+        moves.append((cur.copy(), ch))
+    u = _native.Updateable()
+    assert u.init(balls, args.probe, box)
+    t0 = time.perf_counter()
+    reinit = 0
+    affected = 0
+    for b, ch in moves:
+        assert u.update_with_changed(b, ch)
+        reinit += int(u.last_update_was_full_reinit())
+        affected += len(u.affected_ids())
+    t_gpu = (time.perf_counter() - t0) / len(moves)
+    ph = u.last_timings()
+    u.close()
+    out = {"balls": n, "box_edge_A": round(L, 3), "steps": len(moves), "moved_per_step": 500,
+           "ms_per_step": round(t_gpu * 1e3, 3), "steps_per_s": round(1.0 / t_gpu, 1),
+           "mean_affected": round(affected / len(moves), 1), "full_reinits": reinit,
+           "last_step_phases_ms": {k: round(v, 4) for k, v in ph.items()},
+           "note": "host API (balls uploaded each step, summary not downloaded)"}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+
+        threads = min(os.cpu_count() or 1, 16)
+        o = pyoracle.Updateable(threads=threads)
+        o.init(balls, args.probe, box)
+        k = 2
+        t0 = time.perf_counter()
+        for b, ch in moves[:k]:
+            o.update_with_changed(b, ch)
+        out["cpu_oracle_ms_per_step"] = round((time.perf_counter() - t0) / k * 1e3, 2)
+        out["cpu_cores"] = threads
+    return out
 
 
 if __name__ == "__main__":

@@ -66,6 +66,13 @@ class DeviceBuffer:
             pass
 
 
+def copy_d2d(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("copy larger than the buffers")
+    if _load().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D) != 0:
+        raise RuntimeError("hipMemcpy D2D failed")
+
+
 def synchronize():
     if _load().hipDeviceSynchronize() != 0:
         raise RuntimeError("hipDeviceSynchronize failed")

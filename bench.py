@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--dyn-balls", type=int, default=50000,
                     help="balls of the periodic dynamics leg (0 skips it)")
     ap.add_argument("--dyn-steps", type=int, default=20)
+    ap.add_argument("--adj-balls", type=int, default=20000,
+                    help="balls of the forward + adjoint leg (0 skips it)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (reference kernel for the "
                          "HBM byte counters)")
@@ -109,6 +111,7 @@ def main():
 
     batched = batched_leg(args, lib, ctx) if args.batch_structs > 0 else None
     dynamics = dynamics_leg(args) if args.dyn_balls > 0 else None
+    adjoint = adjoint_leg(args, lib, ctx) if args.adj_balls > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -165,6 +168,8 @@ def main():
         out["batched"] = batched
     if dynamics:
         out["periodic_dynamics"] = dynamics
+    if adjoint:
+        out["adjoint"] = adjoint
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
     print(json.dumps(out), flush=True)
@@ -225,6 +230,55 @@ def batched_leg(args, lib, ctx):
             "note": "full compute_tessellation of every structure, balls resident in HBM; plain = one "
                     "vltr_compute_tessellation_device call per structure, batched = one "
                     "vltr_compute_tessellation_batched_device call"}
+
+
+def adjoint_leg(args, lib, ctx):
+    """Forward vs forward + adjoint on the differentiation workload: N balls by
+    RSA in a sphere, type = radius index, seeded symmetric W in [-1, 1]^(4x4);
+    E = sum_contacts W[t_a][t_b] area + sum SAS + sum volume, gradient with
+    respect to all centres; inputs and gradient resident in HBM."""
+    import ctypes as C
+
+    import numpy as np
+
+    from voronota_ltr import _hip, _native, workload
+
+    n = args.adj_balls
+    balls, types = workload.rsa_sphere(n, seed=n)
+    W = np.ascontiguousarray(workload.type_matrix(n).reshape(16))
+    db = _hip.DeviceBuffer(balls.nbytes)
+    db.upload(np.ascontiguousarray(balls))
+    dt = _hip.DeviceBuffer(4 * n)
+    dt.upload(np.ascontiguousarray(types.astype(np.int32)))
+    dg = _hip.DeviceBuffer(24 * n)
+    e = C.c_double()
+    wp = W.ctypes.data_as(C.POINTER(C.c_double))
+
+    def fwd():
+        _native.check(lib.vltr_compute_tessellation_device(ctx, db.ptr, n, args.probe, None, 0, 0), "forward")
+
+    def fwd_adj():
+        _native.check(lib.vltr_energy_gradient_device(ctx, db.ptr, n, args.probe, dt.ptr, wp, 1.0, 1.0, dg.ptr,
+                                                      C.byref(e)), "forward + adjoint")
+
+    def timeit(fn, reps=10):
+        for _ in range(2):
+            fn()
+        _hip.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        _hip.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    t_f = timeit(fwd)
+    t_fa = timeit(fwd_adj)
+    for b in (db, dt, dg):
+        b.free()
+    return {"balls": n, "fwd_ms": round(t_f * 1e3, 3), "fwd_adjoint_ms": round(t_fa * 1e3, 3),
+            "fwd_adjoint_over_fwd": round(t_fa / t_f, 3), "energy": e.value,
+            "note": "E = sum_contacts W[t_a][t_b] area + sum SAS + sum volume; gradient of all 3N centre "
+                    "coordinates; device-resident calls"}
 
 
 def dynamics_leg(args):

@@ -175,6 +175,23 @@ int vltr_compute_tessellation_batched_device(vltr_context* ctx, const vltr_ball*
                                              size_t n_structs, double probe, int contacts_only);
 int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64_t* contact_off, uint64_t* cell_off);
 
+/* ------------------------------------------------------------ adjoint --- */
+
+/* Energy E = sum_contacts W[t_a][t_b] * area + sas_weight * sum_cells sas_area
+ * + volume_weight * sum_cells volume of compute_tessellation
+ * (tessellation.rs:37-48) and its gradient with respect to the ball centres
+ * (radii and probe fixed), holding the forward pass's contour combinatorics
+ * fixed. types[i] in 0..3, w16 = row-major 4x4 W. Non-periodic, no groups.
+ * grad receives 3 n doubles (x, y, z per ball). No reference counterpart:
+ * the reference has no derivatives. */
+int vltr_energy_gradient(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe, const int32_t* types,
+                         const double* w16, double sas_weight, double volume_weight, double* energy, double* grad);
+/* Same on device memory: d_balls, d_types, d_grad are device pointers; w16 and
+ * energy are host memory. */
+int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
+                                const int32_t* d_types, const double* w16, double sas_weight, double volume_weight,
+                                double* d_grad, double* energy);
+
 /* --------------------------------------------------------- updateable --- */
 
 /* UpdateableTessellation (updateable.rs:154-742) on the device: a

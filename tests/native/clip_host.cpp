@@ -9,6 +9,9 @@
 #include <cstring>
 #include <vector>
 
+#include <functional>
+
+#include "adjoint.h"
 #include "contour.h"
 #include "group_clip.h"
 
@@ -138,9 +141,176 @@ bool run_group(GroupTask& t, vltr::PairRec* out, int* st) {
   *st = t.st[0];
   return agree;
 }
+// Runs fn(lane) on 16 lock-stepped fibers.
+std::function<void(HostLane&)>* g_fn = nullptr;
+FiberGroup* g_fg = nullptr;
+void lane_main_fn(int ln) {
+  HostLane lane{g_fg, ln};
+  (*g_fn)(lane);
+  g_fg->done[ln] = true;
+}
+void run_fibers(std::function<void(HostLane&)> fn) {
+  static FiberGroup fg;
+  g_fn = &fn;
+  g_fg = &fg;
+  for (int k = 0; k < FiberGroup::W; ++k) {
+    fg.round[k] = 0;
+    fg.done[k] = false;
+    getcontext(&fg.fib[k]);
+    fg.fib[k].uc_stack.ss_sp = fg.stack[k].data();
+    fg.fib[k].uc_stack.ss_size = fg.stack[k].size();
+    fg.fib[k].uc_link = &fg.sched;
+    makecontext(&fg.fib[k], reinterpret_cast<void (*)()>(lane_main_fn), 1, k);
+  }
+  for (;;) {
+    int running = 0;
+    for (int k = 0; k < FiberGroup::W; ++k) {
+      if (fg.done[k]) continue;
+      swapcontext(&fg.sched, &fg.fib[k]);
+      ++running;
+    }
+    if (running == 0) break;
+  }
+}
+
+struct HostAcc {
+  double* grad;
+  int64_t n;
+  void add(int64_t id, vltr::V3 v) {
+    const int64_t i = id % n;
+    grad[3 * i] += v.x;
+    grad[3 * i + 1] += v.y;
+    grad[3 * i + 2] += v.z;
+  }
+};
 }  // namespace
 
 extern "C" {
+
+// Gradient of E = sum_contacts W[t_a][t_b] area + sum_cells (sas + volume)
+// with respect to the centres, by the kernels' adjoint code (adjoint.h) run on
+// 16 host fibers: forward records and cut topologies from the group clip,
+// cell seeds from the per-ball sums, then the backward of every contact.
+// Non-periodic; lists up to 64 neighbors. Returns the number of contacts
+// without topology (their gradient is missing), or -1.
+int64_t adjoint_host(const double* sph4, int64_t n, const int64_t* nb_off, const int32_t* nb_idx, const int32_t* pairs,
+                     int64_t P, const int32_t* types, const double* W16, double* energy, double* grad) {
+  double hs[5], hc[5];
+  hexagon(hs, hc);
+  std::vector<vltr::PairRec> recs(P);
+  std::vector<vltr::ContactTopo> topo(P);
+  int64_t missing = 0;
+  GroupTask* task = new GroupTask;
+  for (int64_t t = 0; t < P; ++t) {
+    const int32_t a = pairs[2 * t], b = pairs[2 * t + 1];
+    const int32_t* nbl = nb_idx + nb_off[a];
+    const int nbn = static_cast<int>(nb_off[a + 1] - nb_off[a]);
+    recs[t] = vltr::empty_record();
+    topo[t].m = vltr::kTopoNone;
+    if (nbn > 64) return -1;
+    vltr::PairGeom g;
+    const vltr::Sph A = vltr::load_sph(sph4, a);
+    if (!vltr::pair_setup(A, vltr::load_sph(sph4, b), &g)) continue;
+    bool kill = false;
+    uint64_t m = 0;
+    for (int k = 0; k < nbn; ++k) {
+      if (nbl[k] == b) continue;
+      const vltr::Sph C = vltr::load_sph(sph4, nbl[k]);
+      const int cl = vltr::classify(g, C, vltr::plane_of(A, C), vltr::contains(C, A));
+      if (cl == vltr::kKill) kill = true;
+      if (cl == vltr::kCut) m |= 1ull << k;
+    }
+    if (kill) continue;
+    if (m == 0) {
+      recs[t] = vltr::record_uncut(g, 0);
+      topo[t].m = 0;
+      continue;
+    }
+    vltr::PairRec rec[FiberGroup::W];
+    int st[FiberGroup::W];
+    vltr::TopoOut tout{&topo[t], sph4};
+    run_fibers([&](HostLane& lane) {
+      struct {
+        const double* sph4;
+        const int32_t* nbl;
+        vltr::Sph A;
+        double thr(int k) const { return vltr::norm2(vltr::plane_of(A, vltr::load_sph(sph4, nbl[k])).pc - A.c); }
+        vltr::CutPlane plane(int k) const {
+          const vltr::Plane pl = vltr::plane_of(A, vltr::load_sph(sph4, nbl[k]));
+          return vltr::CutPlane{pl.pc, pl.pn, vltr::unit(pl.pn), nbl[k]};
+        }
+      } fetch{sph4, nbl, A};
+      rec[lane.ln] = vltr::group_clip(lane, &task->lds, g, a, m, 0ull, fetch, hs, hc, 0, &st[lane.ln], &tout);
+    });
+    recs[t] = rec[0];
+    if (st[0] == vltr::kOverflow) {
+      ++missing;
+      topo[t].m = vltr::kTopoTooLong;
+    }
+  }
+  delete task;
+  // cells (types.rs:587-663) and their seeds
+  std::vector<double> sa_pos(n, 0.0), sa_neg(n, 0.0), pv_pos(n, 0.0), pv_neg(n, 0.0), arc(n, 0.0);
+  std::vector<int> added(n, 0);
+  double e = 0.0;
+  for (int64_t t = 0; t < P; ++t) {
+    const vltr::PairRec& r = recs[t];
+    if (!(r.flags & vltr::kRecValid) || !(r.area > 0.0)) continue;
+    const int32_t a = pairs[2 * t], b = pairs[2 * t + 1];
+    e += W16[4 * types[a] + types[b]] * r.area;
+    for (int side = 0; side < 2; ++side) {
+      const int32_t i = side == 0 ? a : b;
+      const double sa = side == 0 ? r.solid_a : r.solid_b;
+      const double pv = side == 0 ? r.pyr_a : r.pyr_b;
+      added[i] = 1;
+      arc[i] += r.arc_length;
+      sa_pos[i] += std::fmax(sa, 0.0);
+      sa_neg[i] -= std::fmin(sa, 0.0);
+      pv_pos[i] += std::fmax(pv, 0.0);
+      pv_neg[i] -= std::fmin(pv, 0.0);
+    }
+  }
+  std::vector<double> seed(n, 0.0);
+  for (int64_t i = 0; i < n; ++i) {
+    if (!added[i]) continue;
+    const double R = sph4[4 * i + 3];
+    double sas = 0.0, vol;
+    bool standard = false;
+    if (arc[i] > 0.0 && std::fabs(sa_pos[i] - sa_neg[i]) > 1e-10) {
+      standard = true;
+      if (sa_pos[i] > sa_neg[i])
+        sas = std::fma(4.0, vltr::kPi, -std::fmax(sa_pos[i] - sa_neg[i], 0.0)) * R * R;
+      else
+        sas = std::fmax(sa_neg[i] - sa_pos[i], 0.0) * R * R;
+      vol = (sas * R / 3.0) + pv_pos[i] - pv_neg[i];
+      if (vol > 4.0 / 3.0 * vltr::kPi * R * R * R) {
+        standard = false;
+        sas = 0.0;
+        vol = pv_pos[i] - pv_neg[i];
+      }
+    } else {
+      vol = pv_pos[i] - pv_neg[i];
+    }
+    e += sas + vol;
+    seed[i] = standard ? -R * R * (1.0 + R / 3.0) : 0.0;
+  }
+  *energy = e;
+  for (int64_t k = 0; k < 3 * n; ++k) grad[k] = 0.0;
+  HostAcc acc{grad, n};
+  static vltr::AdjLds lds;
+  for (int64_t t = 0; t < P; ++t) {
+    const vltr::PairRec& r = recs[t];
+    if (!(r.flags & vltr::kRecValid) || !(r.area > 0.0) || topo[t].m < 0) continue;
+    const int32_t a = pairs[2 * t], b = pairs[2 * t + 1];
+    const double w = W16[4 * types[a] + types[b]];
+    const double sga = r.solid_a < 0.0 ? -1.0 : 1.0, sgb = r.solid_b < 0.0 ? -1.0 : 1.0;
+    run_fibers([&](HostLane& lane) {
+      vltr::group_contact_adjoint(lane, &lds, sph4, a, b, topo[t], w, seed[a], seed[b], sga, sgb, acc);
+    });
+  }
+  return missing;
+}
+
 int64_t xcd_remap_host(int64_t b, int64_t g) { return vltr::xcd_remap(b, g); }
 
 // The kernels' two-stage path on the host: classify every neighbor of every

@@ -1,0 +1,186 @@
+"""Adjoint of E = sum_contacts W[t_a][t_b] area + alpha sum SAS + beta sum volume
+with respect to the ball centres, on the GPU.
+
+* against the oracle's forward-mode dual numbers (the exact derivative of the
+  reference's computation with its contour combinatorics fixed), all three
+  coordinates of 80 balls of a 300-ball cluster;
+* against central finite differences (h = 1e-6 A) of the GPU's own forward
+  results, differenced term by term (each contact and cell separately, so the
+  large total does not cancel), on the 20,000-ball differentiation workload,
+  64 random balls x 3 axes, skipping perturbations that change the contact
+  set; a subset also against differences of the CPU path (the oracle) and
+  against the oracle's exact derivative. The forward terms near a ball carry
+  round-off of about 1e-11 in total (sums of solid angles, small arc angles
+  from acos), which a difference over 2h = 2e-6 A turns into up to ~5e-6: a
+  coordinate whose h = 1e-6 difference misses the bar is checked again at
+  h = 1e-4 (truncation error ~1e-8), and at most 5% may need that;
+* total-SAS-only and total-volume-only weights;
+* the torch.autograd.Function wrapper.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cluster(n, seed):
+    from voronota_ltr import workload
+
+    balls, types = workload.rsa_sphere(n, seed=seed)
+    return balls, types, workload.type_matrix(seed)
+
+
+def _terms(res, types, W, alpha=1.0, beta=1.0):
+    c = res.contacts
+    ids = zip(c["id_a"].astype(np.int64), c["id_b"].astype(np.int64))
+    ct = {(a, b): W[types[a], types[b]] * ar for (a, b), ar in zip(ids, c["area"])}
+    cl = {int(i): alpha * s + beta * v for i, s, v in zip(res.cells["index"], res.cells["sas_area"],
+                                                          res.cells["volume"])}
+    return ct, cl
+
+
+def _fd(balls, types, W, ball, axis, h, alpha=1.0, beta=1.0):
+    """Central difference of E from GPU forward runs, term by term; None when
+    the contact set changes between the two runs."""
+    from voronota_ltr import _native
+
+    bp, bm = balls.copy(), balls.copy()
+    bp[ball, axis] += h
+    bm[ball, axis] -= h
+    cp, lp = _terms(_native.run(bp, 1.4), types, W, alpha, beta)
+    cm, lm = _terms(_native.run(bm, 1.4), types, W, alpha, beta)
+    if cp.keys() != cm.keys() or lp.keys() != lm.keys():
+        return None
+    d = sum(cp[k] - cm[k] for k in cp) + sum(lp[k] - lm[k] for k in lp)
+    return d / (2 * h)
+
+
+def test_gradient_matches_oracle_dual(oracle):
+    from voronota_ltr import _native
+
+    balls, types, W = _cluster(300, 31)
+    print("\n[gpu] adjoint vs dual: 300 balls", flush=True)
+    e, g = _native.energy_gradient(balls, 1.4, types, W)
+    e_ref = oracle.energy(balls, 1.4, types, W)
+    assert e == pytest.approx(e_ref, rel=1e-10)
+    worst = 0.0
+    for ball in np.random.default_rng(3).choice(len(balls), 80, replace=False):
+        for axis in range(3):
+            _, de = oracle.energy_dual(balls, 1.4, types, W, int(ball), axis)
+            err = abs(g[ball, axis] - de) / max(1.0, abs(de))
+            worst = max(worst, err)
+            assert err < 1e-7, (ball, axis, g[ball, axis], de)
+    print(f"[gpu] adjoint vs dual: worst {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.0, 1.0)])
+def test_sas_and_volume_gradients(alpha, beta):
+    from voronota_ltr import _native
+
+    balls, types, _ = _cluster(400, 32)
+    Wz = np.zeros((4, 4))
+    e, g = _native.energy_gradient(balls, 1.4, types, Wz, sas_weight=alpha, volume_weight=beta)
+    r = _native.run(balls, 1.4)
+    assert e == pytest.approx(alpha * r.cells["sas_area"].sum() + beta * r.cells["volume"].sum(), rel=1e-12)
+    rng = np.random.default_rng(5)
+    checked = 0
+    for ball in rng.choice(len(balls), 12, replace=False):
+        for axis in range(3):
+            fd = _fd(balls, types, Wz, int(ball), axis, 1e-6, alpha, beta)
+            if fd is None:
+                continue
+            checked += 1
+            assert abs(g[ball, axis] - fd) <= 1e-6 * abs(fd) + 1e-6, (ball, axis, g[ball, axis], fd)
+    assert checked >= 30
+
+
+def test_differentiation_workload_fd(oracle):
+    """20,000 balls (the differentiation workload), 64 random balls x 3 axes."""
+    from voronota_ltr import _native
+
+    balls, types, W = _cluster(20000, 20000)
+    print("\n[gpu] 20k adjoint", flush=True)
+    e, g = _native.energy_gradient(balls, 1.4, types, W)
+    rng = np.random.default_rng(64)
+    picks = rng.choice(len(balls), 64, replace=False)
+    checked = skipped = rechecked = 0
+    worst = 0.0
+    for k, ball in enumerate(picks):
+        for axis in range(3):
+            fd = _fd(balls, types, W, int(ball), axis, 1e-6)
+            if fd is None:
+                skipped += 1
+                continue
+            checked += 1
+            err = abs(g[ball, axis] - fd) / max(abs(fd), 1.0)
+            if err > 1e-6:
+                rechecked += 1
+                fd4 = _fd(balls, types, W, int(ball), axis, 1e-4)
+                assert fd4 is not None and abs(g[ball, axis] - fd4) <= 1e-6 * max(abs(fd4), 1.0), \
+                    (int(ball), axis, g[ball, axis], fd, fd4)
+                err = abs(g[ball, axis] - fd4) / max(abs(fd4), 1.0)
+            worst = max(worst, err)
+        if k % 16 == 15:
+            print(f"[gpu] 20k adjoint: {k + 1} balls, worst {worst:.2e}", flush=True)
+    print(f"[gpu] 20k adjoint: checked {checked}, skipped {skipped}, rechecked at h=1e-4 {rechecked}, "
+          f"worst {worst:.2e}", flush=True)
+    assert checked >= 150
+    assert rechecked <= 0.05 * checked
+    # the oracle's exact derivative (forward-mode dual numbers) at full size
+    for ball in picks[:4]:
+        _, de = oracle.energy_dual(balls, 1.4, types, W, int(ball), 1)
+        assert abs(g[ball, 1] - de) <= 1e-8 * max(abs(de), 1.0), (int(ball), g[ball, 1], de)
+    # reference gradient by differencing the CPU path (oracle), term by term
+    for ball in picks[:4]:
+        bp, bm = balls.copy(), balls.copy()
+        bp[ball, 0] += 1e-6
+        bm[ball, 0] -= 1e-6
+        tp = oracle.energy_terms(bp, 1.4, types, W)
+        tm = oracle.energy_terms(bm, 1.4, types, W)
+        kp = {tuple(int(x) for x in i): a for i, a in zip(tp.contact_ids, tp.contact_area)}
+        km = {tuple(int(x) for x in i): a for i, a in zip(tm.contact_ids, tm.contact_area)}
+        if kp.keys() != km.keys():
+            continue
+        d = sum(kp[k] - km[k] for k in kp)
+        d += float(np.sum(tp.cell_sas - tm.cell_sas) + np.sum(tp.cell_volume - tm.cell_volume))
+        fd = d / 2e-6
+        assert abs(g[ball, 0] - fd) <= 1e-5 * max(abs(fd), 1.0), (int(ball), g[ball, 0], fd)
+
+
+TORCH_CHECK = r"""
+import sys
+import numpy as np
+import torch
+sys.path.insert(0, sys.argv[1])
+if not torch.cuda.is_available():
+    print("NO_GPU"); sys.exit(0)
+from voronota_ltr import _native, workload
+from voronota_ltr.autograd import tessellation_energy
+balls, types = workload.rsa_sphere(500, seed=33)
+W = workload.type_matrix(33)
+dev = torch.device("cuda", 0)
+centers = torch.tensor(balls[:, :3], dtype=torch.float64, device=dev, requires_grad=True)
+radii = torch.tensor(balls[:, 3], dtype=torch.float64, device=dev)
+e = tessellation_energy(centers, radii, torch.tensor(types, device=dev), torch.tensor(W), 1.4)
+(2.0 * e).backward()
+e_ref, g_ref = _native.energy_gradient(balls, 1.4, types, W)
+assert abs(float(e.detach()) - e_ref) <= 1e-12 * abs(e_ref), (float(e.detach()), e_ref)
+assert np.allclose(centers.grad.cpu().numpy(), 2.0 * g_ref, rtol=1e-12, atol=1e-12)
+print("TORCH_OK")
+"""
+
+
+def test_torch_autograd():
+    """The torch.autograd.Function wrapper, in a fresh process that imports
+    torch before the library (so both use torch's HIP runtime)."""
+    import os
+    import subprocess
+    import sys
+
+    pytest.importorskip("torch")
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "voronota-ltr_amd")
+    out = subprocess.run([sys.executable, "-c", TORCH_CHECK, pkg], capture_output=True, text=True, timeout=300)
+    print(out.stdout[-2000:], out.stderr[-2000:], flush=True)
+    if "NO_GPU" in out.stdout:
+        pytest.skip("torch sees no GPU")
+    assert out.returncode == 0 and "TORCH_OK" in out.stdout

@@ -96,3 +96,44 @@ def test_xcd_remap_is_bijective():
     L.xcd_remap_host.argtypes = [C.c_int64, C.c_int64]
     for g in list(range(1, 130)) + [1000, 4097, 25000]:
         assert sorted(L.xcd_remap_host(b, g) for b in range(g)) == list(range(g))
+
+
+def _adjoint_host(balls, probe, types, W):
+    import ctypes as C
+
+    L = hostclip._load()
+    f = L.adjoint_host
+    f.restype = C.c_int64
+    f.argtypes = [C.c_void_p, C.c_int64] + [C.c_void_p] * 3 + [C.c_int64] + [C.c_void_p] * 4
+    return f
+
+
+@pytest.mark.parametrize("n,seed", [(40, 21), (120, 5)])
+def test_adjoint_matches_dual_numbers(oracle, n, seed):
+    """The kernels' adjoint code (adjoint.h: topology from the group clip,
+    vertex recomputation, hand-written VJPs) run on host fibers, against the
+    oracle's forward-mode dual numbers for every coordinate."""
+    import ctypes as C
+
+    from voronota_ltr import workload
+
+    balls, types = workload.rsa_sphere(n, seed=seed)
+    W = workload.type_matrix(3)
+    off, idx, _, _, pairs = oracle.collisions(balls, 1.4)
+    S = np.ascontiguousarray(hostclip.populated(balls, 1.4))
+    off = np.ascontiguousarray(off, np.int64)
+    idx = np.ascontiguousarray(idx, np.int32)
+    pr = np.ascontiguousarray(pairs, np.int32)
+    ty = np.ascontiguousarray(types, np.int32)
+    Wc = np.ascontiguousarray(W, np.float64)
+    e = C.c_double()
+    g = np.zeros((n, 3))
+    f = _adjoint_host(balls, 1.4, types, W)
+    missing = f(S.ctypes.data, n, off.ctypes.data, idx.ctypes.data, pr.ctypes.data, len(pr), ty.ctypes.data,
+                Wc.ctypes.data, C.byref(e), g.ctypes.data)
+    assert missing == 0
+    assert e.value == pytest.approx(oracle.energy(balls, 1.4, types, W), rel=1e-12)
+    for ball in range(n):
+        for axis in range(3):
+            _, de = oracle.energy_dual(balls, 1.4, types, W, ball, axis)
+            assert abs(g[ball, axis] - de) <= 1e-7 * max(1.0, abs(de)), (ball, axis, g[ball, axis], de)

@@ -37,6 +37,28 @@ struct GPt {
   int32_t l, r;
 };
 
+// Where a clip also records the cut topology of its contour (for the adjoint):
+// the pair's topology record and the spheres (to tell the two candidate circle
+// points apart).
+constexpr int kTopoCap = kGroupR;
+// Cut topology of one contact: m points (m = 0: uncut full circle, m < 0: no
+// topology, so no gradient), ids l[k], r[k] of the pieces before and after
+// point k (a's own id = the circle arc), bit k of roots = which of the two
+// candidate circle points (sign of (x - a) . ((b - a) x (c - a))).
+struct ContactTopo {
+  int32_t m;
+  uint32_t roots;
+  int32_t l[kTopoCap];
+  int32_t r[kTopoCap];
+};
+constexpr int32_t kTopoNone = -1;
+constexpr int32_t kTopoTooLong = -2;
+
+struct TopoOut {
+  ContactTopo* topo;
+  const double* sph4;
+};
+
 // Radical plane of a and a cutting neighbor (plane_of), plus unit(pn) and the
 // neighbor's id.
 struct CutPlane {
@@ -364,13 +386,39 @@ VLTR_HD void area_terms(const GroupLds* s, int t, int m, const Sph& ic, V3 bary,
   }
 }
 
+VLTR_HD bool circle_root(V3 x, V3 a, V3 b, V3 c) { return dot(x - a, cross(b - a, c - a)) > 0.0; }
+
+// Writes the topology of the restricted contour s[0..m).
+template <class G>
+VLTR_HD void write_topology(G& g, const GroupLds* s, const PairGeom& pg, int32_t a_id, int m, const TopoOut& tout) {
+  const int j = g.lane();
+  bool root[2] = {false, false};
+  for (int h = 0; h < 2; ++h) {
+    const int t = j + h * kGroup;
+    if (t >= m) continue;
+    const int32_t l = s->l[t], r = s->r[t];
+    tout.topo->l[t] = l;
+    tout.topo->r[t] = r;
+    if (l == a_id || r == a_id) {
+      const int32_t c = l == a_id ? r : l;
+      root[h] = circle_root(lds_p(s, t), pg.a.c, pg.b.c, load_sph(tout.sph4, c).c);
+    }
+  }
+  const uint32_t r0 = g.ballot(root[0]), r1 = g.ballot(root[1]);
+  if (j == 0) {
+    tout.topo->m = m;
+    tout.topo->roots = r0 | (r1 << kGroup);
+  }
+}
+
 // The whole clip of a pair known to have no killing neighbor and a non-empty
 // cutting set (bit k of (mlo, mhi) = slot k of a's sorted neighbor list).
 // f.plane(k) returns slot k's CutPlane and f.thr(k) the squared distance from
 // a's centre to its plane point. Every lane returns the same record.
 template <class G, class Fetch>
 VLTR_HD PairRec group_clip(G& g, GroupLds* s, const PairGeom& pg, int32_t a_id, uint64_t mlo, uint64_t mhi,
-                           Fetch&& f, const double* hex_s, const double* hex_c, int mode, int* status) {
+                           Fetch&& f, const double* hex_s, const double* hex_c, int mode, int* status,
+                           const TopoOut* tout = nullptr) {
   const int j = g.lane();
   GPt c{pg.ic.c, a_id, a_id};
   int n = 0;
@@ -436,6 +484,7 @@ VLTR_HD PairRec group_clip(G& g, GroupLds* s, const PairGeom& pg, int32_t a_id, 
   const bool central = g.ballot(reflex) != 0u || g.ballot(fail) == 0u;
   if (area <= 0.0) return empty_record();
   *status = kContact;
+  if (tout != nullptr) write_topology(g, s, pg, a_id, m, *tout);
   PairRec rec = empty_record();
   rec.area = area;
   rec.arc_length = sum_arc * pg.ic.r;

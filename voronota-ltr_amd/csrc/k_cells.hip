@@ -90,6 +90,8 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   out.pad = 0;
   const double4 s4 = p.sph[b];
   const double r = s4.w;
+  // adjoint seed dE/d(solid angle) (0 outside the standard branch)
+  double seed = 0.0;
   if (acc.added) {
     // types.rs:624-663
     const double diff = fabs(acc.sa_pos - acc.sa_neg);
@@ -104,9 +106,11 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
       }
       vol = (sas * r / 3.0) + acc.pv_pos - acc.pv_neg;
       const double full = 4.0 / 3.0 * kPi * r * r * r;
+      seed = -r * r * (p.sas_weight + p.volume_weight * r / 3.0);
       if (vol > full) {
         sas = 0.0;
         vol = acc.pv_pos - acc.pv_neg;
+        seed = 0.0;
       }
     } else {
       vol = acc.pv_pos - acc.pv_neg;
@@ -121,6 +125,7 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
     out.stage = 2;
   }
   p.cell[b] = out;
+  if (p.seed != nullptr) p.seed[b] = seed;
 }
 
 __global__ void k_flag_contacts(const PairRec* rec, int64_t n, uint32_t* flag) {

@@ -23,6 +23,7 @@
 #include "contour.h"
 #include "group_clip.h"
 #include "vltr_internal.h"
+#include "wave_group.h"
 
 namespace vltr {
 
@@ -56,22 +57,6 @@ struct ClipLds {
   int32_t item[kClipSlots];
   uint64_t mask[kClipSlots];
   GroupLds grp[kWave / kGroup];
-};
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// 16-lane group of a wavefront (the G of group_clip.h).
-struct WaveGroup {
-  int base, ln;
-  __device__ int lane() const { return ln; }
-  __device__ uint32_t ballot(bool p) const { return static_cast<uint32_t>((__ballot(p) >> base) & 0xFFFFull); }
-  __device__ double shfl(double v, int src) const { return __shfl(v, src, kGroup); }
-  __device__ int32_t shfl(int32_t v, int src) const { return __shfl(v, src, kGroup); }
-  __device__ void sync() const { wave_sync(); }
 };
 
 // Value of v in lane k (k uniform over the wavefront).
@@ -180,6 +165,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     PairGeom g;
     pair_setup(A, load_sph(sph4, w.cid[it & 0xff]), &g);
     p.rec[base + (it >> 8)] = record_uncut(g, p.mode);
+    if (p.topo != nullptr) p.topo[base + (it >> 8)].m = 0;
   }
 
   // Pairs to clip in decreasing cutting-set size, so that the four pairs
@@ -225,7 +211,9 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
       PairGeom g;
       pair_setup(A, load_sph(sph4, w.cid[it & 0xff]), &g);
       int st = kNone;
-      const PairRec rec = group_clip(G, gs, g, static_cast<int32_t>(a), M, 0ull, fetch, c_hex_s, c_hex_c, p.mode, &st);
+      const TopoOut tout{p.topo != nullptr ? p.topo + base + (it >> 8) : nullptr, sph4};
+      const PairRec rec = group_clip(G, gs, g, static_cast<int32_t>(a), M, 0ull, fetch, c_hex_s, c_hex_c, p.mode, &st,
+                                     p.topo != nullptr ? &tout : nullptr);
       if (G.lane() == 0) {
         if (st == kOverflow) {
           push_slow(p, base + (it >> 8));
@@ -291,7 +279,10 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       continue;
     }
     if ((mlo | mhi) == 0ull) {
-      if (ln == 0) p.rec[pidx] = record_uncut(g, p.mode);
+      if (ln == 0) {
+        p.rec[pidx] = record_uncut(g, p.mode);
+        if (p.topo != nullptr) p.topo[pidx].m = 0;
+      }
       continue;
     }
     struct {
@@ -305,7 +296,9 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       }
     } fetch{sph4, nbl, A};
     int st = kNone;
-    const PairRec rec = group_clip(G, gs, g, a, mlo, mhi, fetch, c_hex_s, c_hex_c, p.mode, &st);
+    const TopoOut tout{p.topo != nullptr ? p.topo + pidx : nullptr, sph4};
+    const PairRec rec =
+        group_clip(G, gs, g, a, mlo, mhi, fetch, c_hex_s, c_hex_c, p.mode, &st, p.topo != nullptr ? &tout : nullptr);
     if (ln == 0) {
       if (st == kOverflow) {
         push_slow(p, pidx);
@@ -335,9 +328,28 @@ __global__ __launch_bounds__(64) void k_cut_slow(Pipeline p) {
     const int64_t pi = p.overflow_list[t];
     const int32_t a = p.pair_a[pi], b = p.pair_b[pi];
     const int32_t* nbl = p.nb + static_cast<int64_t>(a) * p.nb_cap;
-    int st;
-    p.rec[pi] = make_record(reinterpret_cast<const double*>(p.sph), a, b, nbl, p.nb_cnt[a], p.mode, c_hex_s, c_hex_c,
-                            ct, out, &st);
+    const double* sph4 = reinterpret_cast<const double*>(p.sph);
+    ClipOut co;
+    const int st = clip(sph4, a, b, nbl, p.nb_cnt[a], c_hex_s, c_hex_c, ct, out, &co);
+    p.rec[pi] = record_of(st, co, load_sph(sph4, a), load_sph(sph4, b), out, p.mode);
+    if (p.topo != nullptr && st == kContact) {
+      ContactTopo& tp = p.topo[pi];
+      if (co.m > kTopoCap) {
+        tp.m = kTopoTooLong;
+      } else {
+        uint32_t roots = 0u;
+        for (int k = 0; k < co.m; ++k) {
+          tp.l[k] = out.l[k];
+          tp.r[k] = out.r[k];
+          if (out.l[k] == a || out.r[k] == a) {
+            const int32_t c = out.l[k] == a ? out.r[k] : out.l[k];
+            if (circle_root(out.p(k), load_sph(sph4, a).c, load_sph(sph4, b).c, load_sph(sph4, c).c)) roots |= 1u << k;
+          }
+        }
+        tp.roots = roots;
+        tp.m = co.m;
+      }
+    }
   }
 }
 

@@ -104,7 +104,7 @@ struct vltr_context {
   // device buffers
   DevBuf balls, groups, sph, seg_of, segs, bkey, brank, bcnt, sorted_id, sorted_sph, small, nb, nb_cnt, excluded,
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
-      in_cnt, incoming, seg_out;
+      in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad;
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
@@ -202,6 +202,11 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
 struct RunExtras {
   const uint8_t* user_excl = nullptr;
   std::function<hipError_t(const Pipeline&, hipStream_t)> post_cells;
+  // adjoint runs: cut topologies (sized here once the pair count is known),
+  // cell seeds (n) and the energy weights
+  DevBuf* topo_buf = nullptr;
+  double* seed = nullptr;
+  double sas_weight = 1.0, volume_weight = 1.0;
 };
 
 int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic_box* pbox, const int32_t* d_groups,
@@ -226,6 +231,9 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   p.groups = d_groups;
   p.n_groups = n_groups;
   p.user_excl = extras ? extras->user_excl : nullptr;
+  p.seed = extras ? extras->seed : nullptr;
+  p.sas_weight = extras ? extras->sas_weight : 1.0;
+  p.volume_weight = extras ? extras->volume_weight : 1.0;
 
   // One cell-hash segment per structure: its own bucket range, sized for it.
   std::vector<GridSeg> segs(static_cast<size_t>(p.n_segs));
@@ -304,6 +312,10 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->out_contacts.ensure(p.pair_cap * sizeof(vltr_contact)));
   VLTR_HIP(ctx->out_cells.ensure(std::max<int64_t>(n, 1) * sizeof(vltr_cell)));
   fill_pointers(ctx, p);
+  if (extras && extras->topo_buf) {
+    VLTR_HIP(extras->topo_buf->ensure(p.pair_cap * sizeof(ContactTopo)));
+    p.topo = extras->topo_buf->as<ContactTopo>();
+  }
   VLTR_HIP(launch_pairs(p, s));
   VLTR_HIP(hipEventRecord(ctx->ev[3], s));
   VLTR_HIP(launch_clip(p, s));
@@ -576,6 +588,70 @@ int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64
   return VLTR_OK;
 }
 
+// Energy and its gradient (forward + adjoint) on device-resident balls.
+int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
+                                const int32_t* d_types, const double* w16, double sas_weight, double volume_weight,
+                                double* d_grad, double* energy) {
+  if (!ctx || !w16 || !energy || (n > 0 && (!d_balls || !d_types || !d_grad)))
+    return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  *energy = 0.0;
+  ctx->last_contacts = ctx->last_cells = 0;
+  if (n == 0) return VLTR_OK;
+  VLTR_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
+  VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, d_balls, n * sizeof(vltr_ball), hipMemcpyDeviceToDevice, s));
+  VLTR_HIP(ctx->seed.ensure(n * sizeof(double)));
+  VLTR_HIP(ctx->energy_part.ensure(adjoint_blocks() * sizeof(double)));
+  RunExtras ex;
+  ex.topo_buf = &ctx->topo;
+  ex.seed = ctx->seed.as<double>();
+  ex.sas_weight = sas_weight;
+  ex.volume_weight = volume_weight;
+  double* part = ctx->energy_part.as<double>();
+  std::vector<double> w(w16, w16 + 16);
+  ex.post_cells = [=](const Pipeline& p, hipStream_t st) -> hipError_t {
+    return launch_adjoint(p, d_types, w.data(), d_grad, part, st);
+  };
+  size_t nc = 0, ncell = 0;
+  const int rc = run_pipeline(ctx, static_cast<int64_t>(n), probe, nullptr, nullptr, 0, 0, &nc, &ncell, nullptr, 0, &ex);
+  if (rc != VLTR_OK) return rc;
+  std::vector<double> parts(static_cast<size_t>(adjoint_blocks()));
+  VLTR_HIP(hipMemcpy(parts.data(), part, parts.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double e = 0.0;
+  for (double v : parts) e += v;
+  *energy = e;
+  ctx->last_contacts = nc;
+  ctx->last_cells = ncell;
+  return VLTR_OK;
+}
+
+int vltr_energy_gradient(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe, const int32_t* types,
+                         const double* w16, double sas_weight, double volume_weight, double* energy, double* grad) {
+  if (!ctx || !w16 || !energy || (n > 0 && (!balls || !types || !grad)))
+    return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  *energy = 0.0;
+  if (n == 0) return VLTR_OK;
+  for (size_t i = 0; i < n; ++i)
+    if (types[i] < 0 || types[i] > 3) return fail(VLTR_ERR_INVALID_ARGUMENT, "types must be in 0..3");
+  VLTR_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  DevBuf db;
+  VLTR_HIP(db.ensure(n * sizeof(vltr_ball)));
+  VLTR_HIP(ctx->types.ensure(n * sizeof(int32_t)));
+  VLTR_HIP(ctx->grad.ensure(3 * n * sizeof(double)));
+  VLTR_HIP(hipMemcpyAsync(db.ptr, balls, n * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
+  VLTR_HIP(hipMemcpyAsync(ctx->types.ptr, types, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  int rc = vltr_energy_gradient_device(ctx, db.as<vltr_ball>(), n, probe, ctx->types.as<int32_t>(), w16, sas_weight,
+                                       volume_weight, ctx->grad.as<double>(), energy);
+  if (rc == VLTR_OK) {
+    if (hipMemcpy(grad, ctx->grad.ptr, 3 * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(VLTR_ERR_DEVICE, "gradient download");
+  }
+  db.release();
+  return rc;
+}
+
 size_t vltr_batch_num_structs(const vltr_batch_result* r) { return r ? r->n_structs : 0; }
 const vltr_contact* vltr_batch_contacts(const vltr_batch_result* r, size_t k, size_t* count) {
   if (!r || k >= r->n_structs) {
@@ -634,7 +710,8 @@ void vltr_context_destroy(vltr_context* ctx) {
                     &ctx->sorted_id, &ctx->sorted_sph, &ctx->small, &ctx->nb, &ctx->nb_cnt, &ctx->excluded,
                     &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec, &ctx->cell, &ctx->overflow_list, &ctx->wide_list,
                     &ctx->flags, &ctx->scan_tmp, &ctx->out_contacts, &ctx->out_cells, &ctx->slow_ws,
-                    &ctx->in_cnt, &ctx->incoming, &ctx->seg_out})
+                    &ctx->in_cnt, &ctx->incoming, &ctx->seg_out, &ctx->topo, &ctx->seed, &ctx->energy_part,
+                    &ctx->types, &ctx->grad})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)

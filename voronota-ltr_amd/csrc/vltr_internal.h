@@ -8,6 +8,7 @@
 #include <cstdint>
 
 #include "contour.h"
+#include "group_clip.h"
 #include "vltr_geom.h"
 #include "voronota_ltr.h"
 
@@ -83,6 +84,9 @@ struct Pipeline {
   double* slow_ws;         // workspace of the slow kernel
   int64_t slow_ws_pairs;   // its lanes (one workspace region each)
 
+  ContactTopo* topo;       // pair_cap or null: cut topology per pair (adjoint runs)
+  double* seed;            // n or null: dE/d(solid angle) per cell (adjoint runs)
+  double sas_weight, volume_weight;  // alpha, beta of the adjoint's energy
   uint32_t* in_cnt;        // n: pairs (a, b), a < b, received by ball b
   int32_t* incoming;       // n * nb_cap: their pair indices (any order)
 };
@@ -120,6 +124,14 @@ hipError_t launch_fallbacks(const Pipeline& p, hipStream_t s);
 int64_t slow_lanes(int nb_cap);
 void set_hexagon_constants(const double* s_half5, const double* c_half5);
 hipError_t upload_hexagon_constants(hipStream_t s);
+
+// k_adjoint.hip: grad (3 n, zeroed here) += dE/d(centres) for
+// E = sum_contacts W[t_a][t_b] area + alpha sum sas + beta sum volume, from the
+// records, topologies and seeds of the last full run; partial energy sums
+// per block into part[0 .. adjoint_blocks()).
+hipError_t launch_adjoint(const Pipeline& p, const int32_t* types, const double* w16, double* grad, double* part,
+                          hipStream_t s);
+int64_t adjoint_blocks();
 
 // k_cells.hip
 hipError_t launch_cells(const Pipeline& p, hipStream_t s);

@@ -81,6 +81,10 @@ SIGNATURES = {
     "vltr_batch_free": (None, [_vp]),
     "vltr_compute_tessellation_batched_device": (C.c_int, [_vp, _vp, _u64p, _sz, C.c_double, C.c_int]),
     "vltr_context_batch_offsets": (C.c_int, [_vp, _sz, _u64p, _u64p]),
+    "vltr_energy_gradient": (C.c_int, [_vp, _vp, _sz, C.c_double, _i32p, _f64p, C.c_double, C.c_double, _f64p,
+                                       _f64p]),
+    "vltr_energy_gradient_device": (C.c_int, [_vp, _vp, _sz, C.c_double, _vp, _f64p, C.c_double, C.c_double, _vp,
+                                              _f64p]),
     "vltr_updateable_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
     "vltr_updateable_destroy": (None, [_vp]),
     "vltr_updateable_init": (C.c_int, [_vp, _vp, _sz, C.c_double, C.POINTER(VltrPeriodicBox), C.POINTER(C.c_int)]),
@@ -395,3 +399,23 @@ class Updateable:
         k = self._lib.vltr_updateable_last_timings(self._h, buf, 7)
         names = ["grid", "neighbors", "pairs", "clip", "slow", "cells", "total"]
         return {names[i]: buf[i] for i in range(k)}
+
+
+def energy_gradient(balls, probe, types, W, sas_weight=1.0, volume_weight=1.0, device=0):
+    """(E, dE/dcentres (n, 3)) for E = sum_contacts W[t_a][t_b] area +
+    sas_weight * sum SAS + volume_weight * sum volume."""
+    lib = load_library()
+    ctx = context(device)
+    b = _balls_array(balls)
+    t = np.ascontiguousarray(np.asarray(types, dtype=np.int32).reshape(-1))
+    w = np.ascontiguousarray(np.asarray(W, dtype=np.float64).reshape(16))
+    if len(t) != len(b):
+        raise ValueError("one type per ball")
+    e = C.c_double()
+    g = np.zeros((len(b), 3), dtype=np.float64)
+    rc = lib.vltr_energy_gradient(ctx, b.ctypes.data_as(_vp) if len(b) else None, len(b), float(probe),
+                                  t.ctypes.data_as(_i32p) if len(t) else None, w.ctypes.data_as(_f64p),
+                                  float(sas_weight), float(volume_weight), C.byref(e),
+                                  g.ctypes.data_as(_f64p) if len(b) else None)
+    check(rc, "vltr_energy_gradient")
+    return e.value, g

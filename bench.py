@@ -82,8 +82,15 @@ def main():
     dbuf = _hip.DeviceBuffer(balls.nbytes)
     dbuf.upload(np.ascontiguousarray(balls))
 
+    # contacts-only runs with group 1 for x < 0 and 0 otherwise: only contacts
+    # between the two halves are kept
+    groups = np.ascontiguousarray((balls[:, 0] < 0.0).astype(np.int32))
+    gbuf = _hip.DeviceBuffer(groups.nbytes)
+    gbuf.upload(groups)
+
     def step(contacts_only=0):
-        rc = lib.vltr_compute_tessellation_device(ctx, dbuf.ptr, args.n, args.probe, None, 0, contacts_only)
+        gp, ng = (gbuf.ptr, args.n) if contacts_only else (None, 0)
+        rc = lib.vltr_compute_tessellation_device(ctx, dbuf.ptr, args.n, args.probe, gp, ng, contacts_only)
         _native.check(rc, "vltr_compute_tessellation_device")
 
     phases = ["grid", "neighbors", "pairs", "clip", "slow", "cells", "total"]
@@ -162,7 +169,8 @@ def main():
         "phases_ms": ph_full,
         "counts": counts,
         "contacts_only": {"balls_per_s": round(args.n / (ms_co / 1e3), 1), "ms_per_step": round(ms_co, 4),
-                          "phases_ms": ph_co, "contacts": counts_co["contacts"]},
+                          "phases_ms": ph_co, "contacts": counts_co["contacts"], "pairs": counts_co["pairs"],
+                          "groups": "1 for x < 0, else 0 (inter-group contacts only)"},
     }
     if batched:
         out["batched"] = batched

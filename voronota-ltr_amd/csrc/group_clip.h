@@ -54,9 +54,21 @@ struct ContactTopo {
 constexpr int32_t kTopoNone = -1;
 constexpr int32_t kTopoTooLong = -2;
 
+// Restricted contour of one contact as computed (network output,
+// contact.rs:120-177): m points (0: uncut full circle, kTopoTooLong: more than
+// kTopoCap), their (left, right) ids, coordinates and arc angles.
+struct ContactNet {
+  int32_t m;
+  int32_t pad;
+  double r_ic;  // radius of the intersection circle
+  int32_t l[kTopoCap], r[kTopoCap];
+  double x[kTopoCap], y[kTopoCap], z[kTopoCap], ang[kTopoCap];
+};
+
 struct TopoOut {
-  ContactTopo* topo;
+  ContactTopo* topo;  // may be null
   const double* sph4;
+  ContactNet* net;    // may be null
 };
 
 // Radical plane of a and a cutting neighbor (plane_of), plus unit(pn) and the
@@ -484,7 +496,24 @@ VLTR_HD PairRec group_clip(G& g, GroupLds* s, const PairGeom& pg, int32_t a_id, 
   const bool central = g.ballot(reflex) != 0u || g.ballot(fail) == 0u;
   if (area <= 0.0) return empty_record();
   *status = kContact;
-  if (tout != nullptr) write_topology(g, s, pg, a_id, m, *tout);
+  if (tout != nullptr && tout->topo != nullptr) write_topology(g, s, pg, a_id, m, *tout);
+  if (tout != nullptr && tout->net != nullptr) {
+    for (int h = 0; h < 2; ++h) {
+      const int t = j + h * kGroup;
+      if (t >= m) continue;
+      ContactNet* nt = tout->net;
+      nt->l[t] = s->l[t];
+      nt->r[t] = s->r[t];
+      nt->x[t] = s->x[t];
+      nt->y[t] = s->y[t];
+      nt->z[t] = s->z[t];
+      nt->ang[t] = s->ang[t];
+    }
+    if (j == 0) {
+      tout->net->m = m;
+      tout->net->r_ic = pg.ic.r;
+    }
+  }
   PairRec rec = empty_record();
   rec.area = area;
   rec.arc_length = sum_arc * pg.ic.r;

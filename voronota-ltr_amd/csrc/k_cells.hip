@@ -172,6 +172,26 @@ __global__ void k_emit_cells(Pipeline p, const uint32_t* pos, vltr_cell* out) {
   out[k] = c;
 }
 
+__global__ void k_emit_net(Pipeline p, const uint32_t* pos, ContactNet* out) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= p.n_pairs) return;
+  const uint32_t k = pos[i];
+  if (pos[i + 1] == k) return;
+  const ContactNet& src = p.net[i];
+  ContactNet& dst = out[k];
+  const int m = src.m;
+  dst.m = m;
+  dst.r_ic = src.r_ic;
+  for (int t = 0; t < m && t < kTopoCap; ++t) {
+    dst.l[t] = src.l[t];
+    dst.r[t] = src.r[t];
+    dst.x[t] = src.x[t];
+    dst.y[t] = src.y[t];
+    dst.z[t] = src.z[t];
+    dst.ang[t] = src.ang[t];
+  }
+}
+
 __global__ void k_seg_offsets(Pipeline p, const uint32_t* pos, int what, uint32_t* out) {
   const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (k > p.n_segs) return;
@@ -273,6 +293,11 @@ hipError_t launch_mark_more(const Pipeline& p, const int32_t* changed, int64_t k
 }
 hipError_t launch_mask_clear(uint8_t* mask, const uint8_t* where, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_mask_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, mask, where, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_emit_net(const Pipeline& p, const uint32_t* pos, ContactNet* out, hipStream_t s) {
+  if (p.n_pairs > 0) hipLaunchKernelGGL(k_emit_net, dim3(blocks_for(p.n_pairs, 256)), dim3(256), 0, s, p, pos, out);
   return hipGetLastError();
 }
 

@@ -166,6 +166,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     pair_setup(A, load_sph(sph4, w.cid[it & 0xff]), &g);
     p.rec[base + (it >> 8)] = record_uncut(g, p.mode);
     if (p.topo != nullptr) p.topo[base + (it >> 8)].m = 0;
+    if (p.net != nullptr) p.net[base + (it >> 8)].m = 0;
   }
 
   // Pairs to clip in decreasing cutting-set size, so that the four pairs
@@ -211,9 +212,11 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
       PairGeom g;
       pair_setup(A, load_sph(sph4, w.cid[it & 0xff]), &g);
       int st = kNone;
-      const TopoOut tout{p.topo != nullptr ? p.topo + base + (it >> 8) : nullptr, sph4};
+      const int64_t pidx = base + (it >> 8);
+      const TopoOut tout{p.topo != nullptr ? p.topo + pidx : nullptr, sph4, p.net != nullptr ? p.net + pidx : nullptr};
+      const bool want = p.topo != nullptr || p.net != nullptr;
       const PairRec rec = group_clip(G, gs, g, static_cast<int32_t>(a), M, 0ull, fetch, c_hex_s, c_hex_c, p.mode, &st,
-                                     p.topo != nullptr ? &tout : nullptr);
+                                     want ? &tout : nullptr);
       if (G.lane() == 0) {
         if (st == kOverflow) {
           push_slow(p, base + (it >> 8));
@@ -282,6 +285,7 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       if (ln == 0) {
         p.rec[pidx] = record_uncut(g, p.mode);
         if (p.topo != nullptr) p.topo[pidx].m = 0;
+        if (p.net != nullptr) p.net[pidx].m = 0;
       }
       continue;
     }
@@ -296,9 +300,9 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       }
     } fetch{sph4, nbl, A};
     int st = kNone;
-    const TopoOut tout{p.topo != nullptr ? p.topo + pidx : nullptr, sph4};
-    const PairRec rec =
-        group_clip(G, gs, g, a, mlo, mhi, fetch, c_hex_s, c_hex_c, p.mode, &st, p.topo != nullptr ? &tout : nullptr);
+    const TopoOut tout{p.topo != nullptr ? p.topo + pidx : nullptr, sph4, p.net != nullptr ? p.net + pidx : nullptr};
+    const bool want = p.topo != nullptr || p.net != nullptr;
+    const PairRec rec = group_clip(G, gs, g, a, mlo, mhi, fetch, c_hex_s, c_hex_c, p.mode, &st, want ? &tout : nullptr);
     if (ln == 0) {
       if (st == kOverflow) {
         push_slow(p, pidx);
@@ -348,6 +352,23 @@ __global__ __launch_bounds__(64) void k_cut_slow(Pipeline p) {
         }
         tp.roots = roots;
         tp.m = co.m;
+      }
+    }
+    if (p.net != nullptr && st == kContact) {
+      ContactNet& nt = p.net[pi];
+      if (co.m > kTopoCap) {
+        nt.m = kTopoTooLong;
+      } else {
+        for (int k = 0; k < co.m; ++k) {
+          nt.l[k] = out.l[k];
+          nt.r[k] = out.r[k];
+          nt.x[k] = out.x[k];
+          nt.y[k] = out.y[k];
+          nt.z[k] = out.z[k];
+          nt.ang[k] = out.ang[k];
+        }
+        nt.m = co.m;
+        nt.r_ic = co.ic.r;
       }
     }
   }

@@ -104,7 +104,7 @@ struct vltr_context {
   // device buffers
   DevBuf balls, groups, sph, seg_of, segs, bkey, brank, bcnt, sorted_id, sorted_sph, small, nb, nb_cnt, excluded,
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
-      in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad;
+      in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad, net, net_out;
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
@@ -205,6 +205,9 @@ struct RunExtras {
   // adjoint runs: cut topologies (sized here once the pair count is known),
   // cell seeds (n) and the energy weights
   DevBuf* topo_buf = nullptr;
+  // network output: restricted contours per pair, compacted per contact
+  DevBuf* net_buf = nullptr;
+  DevBuf* net_out = nullptr;
   double* seed = nullptr;
   double sas_weight = 1.0, volume_weight = 1.0;
 };
@@ -316,6 +319,11 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     VLTR_HIP(extras->topo_buf->ensure(p.pair_cap * sizeof(ContactTopo)));
     p.topo = extras->topo_buf->as<ContactTopo>();
   }
+  if (extras && extras->net_buf && extras->net_out) {
+    VLTR_HIP(extras->net_buf->ensure(p.pair_cap * sizeof(ContactNet)));
+    VLTR_HIP(extras->net_out->ensure(p.pair_cap * sizeof(ContactNet)));
+    p.net = extras->net_buf->as<ContactNet>();
+  }
   VLTR_HIP(launch_pairs(p, s));
   VLTR_HIP(hipEventRecord(ctx->ev[3], s));
   VLTR_HIP(launch_clip(p, s));
@@ -331,6 +339,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(hipMemcpyAsync(host_words + 3, flags + P, 4, hipMemcpyDeviceToHost, s));
   uint32_t* seg_out = ctx->seg_out.as<uint32_t>();
   if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 0, seg_out, s));
+  if (p.net != nullptr) VLTR_HIP(launch_emit_net(p, flags, extras->net_out->as<ContactNet>(), s));
   if (mode == 0) {
     VLTR_HIP(launch_flag_cells(p, flags, s));
     VLTR_HIP(launch_scan_u32(flags, n, ctx->scan_tmp.as<uint32_t>(), s));
@@ -408,10 +417,84 @@ void periodic_dedup_contacts_only(std::vector<vltr_contact>& v, uint64_t n) {
   v.swap(out);
 }
 
+// Tessellation network (contact.rs:120-177, tessellation.rs:334-401) from
+// the restricted contours of the contacts in pair order: per contact point an
+// edge (a, b, right id or none for an arc) and a vertex (a, b, left, right);
+// an uncut contact is one arc edge. Ids become canonical (periodic), sorted
+// within each element; elements are stably sorted by ids and the first of
+// each run is kept (the reference's sort_unstable leaves that choice open).
+int build_network(vltr_result* r, const std::vector<ContactNet>& nets, uint64_t periodic_n) {
+  struct E {
+    uint64_t ids[3];
+    double len;
+  };
+  struct Vx {
+    uint64_t ids[4];
+    double x, y, z;
+  };
+  auto canon = [&](uint64_t id) { return (periodic_n && id != VLTR_NULL_ID) ? id % periodic_n : id; };
+  std::vector<E> edges;
+  std::vector<Vx> verts;
+  for (size_t k = 0; k < nets.size(); ++k) {
+    const ContactNet& nt = nets[k];
+    const vltr_contact& c = r->contacts[k];
+    const uint64_t a = c.id_a, b = c.id_b;
+    if (nt.m == kTopoTooLong) return fail(VLTR_ERR_INTERNAL, "network output: a contour has more than 32 points");
+    if (nt.m == 0) {
+      edges.push_back(E{{a, b, VLTR_NULL_ID}, c.arc_length});
+      continue;
+    }
+    for (int t = 0; t < nt.m; ++t) {
+      const int u = t + 1 < nt.m ? t + 1 : 0;
+      const bool arc = static_cast<uint64_t>(nt.r[t]) == a;
+      double len;
+      if (arc) {
+        len = nt.ang[t] * nt.r_ic;
+      } else {
+        const double dx = nt.x[t] - nt.x[u], dy = nt.y[t] - nt.y[u], dz = nt.z[t] - nt.z[u];
+        len = std::sqrt((dx * dx + dy * dy) + dz * dz);
+      }
+      edges.push_back(E{{a, b, arc ? VLTR_NULL_ID : static_cast<uint64_t>(nt.r[t])}, len});
+      const uint64_t v3 = static_cast<uint64_t>(nt.l[t]) == a ? VLTR_NULL_ID : static_cast<uint64_t>(nt.l[t]);
+      const uint64_t v4 = arc ? VLTR_NULL_ID : static_cast<uint64_t>(nt.r[t]);
+      verts.push_back(Vx{{a, b, v3, v4}, nt.x[t], nt.y[t], nt.z[t]});
+    }
+  }
+  for (auto& e : edges) {
+    for (auto& id : e.ids) id = canon(id);
+    std::sort(e.ids, e.ids + 3);
+  }
+  for (auto& v : verts) {
+    for (auto& id : v.ids) id = canon(id);
+    std::sort(v.ids, v.ids + 4);
+  }
+  std::stable_sort(edges.begin(), edges.end(),
+                   [](const E& x, const E& y) { return std::lexicographical_compare(x.ids, x.ids + 3, y.ids, y.ids + 3); });
+  std::stable_sort(verts.begin(), verts.end(),
+                   [](const Vx& x, const Vx& y) { return std::lexicographical_compare(x.ids, x.ids + 4, y.ids, y.ids + 4); });
+  r->has_network = true;
+  for (size_t i = 0; i < edges.size(); ++i)
+    if (i == 0 || !std::equal(edges[i].ids, edges[i].ids + 3, edges[i - 1].ids)) {
+      vltr_cell_edge e;
+      for (int k = 0; k < 3; ++k) e.ball_indices[k] = edges[i].ids[k];
+      e.length = edges[i].len;
+      r->edges.push_back(e);
+    }
+  for (size_t i = 0; i < verts.size(); ++i)
+    if (i == 0 || !std::equal(verts[i].ids, verts[i].ids + 4, verts[i - 1].ids)) {
+      vltr_cell_vertex v;
+      for (int k = 0; k < 4; ++k) v.ball_indices[k] = verts[i].ids[k];
+      v.x = verts[i].x;
+      v.y = verts[i].y;
+      v.z = verts[i].z;
+      r->vertices.push_back(v);
+    }
+  return VLTR_OK;
+}
+
 int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe, const vltr_periodic_box* pbox,
                const int32_t* groups, size_t n_groups, int with_vertices, int mode, vltr_result** out) {
   if (!ctx || !out || (n > 0 && !balls)) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
-  if (with_vertices) return fail(VLTR_ERR_INVALID_ARGUMENT, "with_cell_vertices is not supported by this build");
   if (n > (size_t(1) << 31) / 27) return fail(VLTR_ERR_INVALID_ARGUMENT, "too many balls");
   *out = nullptr;
   auto* r = new (std::nothrow) vltr_result();
@@ -446,8 +529,14 @@ int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe
       dg = ctx->groups.as<int32_t>();
     }
     size_t nc = 0, ncell = 0;
+    RunExtras ex;
+    const bool net = with_vertices && mode == 0;
+    if (net) {
+      ex.net_buf = &ctx->net;
+      ex.net_out = &ctx->net_out;
+    }
     rc = run_pipeline(ctx, static_cast<int64_t>(n), probe, pbox, dg, dg ? static_cast<int64_t>(n_groups) : 0, mode, &nc,
-                      &ncell);
+                      &ncell, nullptr, 0, net ? &ex : nullptr);
     if (rc != VLTR_OK) break;
     r->contacts.resize(nc);
     r->cells.resize(ncell);
@@ -461,9 +550,22 @@ int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe
       rc = fail(VLTR_ERR_DEVICE, "cell download");
       break;
     }
+    std::vector<ContactNet> nets;
+    if (net && nc) {
+      nets.resize(nc);
+      if (hipMemcpyAsync(nets.data(), ctx->net_out.ptr, nc * sizeof(ContactNet), hipMemcpyDeviceToHost, s) !=
+          hipSuccess) {
+        rc = fail(VLTR_ERR_DEVICE, "contour download");
+        break;
+      }
+    }
     if (hipStreamSynchronize(s) != hipSuccess) {
       rc = fail(VLTR_ERR_DEVICE, "stream synchronize");
       break;
+    }
+    if (net) {
+      rc = build_network(r, nets, pbox ? n : 0);
+      if (rc != VLTR_OK) break;
     }
     if (pbox) {
       if (mode == 0)
@@ -711,7 +813,7 @@ void vltr_context_destroy(vltr_context* ctx) {
                     &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec, &ctx->cell, &ctx->overflow_list, &ctx->wide_list,
                     &ctx->flags, &ctx->scan_tmp, &ctx->out_contacts, &ctx->out_cells, &ctx->slow_ws,
                     &ctx->in_cnt, &ctx->incoming, &ctx->seg_out, &ctx->topo, &ctx->seed, &ctx->energy_part,
-                    &ctx->types, &ctx->grad})
+                    &ctx->types, &ctx->grad, &ctx->net, &ctx->net_out})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)

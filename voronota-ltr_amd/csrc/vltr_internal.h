@@ -85,6 +85,7 @@ struct Pipeline {
   int64_t slow_ws_pairs;   // its lanes (one workspace region each)
 
   ContactTopo* topo;       // pair_cap or null: cut topology per pair (adjoint runs)
+  ContactNet* net;         // pair_cap or null: restricted contours (network output)
   double* seed;            // n or null: dE/d(solid angle) per cell (adjoint runs)
   double sas_weight, volume_weight;  // alpha, beta of the adjoint's energy
   uint32_t* in_cnt;        // n: pairs (a, b), a < b, received by ball b
@@ -143,6 +144,9 @@ hipError_t launch_emit_cells(const Pipeline& p, const uint32_t* pos, vltr_cell* 
 // pos = scanned contact flags) or first cell (what = 1, pos = scanned cell
 // flags) in the compacted output, k = 0..n_segs (out[n_segs] = total).
 hipError_t launch_seg_offsets(const Pipeline& p, const uint32_t* pos, int what, uint32_t* out, hipStream_t s);
+// out[pos[i]] = p.net[i] for every pair i with a contact (pos = scanned contact
+// flags).
+hipError_t launch_emit_net(const Pipeline& p, const uint32_t* pos, ContactNet* out, hipStream_t s);
 // Cells under the updateable's rules (updateable.rs:633-727), in place in
 // p.cell and copied to state: full = 1 after init or a full reinit, else only
 // balls with affected[i] != 0 take the new record (the rest keep state[i]);

@@ -30,7 +30,8 @@ void hexagon(double* s, double* c) {
 // resumes, all 16 have contributed. Values alternate between two slot banks,
 // so a lane can write the next collective before the others read this one.
 struct FiberGroup {
-  static constexpr int W = vltr::kGroup;
+  static constexpr int W = vltr::kGroup;  // at most W lanes; `w` run
+  int w = W;
   ucontext_t sched;
   ucontext_t fib[W];
   std::vector<char> stack[W];
@@ -42,7 +43,9 @@ struct FiberGroup {
   }
 };
 
-struct HostLane {
+template <int GS>
+struct HostLaneT {
+  static constexpr int kSize = GS;
   FiberGroup* f;
   int ln;
   int lane() const { return ln; }
@@ -51,14 +54,14 @@ struct HostLane {
     const int b = f->round[ln]++ & 1;
     f->slot[b][ln] = v;
     yield();
-    return f->slot[b][src & (FiberGroup::W - 1)];
+    return f->slot[b][src & (GS - 1)];
   }
   uint32_t ballot(bool p) {
     const int b = f->round[ln]++ & 1;
     f->slot[b][ln] = p ? 1u : 0u;
     yield();
     uint32_t m = 0;
-    for (int k = 0; k < FiberGroup::W; ++k) m |= static_cast<uint32_t>(f->slot[b][k] & 1u) << k;
+    for (int k = 0; k < GS; ++k) m |= static_cast<uint32_t>(f->slot[b][k] & 1u) << k;
     return m;
   }
   double shfl(double v, int src) {
@@ -75,6 +78,7 @@ struct HostLane {
     yield();
   }
 };
+using HostLane = HostLaneT<vltr::kGroup>;
 
 struct GroupTask {
   const double* sph4;
@@ -141,36 +145,76 @@ bool run_group(GroupTask& t, vltr::PairRec* out, int* st) {
   *st = t.st[0];
   return agree;
 }
-// Runs fn(lane) on 16 lock-stepped fibers.
-std::function<void(HostLane&)>* g_fn = nullptr;
-FiberGroup* g_fg = nullptr;
-void lane_main_fn(int ln) {
-  HostLane lane{g_fg, ln};
-  (*g_fn)(lane);
-  g_fg->done[ln] = true;
-}
-void run_fibers(std::function<void(HostLane&)> fn) {
+// Runs fn(lane) on GS lock-stepped fibers.
+template <int GS>
+struct FiberRun {
+  static std::function<void(HostLaneT<GS>&)>* fn;
+  static FiberGroup* fg;
+  static void lane_main(int ln) {
+    HostLaneT<GS> lane{fg, ln};
+    (*fn)(lane);
+    fg->done[ln] = true;
+  }
+};
+template <int GS>
+std::function<void(HostLaneT<GS>&)>* FiberRun<GS>::fn = nullptr;
+template <int GS>
+FiberGroup* FiberRun<GS>::fg = nullptr;
+
+template <int GS>
+void run_fibers_n(std::function<void(HostLaneT<GS>&)> fn) {
   static FiberGroup fg;
-  g_fn = &fn;
-  g_fg = &fg;
-  for (int k = 0; k < FiberGroup::W; ++k) {
+  FiberRun<GS>::fn = &fn;
+  FiberRun<GS>::fg = &fg;
+  for (int k = 0; k < GS; ++k) {
     fg.round[k] = 0;
     fg.done[k] = false;
     getcontext(&fg.fib[k]);
     fg.fib[k].uc_stack.ss_sp = fg.stack[k].data();
     fg.fib[k].uc_stack.ss_size = fg.stack[k].size();
     fg.fib[k].uc_link = &fg.sched;
-    makecontext(&fg.fib[k], reinterpret_cast<void (*)()>(lane_main_fn), 1, k);
+    makecontext(&fg.fib[k], reinterpret_cast<void (*)()>(&FiberRun<GS>::lane_main), 1, k);
   }
   for (;;) {
     int running = 0;
-    for (int k = 0; k < FiberGroup::W; ++k) {
+    for (int k = 0; k < GS; ++k) {
       if (fg.done[k]) continue;
       swapcontext(&fg.sched, &fg.fib[k]);
       ++running;
     }
     if (running == 0) break;
   }
+}
+void run_fibers(std::function<void(HostLane&)> fn) { run_fibers_n<vltr::kGroup>(fn); }
+
+// One masked pair through group_clip on GS fibers; false when the lanes
+// disagree (a bug).
+template <int GS>
+bool group_record(const double* sph4, const int32_t* nbl, const vltr::PairGeom& g, int32_t a, uint64_t m,
+                  const double* hs, const double* hc, int mode, vltr::PairRec* out, int* status) {
+  static vltr::GroupLdsT<2 * GS> lds;
+  vltr::PairRec rec[GS];
+  int st[GS];
+  const vltr::Sph A = g.a;
+  run_fibers_n<GS>([&](HostLaneT<GS>& lane) {
+    struct {
+      const double* sph4;
+      const int32_t* nbl;
+      vltr::Sph A;
+      double thr(int k) const { return vltr::norm2(vltr::plane_of(A, vltr::load_sph(sph4, nbl[k])).pc - A.c); }
+      vltr::CutPlane plane(int k) const {
+        const vltr::Plane pl = vltr::plane_of(A, vltr::load_sph(sph4, nbl[k]));
+        return vltr::CutPlane{pl.pc, pl.pn, vltr::unit(pl.pn), nbl[k]};
+      }
+    } fetch{sph4, nbl, A};
+    rec[lane.ln] = vltr::group_clip(lane, &lds, g, a, m, 0ull, fetch, hs, hc, mode, &st[lane.ln]);
+  });
+  bool agree = true;
+  for (int k = 1; k < GS; ++k)
+    if (st[k] != st[0] || std::memcmp(&rec[k], &rec[0], sizeof(vltr::PairRec)) != 0) agree = false;
+  *out = rec[0];
+  *status = st[0];
+  return agree;
 }
 
 struct HostAcc {
@@ -385,7 +429,7 @@ int clip_host_records_staged(const double* sph4, const int64_t* nb_off, const in
 // central, bit3 decided by classification, bit4 clipped by the group, bit5
 // redone by the one-lane code, bit6 the 16 lanes disagreed (a bug).
 int clip_host_records_group(const double* sph4, const int64_t* nb_off, const int32_t* nb_idx, const int32_t* pairs,
-                            int64_t P, int mode, double* out, uint32_t* flags) {
+                            int64_t P, int mode, double* out, uint32_t* flags, int group_size) {
   double hs[5], hc[5];
   hexagon(hs, hc);
   std::vector<double> ws(5 * 1024 + 6 * 2048);
@@ -394,7 +438,6 @@ int clip_host_records_group(const double* sph4, const int64_t* nb_off, const int
                      reinterpret_cast<int32_t*>(ws.data() + 3072), reinterpret_cast<int32_t*>(ws.data() + 4096)};
   vltr::GlobalArc ga{2048, w2, w2 + 2048, w2 + 4096, w2 + 6144, reinterpret_cast<int32_t*>(w2 + 8192),
                      reinterpret_cast<int32_t*>(w2 + 10240)};
-  GroupTask* task = new GroupTask;
   for (int64_t t = 0; t < P; ++t) {
     const int32_t a = pairs[2 * t], b = pairs[2 * t + 1];
     const int32_t* nbl = nb_idx + nb_off[a];
@@ -420,16 +463,8 @@ int clip_host_records_group(const double* sph4, const int64_t* nb_off, const int
       if (!kill && m == 0) {
         r = vltr::record_uncut(g, mode);
       } else if (!kill) {
-        task->sph4 = sph4;
-        task->nbl = nbl;
-        task->g = g;
-        task->a_id = a;
-        task->mlo = m;
-        task->mhi = 0;
-        task->mode = mode;
-        task->hs = hs;
-        task->hc = hc;
-        const bool agree = run_group(*task, &r, &st);
+        const bool agree = group_size == 8 ? group_record<8>(sph4, nbl, g, a, m, hs, hc, mode, &r, &st)
+                                           : group_record<16>(sph4, nbl, g, a, m, hs, hc, mode, &r, &st);
         extra = 16u | (agree ? 0u : 64u);
         if (st == vltr::kOverflow) {
           r = vltr::make_record(sph4, a, b, nbl, nbn, mode, hs, hc, gc, ga, &st);
@@ -445,7 +480,6 @@ int clip_host_records_group(const double* sph4, const int64_t* nb_off, const int
     out[6 * t + 5] = r.pyr_b;
     flags[t] = (r.flags & 3u) | extra;
   }
-  delete task;
   return 0;
 }
 

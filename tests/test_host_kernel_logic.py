@@ -28,10 +28,10 @@ def _staged(sph4, off, idx, pairs, mode=0):
     return out, fl
 
 
-def _group(sph4, off, idx, pairs, mode=0):
+def _group(sph4, off, idx, pairs, mode=0, group_size=16):
     L = hostclip._load()
     f = L.clip_host_records_group
-    f.argtypes = [C.c_void_p] * 4 + [C.c_int64, C.c_int, C.c_void_p, C.c_void_p]
+    f.argtypes = [C.c_void_p] * 4 + [C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
     sph4 = np.ascontiguousarray(sph4, np.float64)
     off = np.ascontiguousarray(off, np.int64)
     idx = np.ascontiguousarray(idx, np.int32)
@@ -39,7 +39,7 @@ def _group(sph4, off, idx, pairs, mode=0):
     out = np.zeros((len(pairs), 6))
     fl = np.zeros(len(pairs), np.uint32)
     f(sph4.ctypes.data, off.ctypes.data, idx.ctypes.data, pairs.ctypes.data, len(pairs), mode, out.ctypes.data,
-      fl.ctypes.data)
+      fl.ctypes.data, group_size)
     return out, fl
 
 
@@ -66,11 +66,12 @@ def test_records_bit_identical(oracle, name, make, probe, pbox, mode):
     assert np.array_equal(gf2 & 3, rf & 3)
     assert np.array_equal(got2, ref)
     if name != "2zsk":  # (82k pairs on fibers take ~40 s; 1ctf and RSA cover proteins)
-        got3, gf3 = _group(sph4, off, idx, pairs, mode=mode)
-        assert not (gf3 & 64).any(), "lanes of a group disagreed"
-        assert np.array_equal(gf3 & 3, rf & 3)
-        assert np.array_equal(got3, ref)
-        assert (gf3 & 16).any()
+        for gs in (16, 8):
+            got3, gf3 = _group(sph4, off, idx, pairs, mode=mode, group_size=gs)
+            assert not (gf3 & 64).any(), "lanes of a group disagreed"
+            assert np.array_equal(gf3 & 3, rf & 3)
+            assert np.array_equal(got3, ref)
+            assert (gf3 & 16).any()
 
 
 def test_records_bit_identical_rsa(oracle):
@@ -88,6 +89,11 @@ def test_records_bit_identical_rsa(oracle):
     assert np.array_equal(gf3 & 3, rf & 3) and np.array_equal(got3, ref)
     # nearly every cut pair stays within the group's 16 / 32 points
     assert ((gf3 & 32) > 0).sum() < 1e-3 * ((gf3 & 16) > 0).sum()
+    got4, gf4 = _group(hostclip.populated(balls, 1.4), off, idx, pairs, group_size=8)
+    assert not (gf4 & 64).any()
+    assert np.array_equal(gf4 & 3, rf & 3) and np.array_equal(got4, ref)
+    # with 8 lanes about 1% of the cut pairs outgrow the group
+    assert ((gf4 & 32) > 0).sum() < 0.03 * ((gf4 & 16) > 0).sum()
 
 
 def test_xcd_remap_is_bijective():

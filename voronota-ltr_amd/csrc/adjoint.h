@@ -163,7 +163,7 @@ struct AdjLds {
 
 template <class G>
 VLTR_HD double gsum(G& g, double v) {
-  for (int o = 1; o < kGroup; o <<= 1) v += g.shfl(v, g.lane() ^ o);
+  for (int o = 1; o < G::kSize; o <<= 1) v += g.shfl(v, g.lane() ^ o);
   return v;
 }
 template <class G>
@@ -179,6 +179,8 @@ template <class G, class Acc>
 VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t a_id, int32_t b_id,
                                    const ContactTopo& topo, double w, double g_a, double g_b, double sgn_a,
                                    double sgn_b, Acc& acc, double beta = 1.0) {
+  constexpr int GS = G::kSize;
+  static_assert(2 * GS <= kTopoCap, "two contour points per lane");
   const int j = g.lane();
   const double own = j == 0 ? 1.0 : 0.0;  // uniform terms are added by lane 0 only
   const Sph A = load_sph(sph4, a_id), B = load_sph(sph4, b_id);
@@ -214,7 +216,7 @@ VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t 
   } else {
     const RelPlane pb = rel_plane(A, B);
     for (int hh = 0; hh < 2; ++hh) {
-      const int t = j + hh * kGroup;
+      const int t = j + hh * GS;
       if (t >= m) continue;
       const int32_t l = topo.l[t], rr = topo.r[t];
       V3 y;
@@ -235,7 +237,7 @@ VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t 
     auto P = [&](int t) { return V3{s->x[t], s->y[t], s->z[t]}; };
     V3 bs{0.0, 0.0, 0.0};
     for (int hh = 0; hh < 2; ++hh) {
-      const int t = j + hh * kGroup;
+      const int t = j + hh * GS;
       if (t >= m) continue;
       const int nt = t + 1 < m ? t + 1 : 0;
       double th = 0.0, sgn = 1.0;
@@ -248,7 +250,7 @@ VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t 
     g.sync();
     double al = 0.0;
     for (int hh = 0; hh < 2; ++hh) {
-      const int t = j + hh * kGroup;
+      const int t = j + hh * GS;
       if (t >= m) continue;
       const int nt = t + 1 < m ? t + 1 : 0;
       al += triangle_area(bary, P(t), P(nt));
@@ -264,7 +266,7 @@ VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t 
     V3 self[2] = {V3{0.0, 0.0, 0.0}, V3{0.0, 0.0, 0.0}};
     V3 baryb{0.0, 0.0, 0.0};
     for (int hh = 0; hh < 2; ++hh) {
-      const int t = j + hh * kGroup;
+      const int t = j + hh * GS;
       if (t >= m) continue;
       const int pt = t > 0 ? t - 1 : m - 1;
       const int nt = t + 1 < m ? t + 1 : 0;
@@ -341,7 +343,7 @@ VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t 
     g.sync();
     // points -> spheres (implicit differentiation of the defining equations)
     for (int hh = 0; hh < 2; ++hh) {
-      const int t = j + hh * kGroup;
+      const int t = j + hh * GS;
       if (t >= m) continue;
       const int pt = t > 0 ? t - 1 : m - 1;
       const int nt = t + 1 < m ? t + 1 : 0;

@@ -12,8 +12,10 @@
 // outgrow the group, and the degenerate cuts that leave fewer than three
 // points, return kOverflow: the caller redoes them with the one-lane code.
 //
-// G is the lane group: lane(), ballot(bool) -> 16-bit mask, shfl(v, src)
-// for double and int32, sync() (shared-buffer writes visible to the group).
+// G is the lane group of G::kSize lanes (8 or 16): lane(), ballot(bool) ->
+// kSize-bit mask, shfl(v, src) for double and int32, sync() (shared-buffer
+// writes visible to the group). Contours hold at most kSize points while cut
+// and 2 kSize after the restriction.
 // All control flow that reaches a ballot, shfl or sync is uniform over the
 // group. The kernels implement G with wavefront intrinsics; the CPU tests
 // run the same code on 16 lock-stepped host fibers.
@@ -26,11 +28,15 @@ namespace vltr {
 constexpr int kGroup = 16;
 constexpr int kGroupR = 2 * kGroup;
 
-// Restricted contour of one group: points, (left, right) ids, arc angles.
-struct GroupLds {
-  double x[kGroupR], y[kGroupR], z[kGroupR], ang[kGroupR];
-  int32_t l[kGroupR], r[kGroupR];
+// Restricted contour of one group (CAP = twice the group size): points,
+// (left, right) ids, arc angles.
+template <int CAP>
+struct GroupLdsT {
+  static constexpr int kCap = CAP;
+  double x[CAP], y[CAP], z[CAP], ang[CAP];
+  int32_t l[CAP], r[CAP];
 };
+using GroupLds = GroupLdsT<kGroupR>;
 
 struct GPt {
   V3 p;
@@ -95,6 +101,7 @@ VLTR_HD GPt gshfl(G& g, const GPt& v, int src) {
 // one-lane form: mark_and_cut).
 template <class G>
 VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
+  constexpr int GS = G::kSize;
   const int j = g.lane();
   const bool o = j < n && halfspace(cp.pc, cp.pn, c.p) >= 0;
   if (o) {
@@ -119,7 +126,7 @@ VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
   }
   int src = j, nn = n;
   if (start == end) {
-    if (n + 1 > kGroup) return kOverflow;
+    if (n + 1 > GS) return kOverflow;
     src = j > start ? j - 1 : j;
     nn = n + 1;
     end = start + 1;
@@ -135,7 +142,7 @@ VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
     end = 0;
   }
   if (nn < 3) return kOverflow;
-  c = gshfl(g, c, src & (kGroup - 1));
+  c = gshfl(g, c, src & (GS - 1));
   n = nn;
   const int il = start > 0 ? start - 1 : n - 1;
   const int ir = end + 1 < n ? end + 1 : 0;
@@ -156,28 +163,32 @@ VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
   if (g.ballot(j == start && !f_gt(norm2(pe - c.p), 0.0)) != 0u) {
     const int32_t re = g.shfl(c.r, end);
     if (j == start) c.r = re;
-    c = gshfl(g, c, (j >= end ? j + 1 : j) & (kGroup - 1));
+    c = gshfl(g, c, (j >= end ? j + 1 : j) & (GS - 1));
     --n;
   }
   return kContact;
 }
 
-template <class G>
-VLTR_HD void lds_put(GroupLds* s, int k, V3 p, int32_t l, int32_t r) {
+template <class S>
+VLTR_HD void lds_put(S* s, int k, V3 p, int32_t l, int32_t r) {
   s->x[k] = p.x;
   s->y[k] = p.y;
   s->z[k] = p.z;
   s->l[k] = l;
   s->r[k] = r;
 }
-VLTR_HD V3 lds_p(const GroupLds* s, int k) { return V3{s->x[k], s->y[k], s->z[k]}; }
+template <class S>
+VLTR_HD V3 lds_p(const S* s, int k) {
+  return V3{s->x[k], s->y[k], s->z[k]};
+}
 
 // Restriction to the circle (contact.rs:631-769; one-lane form:
 // restrict_to_circle). Lane j handles segment (j, j+1). On return the
 // restricted contour is s[0..m) with its arc angles; m = 0 means no contour.
-template <class G>
-VLTR_HD int group_restrict(G& g, GroupLds* s, const GPt& c, int n, const Sph& ic, V3 axis, int32_t a_id, int* m_out,
+template <class G, class S>
+VLTR_HD int group_restrict(G& g, S* s, const GPt& c, int n, const Sph& ic, V3 axis, int32_t a_id, int* m_out,
                            double* sum) {
+  constexpr int GS = G::kSize;
   const int j = g.lane();
   const double rr = ic.r * ic.r;
   const bool valid = j < n;
@@ -188,7 +199,7 @@ VLTR_HD int group_restrict(G& g, GroupLds* s, const GPt& c, int n, const Sph& ic
   *m_out = 0;
   if (n_in == n) {
     if (valid) {
-      lds_put<G>(s, j, c.p, c.l, c.r);
+      lds_put(s, j, c.p, c.l, c.r);
       s->ang[j] = 0.0;
     }
     g.sync();
@@ -197,7 +208,7 @@ VLTR_HD int group_restrict(G& g, GroupLds* s, const GPt& c, int n, const Sph& ic
   }
   const int ni = j + 1 < n ? j + 1 : 0;
   const int pj = j > 0 ? j - 1 : n - 1;
-  const GPt nx = gshfl(g, c, ni & (kGroup - 1));
+  const GPt nx = gshfl(g, c, ni & (GS - 1));
   const bool o1 = !in;
   const bool o2 = !((Min >> (ni & 31)) & 1u);
   V3 ins1 = c.p, ins2 = c.p;
@@ -257,7 +268,7 @@ VLTR_HD int group_restrict(G& g, GroupLds* s, const GPt& c, int n, const Sph& ic
   // Final ids of an inside point: segment j-1 runs before segment j, except
   // that segment n-1 runs after segment 0 and so wins for point 0.
   const bool prev_mod = (g.ballot(mod_next) >> (pj & 31)) & 1u;
-  const int32_t prev_r = g.shfl(c.r, pj & (kGroup - 1));
+  const int32_t prev_r = g.shfl(c.r, pj & (GS - 1));
   int32_t fl = c.l, fr = c.r;
   if (j == 0) {
     if (mod_self) {
@@ -284,19 +295,19 @@ VLTR_HD int group_restrict(G& g, GroupLds* s, const GPt& c, int n, const Sph& ic
   const uint32_t below = (1u << j) - 1u;
   const int off = popc32(B0 & below) + 2 * popc32(B1 & below);
   const int m = popc32(B0) + 2 * popc32(B1);
-  if (m > kGroupR) return kOverflow;
+  if (m > S::kCap) return kOverflow;
   if (m - n_in == 0 || m < 2) return kContact;
   if (valid) {
     int o = off;
-    if (in) lds_put<G>(s, o++, c.p, fl, fr);
-    if (nins >= 1) lds_put<G>(s, o++, ins1, i1l, i1r);
-    if (nins == 2) lds_put<G>(s, o, ins2, i2l, i2r);
+    if (in) lds_put(s, o++, c.p, fl, fr);
+    if (nins >= 1) lds_put(s, o++, ins1, i1l, i1r);
+    if (nins == 2) lds_put(s, o, ins2, i2l, i2r);
   }
   g.sync();
   const V3 ctrl = ic.c + axis;
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
-    const int t = j + h * kGroup;
+    const int t = j + h * GS;
     if (t < m) {
       const int nt = t + 1 < m ? t + 1 : 0;
       double ang = 0.0;
@@ -321,11 +332,12 @@ VLTR_HD int group_restrict(G& g, GroupLds* s, const GPt& c, int n, const Sph& ic
 // slot t / 16), in entry order: acc = (acc + t1[k]) + t2[k].
 template <class G>
 VLTR_HD double group_sum2(G& g, double t1a, double t1b, double t2a, double t2b, int m) {
+  constexpr int GS = G::kSize;
   double acc = 0.0;
   for (int k = 0; k < m; ++k) {
-    const int src = k & (kGroup - 1);
-    const double v1 = k < kGroup ? g.shfl(t1a, src) : g.shfl(t1b, src);
-    const double v2 = k < kGroup ? g.shfl(t2a, src) : g.shfl(t2b, src);
+    const int src = k & (GS - 1);
+    const double v1 = k < GS ? g.shfl(t1a, src) : g.shfl(t1b, src);
+    const double v2 = k < GS ? g.shfl(t2a, src) : g.shfl(t2b, src);
     acc = acc + v1;
     acc = acc + v2;
   }
@@ -335,8 +347,8 @@ VLTR_HD double group_sum2(G& g, double t1a, double t1b, double t2a, double t2b, 
 // Per-vertex turn of the Gauss-Bonnet sum (contact.rs:797-837): first and
 // second addend (second is +0.0 when absent). The three cases differ only in
 // the dihedral's two outer points, so min_dihedral is evaluated once.
-VLTR_HD void turn_terms(const Sph& a, const Sph& b, const Sph& ic, const GroupLds* s, int i, int m, double* t1,
-                        double* t2) {
+template <class S>
+VLTR_HD void turn_terms(const Sph& a, const Sph& b, const Sph& ic, const S* s, int i, int m, double* t1, double* t2) {
   const int pi = i > 0 ? i - 1 : m - 1;
   const int ni = i + 1 < m ? i + 1 : 0;
   const V3 p0 = lds_p(s, pi), p1 = lds_p(s, i), p2 = lds_p(s, ni);
@@ -359,13 +371,14 @@ VLTR_HD void turn_terms(const Sph& a, const Sph& b, const Sph& ic, const GroupLd
   *t1 = kPi - min_dihedral(a.c, p1, b1, b2);
 }
 
-template <class G>
-VLTR_HD double group_solid_angle(G& g, const Sph& a, const Sph& b, const Sph& ic, const GroupLds* s, int m) {
+template <class G, class S>
+VLTR_HD double group_solid_angle(G& g, const Sph& a, const Sph& b, const Sph& ic, const S* s, int m) {
+  constexpr int GS = G::kSize;
   const int j = g.lane();
   double t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0};
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
-    const int t = j + h * kGroup;
+    const int t = j + h * GS;
     if (t < m) turn_terms(a, b, ic, s, t, m, &t1[h], &t2[h]);
   }
   const double turn = group_sum2(g, t1[0], t1[1], t2[0], t2[1], m);
@@ -378,8 +391,9 @@ VLTR_HD double group_solid_angle(G& g, const Sph& a, const Sph& b, const Sph& ic
 
 // Area and centrality terms of restricted-contour entry t (contact.rs:773-793,
 // 411-444).
-VLTR_HD void area_terms(const GroupLds* s, int t, int m, const Sph& ic, V3 bary, double* tri, double* seg,
-                        bool* reflex, bool* fail) {
+template <class S>
+VLTR_HD void area_terms(const S* s, int t, int m, const Sph& ic, V3 bary, double* tri, double* seg, bool* reflex,
+                        bool* fail) {
   const int nt = t + 1 < m ? t + 1 : 0;
   const V3 p = lds_p(s, t), q = lds_p(s, nt);
   *tri = triangle_area(bary, p, q);
@@ -401,12 +415,13 @@ VLTR_HD void area_terms(const GroupLds* s, int t, int m, const Sph& ic, V3 bary,
 VLTR_HD bool circle_root(V3 x, V3 a, V3 b, V3 c) { return dot(x - a, cross(b - a, c - a)) > 0.0; }
 
 // Writes the topology of the restricted contour s[0..m).
-template <class G>
-VLTR_HD void write_topology(G& g, const GroupLds* s, const PairGeom& pg, int32_t a_id, int m, const TopoOut& tout) {
+template <class G, class S>
+VLTR_HD void write_topology(G& g, const S* s, const PairGeom& pg, int32_t a_id, int m, const TopoOut& tout) {
+  constexpr int GS = G::kSize;
   const int j = g.lane();
   bool root[2] = {false, false};
   for (int h = 0; h < 2; ++h) {
-    const int t = j + h * kGroup;
+    const int t = j + h * GS;
     if (t >= m) continue;
     const int32_t l = s->l[t], r = s->r[t];
     tout.topo->l[t] = l;
@@ -419,7 +434,7 @@ VLTR_HD void write_topology(G& g, const GroupLds* s, const PairGeom& pg, int32_t
   const uint32_t r0 = g.ballot(root[0]), r1 = g.ballot(root[1]);
   if (j == 0) {
     tout.topo->m = m;
-    tout.topo->roots = r0 | (r1 << kGroup);
+    tout.topo->roots = r0 | (r1 << GS);
   }
 }
 
@@ -427,10 +442,11 @@ VLTR_HD void write_topology(G& g, const GroupLds* s, const PairGeom& pg, int32_t
 // cutting set (bit k of (mlo, mhi) = slot k of a's sorted neighbor list).
 // f.plane(k) returns slot k's CutPlane and f.thr(k) the squared distance from
 // a's centre to its plane point. Every lane returns the same record.
-template <class G, class Fetch>
-VLTR_HD PairRec group_clip(G& g, GroupLds* s, const PairGeom& pg, int32_t a_id, uint64_t mlo, uint64_t mhi,
-                           Fetch&& f, const double* hex_s, const double* hex_c, int mode, int* status,
+template <class G, class S, class Fetch>
+VLTR_HD PairRec group_clip(G& g, S* s, const PairGeom& pg, int32_t a_id, uint64_t mlo, uint64_t mhi, Fetch&& f,
+                           const double* hex_s, const double* hex_c, int mode, int* status,
                            const TopoOut* tout = nullptr) {
+  constexpr int GS = G::kSize;
   const int j = g.lane();
   GPt c{pg.ic.c, a_id, a_id};
   int n = 0;
@@ -484,7 +500,7 @@ VLTR_HD PairRec group_clip(G& g, GroupLds* s, const PairGeom& pg, int32_t a_id, 
   bool reflex = false, fail = false;
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
-    const int t = j + h * kGroup;
+    const int t = j + h * GS;
     if (t < m) {
       bool rf, fl;
       area_terms(s, t, m, pg.ic, bary, &tri[h], &seg[h], &rf, &fl);
@@ -499,7 +515,7 @@ VLTR_HD PairRec group_clip(G& g, GroupLds* s, const PairGeom& pg, int32_t a_id, 
   if (tout != nullptr && tout->topo != nullptr) write_topology(g, s, pg, a_id, m, *tout);
   if (tout != nullptr && tout->net != nullptr) {
     for (int h = 0; h < 2; ++h) {
-      const int t = j + h * kGroup;
+      const int t = j + h * GS;
       if (t >= m) continue;
       ContactNet* nt = tout->net;
       nt->l[t] = s->l[t];

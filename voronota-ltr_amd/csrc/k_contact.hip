@@ -13,13 +13,14 @@
 //     means the uncut full circle, finished one pair per lane afterwards;
 //     otherwise the pair and its 64-bit cutting mask go to the wavefront's
 //     list.
-//  2. Clipping. The wavefront splits into four groups of 16 lanes; each group
+//  2. Clipping. The wavefront splits into eight groups of 8 lanes; each group
 //     clips one listed pair at a time with group_clip (contour points in
-//     lanes, planes read from the shared table), four pairs side by side.
-// Balls with more than 64 neighbors, and pairs whose contour outgrows a
-// group, go to k_cut_slow: one lane per pair, the one-lane code of contour.h
-// with the contour in a global workspace. All paths produce identical
-// records.
+//     lanes, planes read from the shared table), eight pairs side by side.
+// Pairs of balls with more than 64 neighbors, and pairs whose contour
+// outgrows an 8-lane group, go to k_clip_wide (16 lanes per pair); what
+// outgrows that goes to k_cut_slow: one lane per pair, the one-lane code of
+// contour.h with the contour in a global workspace. All paths produce
+// identical records.
 #include "contour.h"
 #include "group_clip.h"
 #include "vltr_internal.h"
@@ -39,6 +40,14 @@ double h_hex_s[5], h_hex_c[5];
 constexpr int kWave = 64;
 constexpr int kClipSlots = 64;  // neighbor slots handled by k_clip
 constexpr int kClipWaves = 4;   // wavefronts per k_clip block
+// Lanes per pair in k_clip (8: eight pairs side by side; contours beyond 8
+// points, about 1% of the clipped pairs, go to k_clip_wide's 16-lane groups).
+#ifndef VLTR_CLIP_GROUP
+#define VLTR_CLIP_GROUP 8
+#endif
+constexpr int kClipGroup = VLTR_CLIP_GROUP;
+using ClipGroup = WaveGroupT<kClipGroup>;
+using ClipGroupLds = GroupLdsT<2 * kClipGroup>;
 // Occupancy target of k_clip (wavefronts per SIMD). Its shared memory allows
 // three 4-wavefront blocks per CU, so 3 caps registers at 168 per lane.
 #ifndef VLTR_CLIP_WAVES_PER_EU
@@ -56,7 +65,7 @@ struct ClipLds {
   // from the front (with their cutting masks), uncut pairs from the back
   int32_t item[kClipSlots];
   uint64_t mask[kClipSlots];
-  GroupLds grp[kWave / kGroup];
+  ClipGroupLds grp[kWave / kClipGroup];
 };
 
 // Value of v in lane k (k uniform over the wavefront).
@@ -192,10 +201,10 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     wave_sync();
   }
 
-  // ---- phase 2: four pairs at a time, 16 lanes each
-  const int grp = lane / kGroup;
-  WaveGroup G{grp * kGroup, lane & (kGroup - 1)};
-  GroupLds* gs = &w.grp[grp];
+  // ---- phase 2: kWave / kClipGroup pairs at a time, kClipGroup lanes each
+  const int grp = lane / kClipGroup;
+  ClipGroup G{grp * kClipGroup, lane & (kClipGroup - 1)};
+  ClipGroupLds* gs = &w.grp[grp];
   struct {
     const ClipLds& w;
     __device__ double thr(int s) const { return w.thr[s]; }
@@ -204,7 +213,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
                       V3{w.nnx[s], w.nny[s], w.nnz[s]}, w.cid[s]};
     }
   } fetch{w};
-  for (int r0 = 0; r0 < n_clip; r0 += kWave / kGroup) {
+  for (int r0 = 0; r0 < n_clip; r0 += kWave / kClipGroup) {
     const int idx = r0 + grp;
     if (idx < n_clip) {
       const int it = w.item[idx];
@@ -219,9 +228,9 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
                                      want ? &tout : nullptr);
       if (G.lane() == 0) {
         if (st == kOverflow) {
-          push_slow(p, base + (it >> 8));
+          push_wide(p, pidx);
         } else {
-          p.rec[base + (it >> 8)] = rec;
+          p.rec[pidx] = rec;
         }
       }
     }

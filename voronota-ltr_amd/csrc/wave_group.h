@@ -1,4 +1,4 @@
-// 16-lane groups of a 64-lane wavefront: the lane-group interface of
+// 8- or 16-lane groups of a 64-lane wavefront: the lane-group interface of
 // group_clip.h and adjoint.h (lane, ballot, shuffle, shared-memory sync)
 // implemented with wavefront intrinsics.
 #pragma once
@@ -15,13 +15,18 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-struct WaveGroup {
+template <int GS>
+struct WaveGroupT {
+  static constexpr int kSize = GS;
   int base, ln;
   __device__ int lane() const { return ln; }
-  __device__ uint32_t ballot(bool p) const { return static_cast<uint32_t>((__ballot(p) >> base) & 0xFFFFull); }
-  __device__ double shfl(double v, int src) const { return __shfl(v, src, kGroup); }
-  __device__ int32_t shfl(int32_t v, int src) const { return __shfl(v, src, kGroup); }
+  __device__ uint32_t ballot(bool p) const {
+    return static_cast<uint32_t>((__ballot(p) >> base) & ((1ull << GS) - 1ull));
+  }
+  __device__ double shfl(double v, int src) const { return __shfl(v, src, GS); }
+  __device__ int32_t shfl(int32_t v, int src) const { return __shfl(v, src, GS); }
   __device__ void sync() const { wave_sync(); }
 };
+using WaveGroup = WaveGroupT<kGroup>;
 
 }  // namespace vltr

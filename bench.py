@@ -326,7 +326,7 @@ def dynamics_leg(args):
            "ms_per_step": round(t_gpu * 1e3, 3), "steps_per_s": round(1.0 / t_gpu, 1),
            "mean_affected": round(affected / len(moves), 1), "full_reinits": reinit,
            "last_step_phases_ms": {k: round(v, 4) for k, v in ph.items()},
-           "note": "host API (balls uploaded each step, summary not downloaded)"}
+           "note": "host API (the moved balls are sent each step; summary not downloaded)"}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle

@@ -247,6 +247,11 @@ __global__ void k_mark_more(Pipeline p, const int32_t* changed, int64_t k, uint8
   affected[p.nb[static_cast<int64_t>(id) * p.nb_cap + j] % p.n] = 1;
 }
 
+__global__ void k_scatter_balls(vltr_ball* balls, const int32_t* ids, const vltr_ball* vals, int64_t count) {
+  const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (k < count) balls[ids[k]] = vals[k];
+}
+
 __global__ void k_mask_clear(uint8_t* mask, const uint8_t* where, int64_t n) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i < n && where[i]) mask[i] = 0;
@@ -291,6 +296,12 @@ hipError_t launch_mark_more(const Pipeline& p, const int32_t* changed, int64_t k
     hipLaunchKernelGGL(k_mark_more, dim3(blocks_for(k * p.nb_cap, 256)), dim3(256), 0, s, p, changed, k, affected);
   return hipGetLastError();
 }
+hipError_t launch_scatter_balls(vltr_ball* balls, const int32_t* ids, const vltr_ball* vals, int64_t count,
+                                hipStream_t s) {
+  if (count > 0) hipLaunchKernelGGL(k_scatter_balls, dim3(blocks_for(count, 256)), dim3(256), 0, s, balls, ids, vals, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_mask_clear(uint8_t* mask, const uint8_t* where, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_mask_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, mask, where, n);
   return hipGetLastError();

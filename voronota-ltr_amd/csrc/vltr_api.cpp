@@ -982,7 +982,8 @@ struct vltr_updateable {
   State st, bk;
   bool has_backup = false;
   // cells and exclusion statuses of the current state and of the backup
-  DevBuf cells_state, cells_backup, excl, excl_backup, affected_mask, ids, rows, rows_cnt;
+  DevBuf cells_state, cells_backup, excl, excl_backup, affected_mask, ids, rows, rows_cnt, scatter;
+  std::vector<uint8_t> mark;  // host scratch of the affected-set construction
 };
 
 namespace {
@@ -1027,7 +1028,8 @@ int upd_rows(Upd* u, const std::vector<uint64_t>& ids, std::vector<std::vector<u
 // otherwise only balls in aff take new cells. refresh: affected balls get
 // their hidden status back as exclusion status (collision update). more:
 // balls whose new neighbors join the affected set.
-int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, const std::vector<uint64_t>& more) {
+int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, const std::vector<uint64_t>& more,
+            const std::vector<uint64_t>* moved = nullptr) {
   vltr_context* ctx = u->ctx;
   auto& st = u->st;
   const size_t n = st.balls.size();
@@ -1039,8 +1041,23 @@ int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, c
   }
   hipStream_t s = ctx->stream;
   VLTR_HIP(hipSetDevice(ctx->device));
-  VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
-  VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, st.balls.data(), n * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
+  if (moved != nullptr && ctx->last_n == n) {
+    // the device copy holds the previous balls: write only the moved ones
+    std::vector<int32_t> ids(moved->begin(), moved->end());
+    std::vector<vltr_ball> vals;
+    vals.reserve(ids.size());
+    for (uint64_t id : *moved) vals.push_back(st.balls[id]);
+    const size_t vals_off = (ids.size() * sizeof(int32_t) + 255) / 256 * 256;
+    VLTR_HIP(u->scatter.ensure(vals_off + vals.size() * sizeof(vltr_ball)));
+    int32_t* d_ids = u->scatter.as<int32_t>();
+    vltr_ball* d_vals = reinterpret_cast<vltr_ball*>(u->scatter.as<unsigned char>() + vals_off);
+    VLTR_HIP(hipMemcpyAsync(d_vals, vals.data(), vals.size() * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
+    VLTR_HIP(hipMemcpyAsync(d_ids, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    VLTR_HIP(launch_scatter_balls(ctx->balls.as<vltr_ball>(), d_ids, d_vals, static_cast<int64_t>(ids.size()), s));
+  } else {
+    VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
+    VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, st.balls.data(), n * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
+  }
   VLTR_HIP(u->cells_state.ensure(n * sizeof(CellRec)));
   VLTR_HIP(u->excl.ensure(n));
   VLTR_HIP(u->affected_mask.ensure(n));
@@ -1139,39 +1156,45 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
     if (rc == VLTR_OK) *ok = 1;
     return rc;
   }
-  std::vector<uint64_t> affected = changed;
-  std::sort(affected.begin(), affected.end());
+  // affected = changed (sorted, repeats kept) plus the changed balls' old
+  // neighbors; the reference inserts them one by one into a sorted vector and
+  // gives up (full reinit) when one more would pass the threshold, i.e. when
+  // |changed| + |new ids| > threshold (spheres_container.rs:105-124)
+  std::vector<uint8_t>& mark = u->mark;
+  mark.assign(n, 0);
+  for (uint64_t id : changed) mark[id] = 1;
   std::vector<std::vector<uint64_t>> old_rows;
   rc = upd_rows(u, changed, &old_rows);
   if (rc != VLTR_OK) return rc;
+  std::vector<uint64_t> affected = changed;
   for (const auto& row : old_rows)
-    for (uint64_t cid : row) {
-      auto it = std::lower_bound(affected.begin(), affected.end(), cid);
-      if (it == affected.end() || *it != cid) {
-        if (affected.size() < threshold) {
-          affected.insert(it, cid);
-        } else {
-          rc = upd_full(u, balls, nb);
-          if (rc == VLTR_OK) *ok = 1;
-          return rc;
-        }
+    for (uint64_t cid : row)
+      if (!mark[cid]) {
+        mark[cid] = 1;
+        affected.push_back(cid);
       }
-    }
+  if (affected.size() > threshold) {
+    rc = upd_full(u, balls, nb);
+    if (rc == VLTR_OK) *ok = 1;
+    return rc;
+  }
+  std::sort(affected.begin(), affected.end());
   for (uint64_t id : changed) st.balls[id] = balls[id];
-  rc = upd_run(u, false, true, affected, changed);
+  rc = upd_run(u, false, true, affected, changed, &changed);
   if (rc != VLTR_OK) return rc;
+  // spheres_container.rs:150-170: new neighbors of the changed balls join
   std::vector<std::vector<uint64_t>> new_rows;
   rc = upd_rows(u, changed, &new_rows);
   if (rc != VLTR_OK) return rc;
-  std::vector<uint64_t> more;
+  bool more = false;
   for (const auto& row : new_rows)
     for (uint64_t cid : row)
-      if (!std::binary_search(affected.begin(), affected.end(), cid)) {
-        auto it = std::lower_bound(more.begin(), more.end(), cid);
-        if (it == more.end() || *it != cid) more.insert(it, cid);
+      if (!mark[cid]) {
+        mark[cid] = 2;
+        affected.push_back(cid);
+        more = true;
       }
-  if (!more.empty()) {
-    affected.insert(affected.end(), more.begin(), more.end());
+  if (more) {
     std::sort(affected.begin(), affected.end());
     affected.erase(std::unique(affected.begin(), affected.end()), affected.end());
   }
@@ -1203,7 +1226,7 @@ void vltr_updateable_destroy(vltr_updateable* u) {
     (void)hipSetDevice(u->ctx->device);
     (void)hipStreamSynchronize(u->ctx->stream);
     for (DevBuf* b : {&u->cells_state, &u->cells_backup, &u->excl, &u->excl_backup, &u->affected_mask, &u->ids, &u->rows,
-                      &u->rows_cnt})
+                      &u->rows_cnt, &u->scatter})
       b->release();
     vltr_context_destroy(u->ctx);
   }

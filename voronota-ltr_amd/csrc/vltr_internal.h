@@ -154,6 +154,9 @@ hipError_t launch_emit_net(const Pipeline& p, const uint32_t* pos, ContactNet* o
 // recomputed ball when refresh != 0).
 hipError_t launch_updateable_cells(const Pipeline& p, CellRec* state, uint8_t* excl, const uint8_t* affected, int full,
                                    int refresh, hipStream_t s);
+// balls[ids[k]] = vals[k], k < count.
+hipError_t launch_scatter_balls(vltr_ball* balls, const int32_t* ids, const vltr_ball* vals, int64_t count,
+                                hipStream_t s);
 // affected[nb % n] = 1 for every neighbor nb of the balls changed[0..k).
 hipError_t launch_mark_more(const Pipeline& p, const int32_t* changed, int64_t k, uint8_t* affected, hipStream_t s);
 // mask[i] = 0 where where[i] != 0.

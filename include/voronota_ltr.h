@@ -77,6 +77,15 @@ typedef struct vltr_context vltr_context;
 typedef struct vltr_result vltr_result;
 typedef struct vltr_batch_result vltr_batch_result;
 typedef struct vltr_updateable vltr_updateable;
+typedef struct vltr_solvent_result vltr_solvent_result;
+
+/* Pseudo-solvent sphere (solvent_spheres.rs:15-32). */
+typedef struct vltr_solvent_sphere {
+  double x, y, z;
+  double radius;
+  double weight;
+  uint64_t parent_index;
+} vltr_solvent_sphere;
 
 /* ------------------------------------------------------------ context --- */
 
@@ -191,6 +200,19 @@ int vltr_energy_gradient(vltr_context* ctx, const vltr_ball* balls, size_t n, do
 int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
                                 const int32_t* d_types, const double* w16, double sas_weight, double volume_weight,
                                 double* d_grad, double* energy);
+
+/* ----------------------------------------------------- solvent spheres --- */
+
+/* compute_solvent_spheres (solvent_spheres.rs:131-168): subdivision_depth
+ * 0..4 (12 .. 2562 samples per exposed atom; larger values act as 4);
+ * periodic_box may be NULL. Invalid probe, volume probe or ball give
+ * VLTR_ERR_INVALID_ARGUMENT with the reference's message. */
+int vltr_compute_solvent_spheres(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe,
+                                 double volume_probe, int subdivision_depth, const vltr_periodic_box* periodic_box,
+                                 vltr_solvent_result** out);
+size_t vltr_solvent_count(const vltr_solvent_result* r);
+const vltr_solvent_sphere* vltr_solvent_spheres(const vltr_solvent_result* r);
+void vltr_solvent_free(vltr_solvent_result* r);
 
 /* --------------------------------------------------------- updateable --- */
 

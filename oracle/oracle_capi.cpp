@@ -354,4 +354,33 @@ int oracle_max_threads() {
 #endif
 }
 
+// solvent_spheres.rs:131-168. out6: per sphere (x, y, z, radius, weight,
+// parent); returns the count (call with out6 = NULL to size).
+int64_t oracle_solvent_spheres(const double* balls, size_t n, double probe, double volume_probe, int depth,
+                               const double* pbox9, int threads, double* out6, int64_t max_out) {
+  auto b = load_balls<double>(balls, n);
+  PeriodicBox<double> pb;
+  if (pbox9) pb = load_pbox<double>(pbox9);
+  const auto s = compute_solvent_spheres(b, probe, volume_probe, depth, pbox9 ? &pb : nullptr, threads);
+  for (size_t i = 0; out6 && i < s.size() && static_cast<int64_t>(i) < max_out; ++i) {
+    out6[6 * i + 0] = s[i].x;
+    out6[6 * i + 1] = s[i].y;
+    out6[6 * i + 2] = s[i].z;
+    out6[6 * i + 3] = s[i].radius;
+    out6[6 * i + 4] = s[i].weight;
+    out6[6 * i + 5] = static_cast<double>(s[i].parent);
+  }
+  return static_cast<int64_t>(s.size());
+}
+
+int64_t oracle_icosahedron(int depth, double* out3, int64_t max_out) {
+  const auto v = subdivided_icosahedron(depth);
+  for (size_t i = 0; out3 && i < v.size() && static_cast<int64_t>(i) < max_out; ++i) {
+    out3[3 * i] = v[i].x;
+    out3[3 * i + 1] = v[i].y;
+    out3[3 * i + 2] = v[i].z;
+  }
+  return static_cast<int64_t>(v.size());
+}
+
 }  // extern "C"

@@ -334,3 +334,30 @@ class Updateable:
             return _read(h)
         finally:
             self._L.oracle_result_free(h)
+
+
+def solvent_spheres(balls, probe, volume_probe=0.0, depth=2, pbox=None, threads=0):
+    """solvent_spheres.rs:131-168: (m, 6) array of x, y, z, radius, weight, parent."""
+    L = _load()
+    f = L.oracle_solvent_spheres
+    f.restype = C.c_int64
+    f.argtypes = [_vp, C.c_size_t, C.c_double, C.c_double, C.c_int, _vp, C.c_int, _vp, C.c_int64]
+    b = np.ascontiguousarray(np.asarray(balls, np.float64).reshape(-1, 4))
+    pb = None if pbox is None else np.ascontiguousarray(np.asarray(pbox, np.float64).reshape(9))
+    args = (b.ctypes.data if len(b) else None, len(b), float(probe), float(volume_probe), int(depth),
+            pb.ctypes.data if pb is not None else None, int(threads))
+    m = f(*args, None, 0)
+    out = np.zeros((max(m, 1), 6))
+    f(*args, out.ctypes.data, m)
+    return out[:m]
+
+
+def icosahedron(depth):
+    L = _load()
+    f = L.oracle_icosahedron
+    f.restype = C.c_int64
+    f.argtypes = [C.c_int, _vp, C.c_int64]
+    m = f(int(depth), None, 0)
+    out = np.zeros((m, 3))
+    f(int(depth), out.ctypes.data, m)
+    return out

@@ -28,6 +28,8 @@
 #include <limits>
 #include <optional>
 #include <utility>
+#include <map>
+#include <array>
 #include <vector>
 
 #include "scalars.hpp"
@@ -1857,5 +1859,104 @@ class UpdateableTessellation {
   int threads_;
   R probe_ = R(0.0);
 };
+
+
+// ---------------------------------------------- subdivided_icosahedron.rs ----
+// subdivided_icosahedron.rs:51-131: unit vertices in order of first appearance
+// (the 12 base vertices, then each new edge midpoint as the faces are walked).
+inline std::vector<V3<double>> subdivided_icosahedron(int depth) {
+  const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+  const double base[12][3] = {{t, 1.0, 0.0},  {-t, 1.0, 0.0},  {t, -1.0, 0.0}, {-t, -1.0, 0.0},
+                              {1.0, 0.0, t},  {1.0, 0.0, -t},  {-1.0, 0.0, t}, {-1.0, 0.0, -t},
+                              {0.0, t, 1.0},  {0.0, -t, 1.0},  {0.0, t, -1.0}, {0.0, -t, -1.0}};
+  std::vector<V3<double>> v;
+  for (const auto& b : base) v.push_back(normalize(V3<double>{b[0], b[1], b[2]}));
+  std::vector<std::array<size_t, 3>> tri = {{0, 8, 4},  {1, 10, 7}, {2, 9, 11}, {7, 3, 1},  {0, 5, 10},
+                                            {3, 9, 6},  {3, 11, 9}, {8, 6, 4},  {2, 4, 9},  {3, 7, 11},
+                                            {4, 2, 0},  {9, 4, 6},  {2, 11, 5}, {0, 10, 8}, {5, 0, 2},
+                                            {10, 5, 7}, {1, 6, 8},  {1, 8, 10}, {6, 1, 3},  {11, 7, 5}};
+  for (int d = 0; d < depth; ++d) {
+    std::vector<std::array<size_t, 3>> next;
+    std::map<std::pair<size_t, size_t>, size_t> mids;
+    for (const auto& f : tri) {
+      const std::pair<size_t, size_t> edges[3] = {{f[1], f[2]}, {f[0], f[2]}, {f[0], f[1]}};
+      size_t mid[3];
+      for (int e = 0; e < 3; ++e) {
+        const size_t a = edges[e].first, b = edges[e].second;
+        const auto key = a < b ? std::make_pair(a, b) : std::make_pair(b, a);
+        auto it = mids.find(key);
+        if (it == mids.end()) {
+          v.push_back(normalize((v[a] + v[b]) * 0.5));
+          it = mids.emplace(key, v.size() - 1).first;
+        }
+        mid[e] = it->second;
+      }
+      next.push_back({f[0], mid[1], mid[2]});
+      next.push_back({f[1], mid[0], mid[2]});
+      next.push_back({f[2], mid[0], mid[1]});
+      next.push_back({mid[0], mid[1], mid[2]});
+    }
+    tri.swap(next);
+  }
+  return v;
+}
+
+// ------------------------------------------------------ solvent_spheres.rs ----
+struct SolventSphere {
+  double x, y, z, radius, weight;
+  size_t parent;
+};
+
+// solvent_spheres.rs:131-425 (validation is the caller's). Stage 1: exposed
+// atoms (cells with sas_area > 0) are sampled on the icosahedron at radius
+// r + probe; a sample is kept when its distance to the atom's surface is
+// smaller than to the surface of every contact partner (canonical balls).
+// Stage 2: tessellation of the atoms (radius max(r - probe + volume_probe,
+// 0.01)) and samples (radius probe + volume_probe) with probe 0; a sample's
+// weight is its cell volume (0 without a cell).
+inline std::vector<SolventSphere> compute_solvent_spheres(const std::vector<Ball<double>>& balls, double probe,
+                                                          double volume_probe, int depth,
+                                                          const PeriodicBox<double>* pbox, int threads) {
+  std::vector<SolventSphere> out;
+  if (balls.empty()) return out;
+  const auto s1 = compute_tessellation<double>(balls, probe, pbox, nullptr, 0, false, threads);
+  const size_t n = balls.size();
+  std::vector<char> exposed(n, 0);
+  for (const auto& c : s1.cells)
+    if (c.sas_area > 0.0) exposed[c.index] = 1;
+  std::vector<std::vector<size_t>> nbrs(n);
+  for (const auto& c : s1.contacts) {
+    if (exposed[c.id_a]) nbrs[c.id_a].push_back(c.id_b);
+    if (exposed[c.id_b]) nbrs[c.id_b].push_back(c.id_a);
+  }
+  const auto sih = subdivided_icosahedron(depth);
+  for (const auto& cell : s1.cells) {
+    if (!(cell.sas_area > 0.0)) continue;
+    const Ball<double>& at = balls[cell.index];
+    const V3<double> center{at.x, at.y, at.z};
+    const double radius = at.r + probe;
+    for (const auto& v : sih) {
+      const V3<double> p = center + v * radius;
+      const double d_atom = norm(p - center) - at.r;
+      bool ok = true;
+      for (size_t nb : nbrs[cell.index]) {
+        const Ball<double>& o = balls[nb];
+        if (norm(p - V3<double>{o.x, o.y, o.z}) - o.r <= d_atom) {
+          ok = false;
+          break;
+        }
+      }
+      if (ok) out.push_back(SolventSphere{p.x, p.y, p.z, probe, 0.0, cell.index});
+    }
+  }
+  if (out.empty()) return out;
+  std::vector<Ball<double>> comb;
+  for (const auto& b : balls) comb.push_back(Ball<double>{b.x, b.y, b.z, std::max(b.r - probe + volume_probe, 0.01)});
+  for (const auto& sp : out) comb.push_back(Ball<double>{sp.x, sp.y, sp.z, probe + volume_probe});
+  const auto s2 = compute_tessellation<double>(comb, 0.0, pbox, nullptr, 0, false, threads);
+  for (const auto& c : s2.cells)
+    if (c.index >= n) out[c.index - n].weight = c.volume;
+  return out;
+}
 
 }  // namespace vltr_oracle

@@ -78,6 +78,14 @@ def test_2zsk(vltr, oracle):
     assert len(gpu.contacts) == 23855 and len(gpu.cells) == 3545
 
 
+@pytest.mark.parametrize("probe", [6.0, 7.0])
+def test_long_neighbor_lists(vltr, oracle, probe):
+    """2zsk with large probes: up to 1122 neighbors per ball, beyond one
+    wavefront's candidate list (block-per-ball neighbor lists) and beyond the
+    group clip kernels' neighbor slots."""
+    _run(vltr, oracle, load_xyzr("balls_2zsk.xyzr"), probe, name=f"2zsk probe {probe}")
+
+
 def test_3dlb(vltr, oracle):
     b = load_xyzr("balls_3dlb.xyzr")
     _run(vltr, oracle, b, 1.4, name="3dlb")

@@ -439,13 +439,14 @@ VLTR_HD void write_topology(G& g, const S* s, const PairGeom& pg, int32_t a_id, 
 }
 
 // The whole clip of a pair known to have no killing neighbor and a non-empty
-// cutting set (bit k of (mlo, mhi) = slot k of a's sorted neighbor list).
+// cutting set: bit k of the cutting mask = slot k of a's sorted neighbor list,
+// the mask given as nw 64-bit words (words(w) = bits 64 w .. 64 w + 63).
 // f.plane(k) returns slot k's CutPlane and f.thr(k) the squared distance from
 // a's centre to its plane point. Every lane returns the same record.
-template <class G, class S, class Fetch>
-VLTR_HD PairRec group_clip(G& g, S* s, const PairGeom& pg, int32_t a_id, uint64_t mlo, uint64_t mhi, Fetch&& f,
-                           const double* hex_s, const double* hex_c, int mode, int* status,
-                           const TopoOut* tout = nullptr) {
+template <class G, class S, class Words, class Fetch>
+VLTR_HD PairRec group_clip_words(G& g, S* s, const PairGeom& pg, int32_t a_id, int nw, Words&& words, Fetch&& f,
+                                 const double* hex_s, const double* hex_c, int mode, int* status,
+                                 const TopoOut* tout = nullptr) {
   constexpr int GS = G::kSize;
   const int j = g.lane();
   GPt c{pg.ic.c, a_id, a_id};
@@ -453,10 +454,10 @@ VLTR_HD PairRec group_clip(G& g, S* s, const PairGeom& pg, int32_t a_id, uint64_
   bool init = false;
   double d2 = 0.0;  // squared distance of this lane's point from a's centre
   *status = kNone;
-  for (int half = 0; half < 2; ++half) {
-    uint64_t mm = half == 0 ? mlo : mhi;
+  for (int w = 0; w < nw; ++w) {
+    uint64_t mm = words(w);
     while (mm) {
-      const int k = half * 64 + ctz64(mm);
+      const int k = w * 64 + ctz64(mm);
       mm &= mm - 1;
       if (init) {
         // contact.rs:472-481: some point at least as far from a's centre as
@@ -546,6 +547,20 @@ VLTR_HD PairRec group_clip(G& g, S* s, const PairGeom& pg, int32_t a_id, uint64_
     rec.pyr_b = norm(pg.ic.c - pg.b.c) * area / 3.0 * sb;
   }
   return rec;
+}
+
+// Cutting mask of at most 128 slots as two words.
+struct MaskPair {
+  uint64_t lo, hi;
+  VLTR_HD uint64_t operator()(int w) const { return w == 0 ? lo : hi; }
+};
+
+template <class G, class S, class Fetch>
+VLTR_HD PairRec group_clip(G& g, S* s, const PairGeom& pg, int32_t a_id, uint64_t mlo, uint64_t mhi, Fetch&& f,
+                           const double* hex_s, const double* hex_c, int mode, int* status,
+                           const TopoOut* tout = nullptr) {
+  return group_clip_words(g, s, pg, a_id, mhi != 0ull ? 2 : 1, MaskPair{mlo, mhi}, f, hex_s, hex_c, mode, status,
+                          tout);
 }
 
 }  // namespace vltr

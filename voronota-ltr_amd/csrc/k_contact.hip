@@ -238,19 +238,27 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
 }
 
 // Pairs of balls with more than 64 neighbors (listed by k_clip): one 16-lane
-// group per pair classifies the pair's neighbors itself (16 at a time, up to
-// 128) and clips with group_clip, computing each cutting plane when it is
-// needed. Longer lists and contour overflows go to k_cut_slow. A fixed grid
-// walks the device-side count.
-constexpr int kWideBlocks = 1024;
-constexpr int kWideSlots = 128;
+// group per pair classifies the pair's neighbors itself, 16 at a time, into a
+// cutting mask in shared memory (one bit per neighbor slot, up to kNbMax;
+// the first killing neighbor ends the pair) and clips with group_clip_words,
+// computing each cutting plane when it is needed. Contour overflows go to
+// k_cut_slow. A fixed grid walks the device-side count.
+constexpr int kWideBlocks = 2048;
+constexpr int kWideWords = kNbMax / 64;
+
+struct WideMask {
+  const uint64_t* w;
+  __device__ uint64_t operator()(int k) const { return w[k]; }
+};
 
 __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
   __shared__ GroupLds grp[256 / kGroup];
+  __shared__ uint64_t cutw[256 / kGroup][kWideWords];
   const int gl = threadIdx.x / kGroup;
   const int ln = threadIdx.x & (kGroup - 1);
   WaveGroup G{(gl % (kWave / kGroup)) * kGroup, ln};
   GroupLds* gs = &grp[gl];
+  uint64_t* cw = cutw[gl];
   const double* sph4 = reinterpret_cast<const double*>(p.sph);
   const int64_t n_wide = *p.wide_cnt;
   const int64_t groups = static_cast<int64_t>(gridDim.x) * (256 / kGroup);
@@ -259,7 +267,7 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
     const int32_t a = p.pair_a[pidx], b = p.pair_b[pidx];
     const int cnt = p.nb_cnt[a];
     const int32_t* nbl = p.nb + static_cast<int64_t>(a) * p.nb_cap;
-    if (cnt > kWideSlots) {
+    if (cnt > kNbMax) {
       if (ln == 0) push_slow(p, pidx);
       continue;
     }
@@ -270,7 +278,9 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       continue;
     }
     bool kill = false;
-    uint64_t mlo = 0, mhi = 0;
+    bool any = false;
+    uint64_t word = 0;
+    const int nw = (cnt + 63) / 64;
     for (int s0 = 0; s0 < cnt; s0 += kGroup) {
       const int sl = s0 + ln;
       int cl = kSkip;
@@ -278,19 +288,24 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
         const Sph C = load_sph(sph4, nbl[sl]);
         cl = classify(g, C, plane_of(A, C), contains(C, A));
       }
-      kill = kill || G.ballot(cl == kKill) != 0u;
+      if (G.ballot(cl == kKill) != 0u) {
+        kill = true;
+        break;
+      }
       const uint64_t bits = static_cast<uint64_t>(G.ballot(cl == kCut));
-      if (s0 < 64) {
-        mlo |= bits << s0;
-      } else {
-        mhi |= bits << (s0 - 64);
+      word |= bits << (s0 & 63);
+      if ((s0 & 63) + kGroup == 64 || s0 + kGroup >= cnt) {
+        if (ln == 0) cw[s0 >> 6] = word;
+        any = any || word != 0ull;
+        word = 0;
       }
     }
+    G.sync();
     if (kill) {
       if (ln == 0) p.rec[pidx] = empty_record();
       continue;
     }
-    if ((mlo | mhi) == 0ull) {
+    if (!any) {
       if (ln == 0) {
         p.rec[pidx] = record_uncut(g, p.mode);
         if (p.topo != nullptr) p.topo[pidx].m = 0;
@@ -311,7 +326,9 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
     int st = kNone;
     const TopoOut tout{p.topo != nullptr ? p.topo + pidx : nullptr, sph4, p.net != nullptr ? p.net + pidx : nullptr};
     const bool want = p.topo != nullptr || p.net != nullptr;
-    const PairRec rec = group_clip(G, gs, g, a, mlo, mhi, fetch, c_hex_s, c_hex_c, p.mode, &st, want ? &tout : nullptr);
+    const PairRec rec = group_clip_words(G, gs, g, a, nw, WideMask{cw}, fetch, c_hex_s, c_hex_c, p.mode, &st,
+                                         want ? &tout : nullptr);
+    G.sync();
     if (ln == 0) {
       if (st == kOverflow) {
         push_slow(p, pidx);

@@ -72,6 +72,7 @@ __global__ void k_init(Pipeline p) {
     *p.status = 0u;
     *p.overflow_cnt = 0u;
     *p.wide_cnt = 0u;
+    *p.nb_big_cnt = 0u;
   }
 }
 
@@ -222,6 +223,11 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
   __syncthreads();
 
   if (!active) return;
+  if (!any_hidden && cnt > kListCap) {
+    // too many for this wavefront's list: k_neighbors_big lists the ball
+    if (lane == 0) p.nb_big[atomicAdd(p.nb_big_cnt, 1u)] = static_cast<int32_t>(a);
+    return;
+  }
   const int cap = p.nb_cap;
   int32_t* out = p.nb + a * static_cast<int64_t>(cap);
   uint32_t pairs = 0;
@@ -243,7 +249,124 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
     p.excluded[a] = any_hidden ? 1 : 0;
     p.pair_cnt[a] = any_hidden ? 0u : pairs;
     if (!any_hidden && cnt > cap) atomicOr(p.status, kStatNbOverflow);
-    if (cnt > kListCap) atomicOr(p.status, kStatNbOverflow);
+  }
+}
+
+// Balls with more than kListCap candidates: one block per ball (grid-stride
+// over the device-side list), candidates collected in LDS in any order and
+// put in (key, id) order by a bitonic sort, which is the order the wavefront
+// kernel's rank sort gives. None of these balls is hidden (the wavefront
+// kernel saw all its candidates and kept hidden balls itself).
+constexpr int kBigBlock = 256;
+constexpr int kBigGrid = 1024;
+
+__global__ __launch_bounds__(kBigBlock) void k_neighbors_big(Pipeline p) {
+  __shared__ double key[kNbMax];
+  __shared__ int32_t ids[kNbMax];
+  __shared__ uint32_t r_start[32], r_pre[32];
+  __shared__ uint32_t n_cand, total_s;
+  __shared__ uint32_t pairs_s;
+  const int t = threadIdx.x;
+  const uint32_t nbig = *p.nb_big_cnt;
+  for (uint32_t item = blockIdx.x; item < nbig; item += gridDim.x) {
+    const int64_t a = p.nb_big[item];
+    const double4 A4 = p.sph[a];
+    const Sph A{V3{A4.x, A4.y, A4.z}, A4.w};
+    if (t < 64) {
+      const int lane = t;
+      int32_t cx, cy, cz;
+      cell_of(A4, box_size(p.box_bits), &cx, &cy, &cz);
+      const GridSeg sg = p.segs[p.seg_of[a]];
+      uint32_t rs = 0, rc = 0;
+      if (lane < 27) {
+        const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+        const uint32_t b = static_cast<uint32_t>(sg.tab_off) + morton(cx + dx, cy + dy, cz + dz, sg.kbits);
+        rs = p.bucket_cnt[b];
+        rc = p.bucket_cnt[b + 1] - rs;
+      }
+      uint32_t inc = rc;
+      for (int off = 1; off < 32; off <<= 1) {
+        const uint32_t v = __shfl_up(inc, off, kWave);
+        if (lane >= off) inc += v;
+      }
+      if (lane < 27) {
+        r_start[lane] = rs;
+        r_pre[lane] = inc - rc;
+      }
+      if (lane == 26) total_s = inc;
+      if (lane == 0) {
+        n_cand = 0u;
+        pairs_s = 0u;
+      }
+    }
+    __syncthreads();
+    const uint32_t total = total_s;
+    for (uint32_t j = t; j < total; j += kBigBlock) {
+      int r = 0;
+      for (int k = 1; k < 27; ++k)
+        if (r_pre[k] <= j) r = k;
+      const uint32_t pos = r_start[r] + (j - r_pre[r]);
+      const int32_t c = p.sorted_id[pos];
+      if (c == a) continue;
+      const double4 C4 = p.sorted_sph[pos];
+      const Sph Cs{V3{C4.x, C4.y, C4.z}, C4.w};
+      if (intersects(A, Cs) && !(contains(Cs, A) && (!equals(Cs, A) || a > c)) && !contains(A, Cs)) {
+        const uint32_t slot = atomicAdd(&n_cand, 1u);
+        if (slot < static_cast<uint32_t>(kNbMax)) {
+          key[slot] = circle_center_distance(A, Cs);
+          ids[slot] = c;
+        }
+      }
+    }
+    __syncthreads();
+    const uint32_t cnt = n_cand;
+    if (cnt > static_cast<uint32_t>(kNbMax)) {
+      if (t == 0) atomicOr(p.status, kStatNbTooLong);
+      __syncthreads();
+      continue;
+    }
+    uint32_t len = 1;
+    while (len < cnt) len <<= 1;
+    for (uint32_t j = cnt + t; j < len; j += kBigBlock) {
+      key[j] = __longlong_as_double(0x7ff0000000000000ll);  // +inf: after every finite key
+      ids[j] = 0x7fffffff;
+    }
+    __syncthreads();
+    // bitonic sort of len entries, ascending by (key, id)
+    for (uint32_t size = 2; size <= len; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t i = t; i < len / 2; i += kBigBlock) {
+          const uint32_t lo = 2 * i - (i & (stride - 1));
+          const uint32_t hi = lo + stride;
+          const bool up = (lo & size) == 0;
+          const NbEntry x{key[lo], ids[lo]}, y{key[hi], ids[hi]};
+          if (nb_less(y, x) == up) {
+            key[lo] = y.key;
+            ids[lo] = y.id;
+            key[hi] = x.key;
+            ids[hi] = x.id;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    const int cap = p.nb_cap;
+    const int kept = static_cast<int>(cnt) < cap ? static_cast<int>(cnt) : cap;
+    int32_t* out = p.nb + a * static_cast<int64_t>(cap);
+    uint32_t pairs = 0;
+    for (int e = t; e < kept; e += kBigBlock) {
+      out[e] = ids[e];
+      if (pair_qualifies(p, a, ids[e])) ++pairs;
+    }
+    if (pairs) atomicAdd(&pairs_s, pairs);
+    __syncthreads();
+    if (t == 0) {
+      p.nb_cnt[a] = kept;
+      p.excluded[a] = 0;
+      p.pair_cnt[a] = pairs_s;
+      if (static_cast<int>(cnt) > cap) atomicOr(p.status, kStatNbOverflow);
+    }
+    __syncthreads();
   }
 }
 
@@ -405,7 +528,11 @@ hipError_t launch_grid(const Pipeline& p, hipStream_t s) {
 
 hipError_t launch_neighbors(const Pipeline& p, hipStream_t s) {
   if (p.n_pop > 0) hipLaunchKernelGGL(k_scatter, dim3(blocks_for(p.n_pop, 256)), dim3(256), 0, s, p);
-  if (p.n > 0) hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.n, kNbWaves)), dim3(kNbBlock), 0, s, p);
+  if (p.n > 0) {
+    hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.n, kNbWaves)), dim3(kNbBlock), 0, s, p);
+    const int64_t g = blocks_for(p.n, 64) < kBigGrid ? blocks_for(p.n, 64) : kBigGrid;
+    hipLaunchKernelGGL(k_neighbors_big, dim3(g), dim3(kBigBlock), 0, s, p);
+  }
   return hipGetLastError();
 }
 

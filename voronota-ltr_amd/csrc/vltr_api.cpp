@@ -10,6 +10,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <map>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -104,7 +107,8 @@ struct vltr_context {
   // device buffers
   DevBuf balls, groups, sph, seg_of, segs, bkey, brank, bcnt, sorted_id, sorted_sph, small, nb, nb_cnt, excluded,
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
-      in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad, net, net_out;
+      in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad, net, net_out, solv_cnt, solv_comb, solv_parent,
+      solv_sih, solv_ioff, solv_cur, solv_img, nb_big;
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
@@ -112,6 +116,10 @@ struct vltr_context {
   size_t last_structs = 0;  // structures of the last batched run (0: not batched)
   int last_nb_cap = 0;
   uint64_t counts[7] = {};
+};
+
+struct vltr_solvent_result {
+  std::vector<vltr_solvent_sphere> spheres;
 };
 
 struct vltr_batch_result {
@@ -179,6 +187,8 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.status = reinterpret_cast<uint32_t*>(small + 8);
   p.overflow_cnt = reinterpret_cast<uint32_t*>(small + 12);
   p.wide_cnt = reinterpret_cast<uint32_t*>(small + 16);
+  p.nb_big_cnt = reinterpret_cast<uint32_t*>(small + 20);
+  p.nb_big = ctx->nb_big.as<int32_t>();
   p.nb = ctx->nb.as<int32_t>();
   p.nb_cnt = ctx->nb_cnt.as<int32_t>();
   p.excluded = ctx->excluded.as<uint8_t>();
@@ -264,6 +274,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->sorted_sph.ensure(n_pop * sizeof(double4)));
   VLTR_HIP(ctx->small.ensure(64));
   VLTR_HIP(ctx->nb_cnt.ensure(n * sizeof(int32_t)));
+  VLTR_HIP(ctx->nb_big.ensure(n * sizeof(int32_t)));
   VLTR_HIP(ctx->excluded.ensure(n));
   VLTR_HIP(ctx->pair_cnt.ensure((n + 1) * sizeof(uint32_t)));
   VLTR_HIP(ctx->cell.ensure(n * sizeof(CellRec)));
@@ -290,8 +301,9 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     VLTR_HIP(hipMemcpyAsync(host_words + 1, p.status, 4, hipMemcpyDeviceToHost, s));
     VLTR_HIP(hipEventRecord(ctx->ev[2], s));
     VLTR_HIP(hipStreamSynchronize(s));
+    if (host_words[1] & kStatNbTooLong) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
     if (host_words[1] & kStatNbOverflow) {
-      if (ctx->nb_cap >= 4096) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
+      if (ctx->nb_cap >= kNbMax) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
       ctx->nb_cap *= 2;
       continue;
     }
@@ -354,7 +366,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   if (host_words[5] & kStatNbOverflow) {
     // an incoming-pair list outgrew nb_cap (a hidden ball with many
     // neighbors): redo everything with longer lists
-    if (ctx->nb_cap >= 4096) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
+    if (ctx->nb_cap >= kNbMax) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
     ctx->nb_cap *= 2;
     goto restart;
   }
@@ -582,6 +594,163 @@ int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe
   return VLTR_OK;
 }
 
+// A double as Rust's Display prints it: the shortest digits that read back
+// to the same value, in positional notation; "NaN", "inf", "-inf".
+std::string rust_display(double v) {
+  if (std::isnan(v)) return "NaN";
+  if (std::isinf(v)) return v < 0 ? "-inf" : "inf";
+  char buf[64];
+  int prec = 1;
+  for (; prec < 17; ++prec) {
+    std::snprintf(buf, sizeof buf, "%.*e", prec - 1, v);
+    if (std::strtod(buf, nullptr) == v) break;
+  }
+  std::snprintf(buf, sizeof buf, "%.*e", prec - 1, v);
+  // buf = [-]d.ddde[+-]X: place the point X digits to the right
+  std::string str(buf);
+  const bool neg = str[0] == '-';
+  if (neg) str.erase(0, 1);
+  const size_t epos = str.find('e');
+  const int ex = std::atoi(str.c_str() + epos + 1);
+  std::string digits = str.substr(0, epos);
+  digits.erase(std::remove(digits.begin(), digits.end(), '.'), digits.end());
+  while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+  std::string out;
+  const int point = ex + 1;  // digits before the decimal point
+  if (point <= 0) {
+    out = "0." + std::string(static_cast<size_t>(-point), '0') + digits;
+  } else if (point >= static_cast<int>(digits.size())) {
+    out = digits + std::string(static_cast<size_t>(point) - digits.size(), '0');
+  } else {
+    out = digits.substr(0, static_cast<size_t>(point)) + "." + digits.substr(static_cast<size_t>(point));
+  }
+  if (out == "0") return neg ? "-0" : "0";
+  return (neg ? "-" : "") + out;
+}
+
+// subdivided_icosahedron.rs:51-131: unit vertices in order of first
+// appearance (base vertices, then edge midpoints as the faces are walked).
+std::vector<double> icosahedron(int depth) {
+  const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+  const double base[12][3] = {{t, 1.0, 0.0},  {-t, 1.0, 0.0},  {t, -1.0, 0.0}, {-t, -1.0, 0.0},
+                              {1.0, 0.0, t},  {1.0, 0.0, -t},  {-1.0, 0.0, t}, {-1.0, 0.0, -t},
+                              {0.0, t, 1.0},  {0.0, -t, 1.0},  {0.0, t, -1.0}, {0.0, -t, -1.0}};
+  std::vector<V3> v;
+  for (const auto& b : base) v.push_back(unit(V3{b[0], b[1], b[2]}));
+  std::vector<std::array<int, 3>> tri = {{0, 8, 4},  {1, 10, 7}, {2, 9, 11}, {7, 3, 1},  {0, 5, 10},
+                                         {3, 9, 6},  {3, 11, 9}, {8, 6, 4},  {2, 4, 9},  {3, 7, 11},
+                                         {4, 2, 0},  {9, 4, 6},  {2, 11, 5}, {0, 10, 8}, {5, 0, 2},
+                                         {10, 5, 7}, {1, 6, 8},  {1, 8, 10}, {6, 1, 3},  {11, 7, 5}};
+  for (int d = 0; d < depth; ++d) {
+    std::vector<std::array<int, 3>> next;
+    std::map<std::pair<int, int>, int> mids;
+    for (const auto& f : tri) {
+      const std::pair<int, int> edges[3] = {{f[1], f[2]}, {f[0], f[2]}, {f[0], f[1]}};
+      int mid[3];
+      for (int e = 0; e < 3; ++e) {
+        const int a = edges[e].first, b = edges[e].second;
+        const auto key = a < b ? std::make_pair(a, b) : std::make_pair(b, a);
+        auto it = mids.find(key);
+        if (it == mids.end()) {
+          v.push_back(unit((v[a] + v[b]) * 0.5));
+          it = mids.emplace(key, static_cast<int>(v.size()) - 1).first;
+        }
+        mid[e] = it->second;
+      }
+      next.push_back({f[0], mid[1], mid[2]});
+      next.push_back({f[1], mid[0], mid[2]});
+      next.push_back({f[2], mid[0], mid[1]});
+      next.push_back({mid[0], mid[1], mid[2]});
+    }
+    tri.swap(next);
+  }
+  std::vector<double> out;
+  for (const auto& p : v) {
+    out.push_back(p.x);
+    out.push_back(p.y);
+    out.push_back(p.z);
+  }
+  return out;
+}
+
+// solvent_spheres.rs:131-168 on the device: stage 1, sampling of the exposed
+// atoms (k_solvent), stage 2 on atoms + samples, weights = stage-2 volumes.
+int solvent_spheres(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe, double volume_probe, int depth,
+                    const vltr_periodic_box* pbox, std::vector<vltr_solvent_sphere>* out) {
+  out->clear();
+  if (n == 0) return VLTR_OK;
+  VLTR_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const std::vector<double> sih = icosahedron(depth);
+  const int K = static_cast<int>(sih.size() / 3);
+  VLTR_HIP(ctx->solv_sih.ensure(sih.size() * sizeof(double)));
+  VLTR_HIP(hipMemcpyAsync(ctx->solv_sih.ptr, sih.data(), sih.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  VLTR_HIP(ctx->solv_cnt.ensure((n + 1) * sizeof(uint32_t)));
+  VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
+  VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, balls, n * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
+  const double sr = probe + volume_probe;
+  Pipeline saved{};
+  RunExtras ex;
+  uint32_t* cnt = ctx->solv_cnt.as<uint32_t>();
+  const double* d_sih = ctx->solv_sih.as<double>();
+  const uint32_t* ioff = nullptr;
+  const int32_t* img = nullptr;
+  ex.post_cells = [&](const Pipeline& p, hipStream_t st) -> hipError_t {
+    saved = p;
+    uint32_t* tmp = ctx->scan_tmp.as<uint32_t>();
+    hipError_t e = hipSuccess;
+    if (p.periodic) {
+      if ((e = ctx->solv_ioff.ensure((n + 1) * sizeof(uint32_t))) != hipSuccess) return e;
+      if ((e = ctx->solv_cur.ensure(n * sizeof(uint32_t))) != hipSuccess) return e;
+      if ((e = ctx->solv_img.ensure(std::max<int64_t>(p.n_pairs, 1) * sizeof(int32_t))) != hipSuccess) return e;
+      ioff = ctx->solv_ioff.as<uint32_t>();
+      img = ctx->solv_img.as<int32_t>();
+      e = launch_solvent_images(p, ctx->solv_ioff.as<uint32_t>(), ctx->solv_cur.as<uint32_t>(),
+                                ctx->solv_img.as<int32_t>(), tmp, st);
+      if (e != hipSuccess) return e;
+    }
+    e = launch_solvent(p, ioff, img, d_sih, K, probe, sr, cnt, 0, nullptr, nullptr, st);
+    if (e != hipSuccess) return e;
+    return launch_scan_u32(cnt, static_cast<int64_t>(n), tmp, st);
+  };
+  size_t nc = 0, ncell = 0;
+  int rc = run_pipeline(ctx, static_cast<int64_t>(n), probe, pbox, nullptr, 0, 0, &nc, &ncell, nullptr, 0, &ex);
+  if (rc != VLTR_OK) return rc;
+  uint32_t m = 0;
+  VLTR_HIP(hipMemcpy(&m, cnt + n, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (m == 0) return VLTR_OK;
+  const size_t total = n + m;
+  if (total > (size_t(1) << 31) / 27) return fail(VLTR_ERR_INVALID_ARGUMENT, "too many solvent spheres");
+  VLTR_HIP(ctx->solv_comb.ensure(total * sizeof(vltr_ball)));
+  VLTR_HIP(ctx->solv_parent.ensure(m * sizeof(uint32_t)));
+  double4* comb = ctx->solv_comb.as<double4>();
+  VLTR_HIP(launch_solvent_atoms(saved.balls_in, static_cast<int64_t>(n), probe, volume_probe, comb, s));
+  VLTR_HIP(launch_solvent(saved, ioff, img, d_sih, K, probe, sr, cnt, 1, comb, ctx->solv_parent.as<uint32_t>(), s));
+  std::vector<vltr_ball> pts(m);
+  std::vector<uint32_t> parents(m);
+  VLTR_HIP(hipMemcpyAsync(pts.data(), comb + n, m * sizeof(vltr_ball), hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipMemcpyAsync(parents.data(), ctx->solv_parent.ptr, m * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  // stage 2 (probe 0) on atoms + samples; the stage-1 buffers are reused
+  VLTR_HIP(hipStreamSynchronize(s));
+  VLTR_HIP(ctx->balls.ensure(total * sizeof(vltr_ball)));
+  VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, comb, total * sizeof(vltr_ball), hipMemcpyDeviceToDevice, s));
+  rc = run_pipeline(ctx, static_cast<int64_t>(total), 0.0, pbox, nullptr, 0, 0, &nc, &ncell);
+  if (rc != VLTR_OK) return rc;
+  std::vector<CellRec> cells(m);
+  VLTR_HIP(hipMemcpy(cells.data(), ctx->cell.as<CellRec>() + n, m * sizeof(CellRec), hipMemcpyDeviceToHost));
+  out->resize(m);
+  for (uint32_t k = 0; k < m; ++k) {
+    vltr_solvent_sphere& o = (*out)[k];
+    o.x = pts[k].x;
+    o.y = pts[k].y;
+    o.z = pts[k].z;
+    o.radius = probe;
+    o.weight = cells[k].stage == 2 ? cells[k].volume : 0.0;
+    o.parent_index = parents[k];
+  }
+  return VLTR_OK;
+}
+
 int check_offsets(const uint64_t* offsets, size_t n_structs) {
   if (!offsets || n_structs == 0) return fail(VLTR_ERR_INVALID_ARGUMENT, "no structures");
   if (offsets[0] != 0) return fail(VLTR_ERR_INVALID_ARGUMENT, "offsets[0] must be 0");
@@ -689,6 +858,40 @@ int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64
   }
   return VLTR_OK;
 }
+
+// solvent_spheres.rs:131-201 (validation messages as its Display impl).
+int vltr_compute_solvent_spheres(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe,
+                                 double volume_probe, int subdivision_depth, const vltr_periodic_box* periodic_box,
+                                 vltr_solvent_result** out) {
+  if (!ctx || !out || (n > 0 && !balls)) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (!std::isfinite(probe) || probe < 0.0)
+    return fail(VLTR_ERR_INVALID_ARGUMENT,
+                "invalid probe radius: " + rust_display(probe) + " (must be non-negative and finite)");
+  if (!std::isfinite(volume_probe) || volume_probe < 0.0)
+    return fail(VLTR_ERR_INVALID_ARGUMENT,
+                "invalid volume probe: " + rust_display(volume_probe) + " (must be non-negative and finite)");
+  if (subdivision_depth < 0) return fail(VLTR_ERR_INVALID_ARGUMENT, "subdivision depth must be 0..4");
+  for (size_t i = 0; i < n; ++i) {
+    const vltr_ball& b = balls[i];
+    if (!std::isfinite(b.x) || !std::isfinite(b.y) || !std::isfinite(b.z))
+      return fail(VLTR_ERR_INVALID_ARGUMENT, "invalid ball at index " + std::to_string(i) + ": coordinates must be finite");
+    if (!std::isfinite(b.r) || b.r <= 0.0)
+      return fail(VLTR_ERR_INVALID_ARGUMENT,
+                  "invalid ball at index " + std::to_string(i) + ": radius must be positive and finite");
+  }
+  auto r = std::make_unique<vltr_solvent_result>();
+  const int rc = solvent_spheres(ctx, balls, n, probe, volume_probe, std::min(subdivision_depth, 4), periodic_box,
+                                 &r->spheres);
+  if (rc != VLTR_OK) return rc;
+  *out = r.release();
+  return VLTR_OK;
+}
+size_t vltr_solvent_count(const vltr_solvent_result* r) { return r ? r->spheres.size() : 0; }
+const vltr_solvent_sphere* vltr_solvent_spheres(const vltr_solvent_result* r) {
+  return r && !r->spheres.empty() ? r->spheres.data() : nullptr;
+}
+void vltr_solvent_free(vltr_solvent_result* r) { delete r; }
 
 // Energy and its gradient (forward + adjoint) on device-resident balls.
 int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
@@ -813,7 +1016,9 @@ void vltr_context_destroy(vltr_context* ctx) {
                     &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec, &ctx->cell, &ctx->overflow_list, &ctx->wide_list,
                     &ctx->flags, &ctx->scan_tmp, &ctx->out_contacts, &ctx->out_cells, &ctx->slow_ws,
                     &ctx->in_cnt, &ctx->incoming, &ctx->seg_out, &ctx->topo, &ctx->seed, &ctx->energy_part,
-                    &ctx->types, &ctx->grad, &ctx->net, &ctx->net_out})
+                    &ctx->types, &ctx->grad, &ctx->net, &ctx->net_out, &ctx->solv_cnt, &ctx->solv_comb,
+                    &ctx->solv_parent, &ctx->solv_sih, &ctx->solv_ioff, &ctx->solv_cur, &ctx->solv_img,
+                    &ctx->nb_big})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)

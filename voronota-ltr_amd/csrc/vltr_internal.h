@@ -37,6 +37,8 @@ struct GridSeg {
 constexpr uint32_t kStatNbOverflow = 1u;    // a neighbor list exceeded its capacity
 constexpr uint32_t kStatPairOverflow = 2u;  // more pairs than the pair buffers hold
 constexpr uint32_t kStatContourOverflow = 4u;
+constexpr uint32_t kStatNbTooLong = 8u;        // a ball has more than kNbMax neighbors
+constexpr int kNbMax = 4096;
 
 // Launch parameters of the tessellation pipeline for one call (one or many
 // structures). All pointers are device pointers.
@@ -81,6 +83,8 @@ struct Pipeline {
   uint32_t* overflow_cnt;  // 1
   int32_t* wide_list;      // pair_cap (pairs of balls with more than 64 neighbors)
   uint32_t* wide_cnt;      // 1
+  int32_t* nb_big;         // n: balls with more candidates than one wavefront's list holds
+  uint32_t* nb_big_cnt;    // 1
   double* slow_ws;         // workspace of the slow kernel
   int64_t slow_ws_pairs;   // its lanes (one workspace region each)
 
@@ -133,6 +137,17 @@ hipError_t upload_hexagon_constants(hipStream_t s);
 hipError_t launch_adjoint(const Pipeline& p, const int32_t* types, const double* w16, double* grad, double* part,
                           hipStream_t s);
 int64_t adjoint_blocks();
+
+// k_solvent.hip: emit == 0 counts the kept icosahedron samples (K unit
+// vectors in sih) of every exposed ball into cnt; emit != 0 (cnt scanned)
+// writes them as balls of radius sr at comb[n + k] with their parents.
+hipError_t launch_solvent_images(const Pipeline& p, uint32_t* ioff, uint32_t* cur, int32_t* img, uint32_t* scan_tmp,
+                                 hipStream_t s);
+hipError_t launch_solvent(const Pipeline& p, const uint32_t* ioff, const int32_t* img, const double* sih, int K,
+                          double probe, double sr, uint32_t* cnt, int emit, double4* comb, uint32_t* parent,
+                          hipStream_t s);
+hipError_t launch_solvent_atoms(const double4* balls, int64_t n, double probe, double volume_probe, double4* comb,
+                                hipStream_t s);
 
 // k_cells.hip
 hipError_t launch_cells(const Pipeline& p, hipStream_t s);

@@ -194,3 +194,17 @@ class UpdateableTessellation:
 
     def backup_enabled(self):
         return self._u.backup_enabled()
+
+
+def compute_solvent_spheres(balls, probe, volume_probe=None, subdivision_depth=None, periodic_box=None):
+    """solvent_spheres.rs:131-168: list of dicts (x, y, z, radius, weight,
+    parent_index); volume_probe defaults to 0.0 and subdivision_depth to 2
+    (162 samples per exposed atom)."""
+    depth = 2 if subdivision_depth is None else int(subdivision_depth)
+    if depth < 0:
+        raise OverflowError("can't convert negative int to unsigned")
+    pb = parse_periodic_box(periodic_box) if periodic_box is not None else None
+    arr = _native.solvent_spheres(parse_balls(balls), probe, 0.0 if volume_probe is None else volume_probe,
+                                  min(depth, 4), pb)
+    return [{"x": float(r["x"]), "y": float(r["y"]), "z": float(r["z"]), "radius": float(r["radius"]),
+             "weight": float(r["weight"]), "parent_index": int(r["parent_index"])} for r in arr]

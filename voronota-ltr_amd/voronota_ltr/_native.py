@@ -34,6 +34,8 @@ CONTACT_DTYPE = np.dtype([("id_a", "<u8"), ("id_b", "<u8"), ("area", "<f8"), ("a
 CELL_DTYPE = np.dtype([("index", "<u8"), ("sas_area", "<f8"), ("volume", "<f8")])
 VERTEX_DTYPE = np.dtype([("ball_indices", "<u8", (4,)), ("x", "<f8"), ("y", "<f8"), ("z", "<f8")])
 EDGE_DTYPE = np.dtype([("ball_indices", "<u8", (3,)), ("length", "<f8")])
+SOLVENT_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("radius", "<f8"), ("weight", "<f8"),
+                          ("parent_index", "<u8")])
 assert CONTACT_DTYPE.itemsize == 40 and CELL_DTYPE.itemsize == 24
 
 _vp = C.c_void_p
@@ -85,6 +87,11 @@ SIGNATURES = {
                                        _f64p]),
     "vltr_energy_gradient_device": (C.c_int, [_vp, _vp, _sz, C.c_double, _vp, _f64p, C.c_double, C.c_double, _vp,
                                               _f64p]),
+    "vltr_compute_solvent_spheres": (C.c_int, [_vp, _vp, _sz, C.c_double, C.c_double, C.c_int,
+                                               C.POINTER(VltrPeriodicBox), C.POINTER(_vp)]),
+    "vltr_solvent_count": (_sz, [_vp]),
+    "vltr_solvent_spheres": (_vp, [_vp]),
+    "vltr_solvent_free": (None, [_vp]),
     "vltr_updateable_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
     "vltr_updateable_destroy": (None, [_vp]),
     "vltr_updateable_init": (C.c_int, [_vp, _vp, _sz, C.c_double, C.POINTER(VltrPeriodicBox), C.POINTER(C.c_int)]),
@@ -419,3 +426,23 @@ def energy_gradient(balls, probe, types, W, sas_weight=1.0, volume_weight=1.0, d
                                   g.ctypes.data_as(_f64p) if len(b) else None)
     check(rc, "vltr_energy_gradient")
     return e.value, g
+
+
+def solvent_spheres(balls, probe, volume_probe=0.0, depth=2, pbox=None, device=0):
+    """compute_solvent_spheres (solvent_spheres.rs:131-168) as a structured
+    array (x, y, z, radius, weight, parent_index)."""
+    lib = load_library()
+    ctx = context(device)
+    b = _balls_array(balls)
+    box = make_pbox(pbox)
+    h = C.c_void_p()
+    rc = lib.vltr_compute_solvent_spheres(ctx, b.ctypes.data_as(_vp) if len(b) else None, len(b), float(probe),
+                                          float(volume_probe), int(depth), C.byref(box) if box is not None else None,
+                                          C.byref(h))
+    if rc == 1:
+        raise ValueError(last_error())
+    check(rc, "vltr_compute_solvent_spheres")
+    try:
+        return _copy(lib.vltr_solvent_spheres(h), lib.vltr_solvent_count(h), SOLVENT_DTYPE)
+    finally:
+        lib.vltr_solvent_free(h)

@@ -48,17 +48,18 @@ constexpr int kClipWaves = 4;   // wavefronts per k_clip block
 constexpr int kClipGroup = VLTR_CLIP_GROUP;
 using ClipGroup = WaveGroupT<kClipGroup>;
 using ClipGroupLds = GroupLdsT<2 * kClipGroup>;
-// Occupancy target of k_clip (wavefronts per SIMD). Its shared memory allows
-// three 4-wavefront blocks per CU, so 3 caps registers at 168 per lane.
+// Occupancy target of k_clip (wavefronts per SIMD). Its shared memory
+// (38.9 KB per 4-wavefront block) allows four blocks per CU; 4 caps registers
+// at 128 per lane, which spills some phase-1 state but measures faster than
+// 3 wavefronts with 168 registers (clip 3.20 vs 3.49 ms at 100k balls).
 #ifndef VLTR_CLIP_WAVES_PER_EU
-#define VLTR_CLIP_WAVES_PER_EU 3
+#define VLTR_CLIP_WAVES_PER_EU 4
 #endif
 
 // Shared memory of one k_clip wavefront.
 struct ClipLds {
   double pcx[kClipSlots], pcy[kClipSlots], pcz[kClipSlots];
   double pnx[kClipSlots], pny[kClipSlots], pnz[kClipSlots];
-  double nnx[kClipSlots], nny[kClipSlots], nnz[kClipSlots];
   double thr[kClipSlots];  // squared distance from a's centre to the plane point
   int32_t cid[kClipSlots];
   // listed pairs: (pair offset within a << 8) | slot of b; pairs to clip
@@ -121,16 +122,12 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     P = plane_of(A, C);
     c_has_a = contains(C, A);
     gk_ok = pair_setup(A, C, &gk);
-    const V3 nn = unit(P.pn);
     w.pcx[lane] = P.pc.x;
     w.pcy[lane] = P.pc.y;
     w.pcz[lane] = P.pc.z;
     w.pnx[lane] = P.pn.x;
     w.pny[lane] = P.pn.y;
     w.pnz[lane] = P.pn.z;
-    w.nnx[lane] = nn.x;
-    w.nny[lane] = nn.y;
-    w.nnz[lane] = nn.z;
     w.thr[lane] = norm2(P.pc - A.c);
     w.cid[lane] = c0;
   }
@@ -205,12 +202,14 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
   const int grp = lane / kClipGroup;
   ClipGroup G{grp * kClipGroup, lane & (kClipGroup - 1)};
   ClipGroupLds* gs = &w.grp[grp];
+  // the unit normal is recomputed per applied plane: keeping it in shared
+  // memory would cost the fourth wavefront per SIMD
   struct {
     const ClipLds& w;
     __device__ double thr(int s) const { return w.thr[s]; }
     __device__ CutPlane plane(int s) const {
-      return CutPlane{V3{w.pcx[s], w.pcy[s], w.pcz[s]}, V3{w.pnx[s], w.pny[s], w.pnz[s]},
-                      V3{w.nnx[s], w.nny[s], w.nnz[s]}, w.cid[s]};
+      const V3 pn{w.pnx[s], w.pny[s], w.pnz[s]};
+      return CutPlane{V3{w.pcx[s], w.pcy[s], w.pcz[s]}, pn, unit(pn), w.cid[s]};
     }
   } fetch{w};
   for (int r0 = 0; r0 < n_clip; r0 += kWave / kClipGroup) {

@@ -171,6 +171,33 @@ int oracle_pair_records(const double* balls, size_t n, double probe, const doubl
   return 0;
 }
 
+// Clipped contact polygons for graphics.rs:45-84 (non-periodic): for each
+// pair with area > 0, in pair order, 14 doubles (a, b, ic centre xyz, ic r,
+// axis xyz, barycenter xyz, m) then m x 6 doubles (x, y, z, angle, left id,
+// right id). Returns the number of doubles (out may be NULL to size it).
+int64_t oracle_contact_polygons(const double* balls, size_t n, double probe, const int32_t* groups, size_t n_groups,
+                                double* out, int64_t max_out) {
+  auto b = load_balls<double>(balls, n);
+  if (n == 0) return 0;
+  auto ctx = make_collision_context<double>(b, probe, nullptr, groups, n_groups, 1);
+  std::vector<double> v;
+  for (const auto& pr : ctx.pairs) {
+    const size_t a = pr.first, bb = pr.second;
+    auto cc = clip_contact(ctx.searcher.spheres(), a, bb, ctx.all_collisions[a]);
+    if (!cc || !(cc->area > 0.0)) continue;
+    const double head[14] = {double(a), double(bb), cc->ic.center.x, cc->ic.center.y, cc->ic.center.z, cc->ic.r,
+                             cc->axis.x, cc->axis.y, cc->axis.z, cc->barycenter.x, cc->barycenter.y,
+                             cc->barycenter.z, double(cc->contour.size()), 0.0};
+    v.insert(v.end(), head, head + 14);
+    for (const auto& cp : cc->contour) {
+      const double pt[6] = {cp.p.x, cp.p.y, cp.p.z, cp.angle, double(cp.left_id), double(cp.right_id)};
+      v.insert(v.end(), pt, pt + 6);
+    }
+  }
+  if (out) std::copy(v.begin(), v.begin() + std::min<int64_t>(max_out, static_cast<int64_t>(v.size())), out);
+  return static_cast<int64_t>(v.size());
+}
+
 size_t oracle_result_num_balls(const oracle_result* r) { return r->num_balls; }
 size_t oracle_result_num_contacts(const oracle_result* r) { return r->contacts.size(); }
 size_t oracle_result_num_cells(const oracle_result* r) { return r->cells.size(); }

@@ -361,3 +361,29 @@ def icosahedron(depth):
     out = np.zeros((m, 3))
     f(int(depth), out.ctypes.data, m)
     return out
+
+
+def contact_polygons(balls, probe, groups=None):
+    """Per contact with area > 0 (pair order): dict a, b, ic_center, ic_r,
+    axis, bary, contour [(x, y, z, angle, left, right), ...]."""
+    L = _load()
+    f = L.oracle_contact_polygons
+    f.restype = C.c_int64
+    f.argtypes = [_vp, C.c_size_t, C.c_double, _vp, C.c_size_t, _vp, C.c_int64]
+    b = np.ascontiguousarray(np.asarray(balls, np.float64).reshape(-1, 4))
+    g = None if groups is None else np.ascontiguousarray(np.asarray(groups, np.int32))
+    args = (b.ctypes.data if len(b) else None, len(b), float(probe), g.ctypes.data if g is not None else None,
+            len(g) if g is not None else 0)
+    m = f(*args, None, 0)
+    out = np.zeros(max(m, 1))
+    f(*args, out.ctypes.data, m)
+    polys, i = [], 0
+    while i < m:
+        h = out[i:i + 14]
+        k = int(h[12])
+        pts = out[i + 14:i + 14 + 6 * k].reshape(k, 6)
+        polys.append({"a": int(h[0]), "b": int(h[1]), "ic_center": tuple(h[2:5]), "ic_r": float(h[5]),
+                      "axis": tuple(h[6:9]), "bary": tuple(h[9:12]),
+                      "contour": [(p[0], p[1], p[2], p[3], int(p[4]), int(p[5])) for p in pts]})
+        i += 14 + 6 * k
+    return polys

@@ -1,7 +1,8 @@
 """C ABI and Python API surface, checked without a GPU.
 
 * the HIP library loads and exports every function include/voronota_ltr.h
-  declares (dlopen only: no HIP runtime call is made);
+  and include/voronota_ltr_input.h declare (dlopen only: no HIP runtime call
+  is made);
 * the ctypes binding declares exactly the header's functions;
 * the Python module parses balls and periodic boxes like src/python.rs:28-120
   (same accepted forms, same exception types);
@@ -16,12 +17,12 @@ import pytest
 
 from conftest import ROOT
 
-HEADER = os.path.join(ROOT, "include", "voronota_ltr.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("voronota_ltr.h", "voronota_ltr_input.h")]
 LIB = os.path.join(ROOT, "voronota-ltr_amd", "voronota_ltr", "libvltr_hip.so")
 
 
 def header_functions():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(vltr_[a-z0-9_]+)\s*\(", text)))
 

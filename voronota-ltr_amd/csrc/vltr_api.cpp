@@ -22,6 +22,7 @@
 #include <memory>
 #include <vector>
 
+#include "vltr_host.h"
 #include "vltr_internal.h"
 #include "voronota_ltr.h"
 
@@ -129,14 +130,6 @@ struct vltr_batch_result {
   std::vector<vltr_cell> cells;
 };
 
-struct vltr_result {
-  size_t num_balls = 0;
-  std::vector<vltr_contact> contacts;
-  std::vector<vltr_cell> cells;
-  bool has_network = false;
-  std::vector<vltr_cell_vertex> vertices;
-  std::vector<vltr_cell_edge> edges;
-};
 
 namespace {
 
@@ -505,7 +498,8 @@ int build_network(vltr_result* r, const std::vector<ContactNet>& nets, uint64_t 
 }
 
 int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe, const vltr_periodic_box* pbox,
-               const int32_t* groups, size_t n_groups, int with_vertices, int mode, vltr_result** out) {
+               const int32_t* groups, size_t n_groups, int with_vertices, int mode, vltr_result** out,
+               std::vector<ContactNet>* keep_nets = nullptr) {
   if (!ctx || !out || (n > 0 && !balls)) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
   if (n > (size_t(1) << 31) / 27) return fail(VLTR_ERR_INVALID_ARGUMENT, "too many balls");
   *out = nullptr;
@@ -578,6 +572,7 @@ int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe
     if (net) {
       rc = build_network(r, nets, pbox ? n : 0);
       if (rc != VLTR_OK) break;
+      if (keep_nets) keep_nets->swap(nets);
     }
     if (pbox) {
       if (mode == 0)
@@ -776,6 +771,18 @@ int run_batched(vltr_context* ctx, const uint64_t* offsets, size_t n_structs, do
   return run_pipeline(ctx, n, probe, nullptr, nullptr, 0, mode, nc, ncell, offsets, static_cast<int64_t>(n_structs));
 }
 
+}  // namespace
+
+namespace vltr {
+int host_fail(int code, const std::string& msg) { return fail(code, msg); }
+int host_tessellate(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe, const vltr_periodic_box* pbox,
+                    const int32_t* groups, size_t n_groups, int with_vertices, vltr_result** out,
+                    std::vector<ContactNet>* nets) {
+  return host_entry(ctx, balls, n, probe, pbox, groups, n_groups, with_vertices, 0, out, nets);
+}
+}  // namespace vltr
+
+namespace {
 }  // namespace
 
 extern "C" {

@@ -106,7 +106,11 @@ def compute_tessellation(balls, probe, periodic_box=None, groups=None, with_cell
     b = parse_balls(balls)
     pb = parse_periodic_box(periodic_box) if periodic_box is not None else None
     g = None if groups is None else [int(x) for x in groups]
-    r = _native.run(b, float(probe), pb, g, with_cell_vertices, False)
+    return _result_dict(_native.run(b, float(probe), pb, g, with_cell_vertices, False), with_cell_vertices)
+
+
+def _result_dict(r, with_cell_vertices):
+    """result_to_dict (python.rs:189-210)."""
     out = {
         "num_balls": int(r.num_balls),
         "contacts": _contacts_to_list(r.contacts),
@@ -208,3 +212,13 @@ def compute_solvent_spheres(balls, probe, volume_probe=None, subdivision_depth=N
                                   min(depth, 4), pb)
     return [{"x": float(r["x"]), "y": float(r["y"]), "z": float(r["z"]), "radius": float(r["radius"]),
              "weight": float(r["weight"]), "parent_index": int(r["parent_index"])} for r in arr]
+
+
+def compute_tessellation_from_file(path, probe, periodic_box=None, with_cell_vertices=False, group_selections=None):
+    """src/python.rs:292-324: parse a PDB / mmCIF / XYZR file (default
+    options and radii), optionally group atoms by selections, tessellate; the
+    dict of compute_tessellation. Every failure raises OSError (IOError)."""
+    pb = parse_periodic_box(periodic_box) if periodic_box is not None else None
+    res = _native.from_file(path, probe, pb, with_cell_vertices,
+                            None if group_selections is None else list(group_selections))
+    return _result_dict(res, with_cell_vertices)

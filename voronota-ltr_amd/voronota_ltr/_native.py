@@ -87,6 +87,29 @@ SIGNATURES = {
                                        _f64p]),
     "vltr_energy_gradient_device": (C.c_int, [_vp, _vp, _sz, C.c_double, _vp, _f64p, C.c_double, C.c_double, _vp,
                                               _f64p]),
+    "vltr_radii_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "vltr_radii_load_text": (C.c_int, [_vp, C.c_char_p]),
+    "vltr_radii_get": (C.c_double, [_vp, C.c_char_p, C.c_char_p]),
+    "vltr_radii_free": (None, [_vp]),
+    "vltr_parse_file": (C.c_int, [C.c_char_p, _vp, _vp, C.POINTER(_vp)]),
+    "vltr_parse_text": (C.c_int, [C.c_char_p, _sz, C.c_int, _vp, _vp, C.POINTER(_vp)]),
+    "vltr_structure_format": (C.c_int, [_vp]),
+    "vltr_structure_num_balls": (_sz, [_vp]),
+    "vltr_structure_balls": (_vp, [_vp]),
+    "vltr_structure_num_records": (_sz, [_vp]),
+    "vltr_record_string": (C.c_char_p, [_vp, _sz, C.c_int]),
+    "vltr_record_numbers": (C.c_int, [_vp, _sz, _i32p, _i32p, _f64p]),
+    "vltr_structure_free": (None, [_vp]),
+    "vltr_grouping_chain": (C.c_int, [_vp, _i32p]),
+    "vltr_grouping_residue": (C.c_int, [_vp, _i32p]),
+    "vltr_grouping_selections": (C.c_int, [_vp, C.POINTER(C.c_char_p), _sz, _i32p]),
+    "vltr_selection_match": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint8)]),
+    "vltr_compute_tessellation_from_file": (C.c_int, [_vp, C.c_char_p, C.c_double, C.POINTER(VltrPeriodicBox),
+                                                      C.c_int, C.POINTER(C.c_char_p), _sz, C.POINTER(_vp)]),
+    "vltr_result_to_json": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_sz)]),
+    "vltr_pymol_graphics": (C.c_int, [_vp, _vp, _sz, C.c_double, _i32p, _sz, C.c_double, C.c_char_p,
+                                      C.POINTER(_vp), C.POINTER(_sz)]),
+    "vltr_string_free": (None, [_vp]),
     "vltr_compute_solvent_spheres": (C.c_int, [_vp, _vp, _sz, C.c_double, C.c_double, C.c_int,
                                                C.POINTER(VltrPeriodicBox), C.POINTER(_vp)]),
     "vltr_solvent_count": (_sz, [_vp]),
@@ -446,3 +469,186 @@ def solvent_spheres(balls, probe, volume_probe=0.0, depth=2, pbox=None, device=0
         return _copy(lib.vltr_solvent_spheres(h), lib.vltr_solvent_count(h), SOLVENT_DTYPE)
     finally:
         lib.vltr_solvent_free(h)
+
+
+# ------------------------------------------------------------ file front end
+VLTR_ERR_IO = 5
+FORMATS = {"auto": -1, "pdb": 0, "mmcif": 1, "xyzr": 2}
+RECORD_FIELDS = ["record_name", "name", "alt_loc", "res_name", "chain_id", "i_code", "element"]
+
+
+class ParseOptions(C.Structure):
+    """vltr_parse_options (src/input/mod.rs:69-77)."""
+    _fields_ = [("exclude_heteroatoms", C.c_int), ("include_hydrogens", C.c_int), ("as_assembly", C.c_int)]
+
+
+class Radii:
+    """RadiiLookup (src/input/radii.rs): default rules, or none with
+    default=False; load_text raises ValueError with the reference's message."""
+
+    def __init__(self, default=True):
+        lib = load_library()
+        self._h = C.c_void_p()
+        check(lib.vltr_radii_create(int(bool(default)), C.byref(self._h)), "vltr_radii_create")
+
+    def load_text(self, text):
+        if load_library().vltr_radii_load_text(self._h, text.encode()) != VLTR_OK:
+            raise ValueError(last_error())
+
+    def get(self, res_name, atom_name):
+        return load_library().vltr_radii_get(self._h, res_name.encode(), atom_name.encode())
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.vltr_radii_free(self._h)
+            self._h = None
+
+
+class Structure:
+    """A parsed file: balls (n, 4) and atom records (dicts; none for XYZR)."""
+
+    def __init__(self, handle):
+        lib = load_library()
+        self._h = handle
+        self.format = {v: k for k, v in FORMATS.items()}[lib.vltr_structure_format(handle)]
+        n = lib.vltr_structure_num_balls(handle)
+        self.balls = _copy(lib.vltr_structure_balls(handle), n, np.dtype((np.float64, 4)))
+        self.balls = np.asarray(self.balls, dtype=np.float64).reshape(n, 4)
+        self.records = []
+        serial, res_seq = C.c_int32(), C.c_int32()
+        xyzb = (C.c_double * 4)()
+        for i in range(lib.vltr_structure_num_records(handle)):
+            rec = {f: lib.vltr_record_string(handle, i, k).decode() for k, f in enumerate(RECORD_FIELDS)}
+            lib.vltr_record_numbers(handle, i, C.byref(serial), C.byref(res_seq), xyzb)
+            rec.update(serial=serial.value, res_seq=res_seq.value, x=xyzb[0], y=xyzb[1], z=xyzb[2],
+                       b_factor=xyzb[3])
+            self.records.append(rec)
+
+    def _groups(self, fn, *args):
+        out = np.zeros(len(self.records), np.int32)
+        rc = fn(self._h, *args, out.ctypes.data_as(_i32p))
+        if rc != VLTR_OK:
+            raise ValueError(last_error())
+        return out
+
+    def chain_grouping(self):
+        return self._groups(load_library().vltr_grouping_chain)
+
+    def residue_grouping(self):
+        return self._groups(load_library().vltr_grouping_residue)
+
+    def selection_grouping(self, selections):
+        arr = (C.c_char_p * len(selections))(*[s.encode() for s in selections])
+        return self._groups(load_library().vltr_grouping_selections, arr, len(selections))
+
+    def select(self, selection):
+        """Boolean mask of the records a selection matches."""
+        out = np.zeros(len(self.records), np.uint8)
+        rc = load_library().vltr_selection_match(self._h, selection.encode(), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc != VLTR_OK:
+            raise ValueError(last_error())
+        return out.astype(bool)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.vltr_structure_free(self._h)
+            self._h = None
+
+
+def _options(exclude_heteroatoms, include_hydrogens, as_assembly):
+    return ParseOptions(int(bool(exclude_heteroatoms)), int(bool(include_hydrogens)), int(bool(as_assembly)))
+
+
+def parse_file(path, exclude_heteroatoms=False, include_hydrogens=False, as_assembly=False, radii=None):
+    """parse_file_with_records (src/input/mod.rs:300-334); OSError when the
+    file cannot be read or its format not detected."""
+    lib = load_library()
+    h = C.c_void_p()
+    opts = _options(exclude_heteroatoms, include_hydrogens, as_assembly)
+    rc = lib.vltr_parse_file(str(path).encode(), C.byref(opts), radii._h if radii is not None else None, C.byref(h))
+    if rc == VLTR_ERR_IO:
+        raise OSError(last_error())
+    check(rc, "vltr_parse_file")
+    return Structure(h)
+
+
+def parse_text(text, fmt="auto", exclude_heteroatoms=False, include_hydrogens=False, as_assembly=False,
+               radii=None):
+    lib = load_library()
+    h = C.c_void_p()
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    opts = _options(exclude_heteroatoms, include_hydrogens, as_assembly)
+    check(lib.vltr_parse_text(data, len(data), FORMATS[fmt], C.byref(opts),
+                              radii._h if radii is not None else None, C.byref(h)), "vltr_parse_text")
+    return Structure(h)
+
+
+def _take_string(ptr, n):
+    try:
+        return C.string_at(ptr, n).decode()
+    finally:
+        load_library().vltr_string_free(ptr)
+
+
+def from_file(path, probe, pbox=None, with_cell_vertices=False, selections=None, device=0, want_json=False):
+    """compute_tessellation_from_file (src/input/mod.rs:518-552): Result (and
+    the CLI's JSON text with want_json); OSError on every failure, as the
+    PyO3 binding raises (src/python.rs:310-321)."""
+    lib = load_library()
+    ctx = context(device)
+    box = make_pbox(pbox)
+    arr, ns = None, 0
+    if selections is not None:
+        arr = (C.c_char_p * max(len(selections), 1))(*[s.encode() for s in selections])
+        ns = len(selections)
+    h = C.c_void_p()
+    rc = lib.vltr_compute_tessellation_from_file(ctx, str(path).encode(), float(probe),
+                                                 C.byref(box) if box is not None else None,
+                                                 int(bool(with_cell_vertices)), arr, ns, C.byref(h))
+    if rc in (VLTR_ERR_IO, 1):
+        raise OSError(last_error())
+    check(rc, "vltr_compute_tessellation_from_file")
+    try:
+        res = Result(h)
+        if want_json:
+            txt, n = C.c_void_p(), C.c_size_t()
+            check(lib.vltr_result_to_json(h, C.byref(txt), C.byref(n)), "vltr_result_to_json")
+            res.json = _take_string(txt, n.value)
+        return res
+    finally:
+        lib.vltr_result_free(h)
+
+
+def run_json(balls, probe, pbox=None, groups=None, with_cell_vertices=False, device=0):
+    """The CLI's JSON document for a tessellation (src/main.rs:22-34)."""
+    lib = load_library()
+    ctx = context(device)
+    b = _balls_array(balls)
+    box = make_pbox(pbox)
+    g = None if groups is None else np.ascontiguousarray(np.asarray(groups, np.int32))
+    h = C.c_void_p()
+    check(lib.vltr_compute_tessellation(ctx, b.ctypes.data_as(_vp) if len(b) else None, len(b), float(probe),
+                                        C.byref(box) if box is not None else None,
+                                        g.ctypes.data_as(_i32p) if g is not None and len(g) else None,
+                                        len(g) if g is not None else 0, int(bool(with_cell_vertices)), C.byref(h)),
+          "vltr_compute_tessellation")
+    try:
+        txt, n = C.c_void_p(), C.c_size_t()
+        check(lib.vltr_result_to_json(h, C.byref(txt), C.byref(n)), "vltr_result_to_json")
+        return _take_string(txt, n.value)
+    finally:
+        lib.vltr_result_free(h)
+
+
+def pymol_graphics(balls, probe, groups=None, angle_step=0.5, object_name="contacts", device=0):
+    """GraphicsWriter::from_balls + write_pymol (src/graphics.rs:45-178)."""
+    lib = load_library()
+    ctx = context(device)
+    b = _balls_array(balls)
+    g = None if groups is None else np.ascontiguousarray(np.asarray(groups, np.int32))
+    txt, n = C.c_void_p(), C.c_size_t()
+    check(lib.vltr_pymol_graphics(ctx, b.ctypes.data_as(_vp) if len(b) else None, len(b), float(probe),
+                                  g.ctypes.data_as(_i32p) if g is not None and len(g) else None,
+                                  len(g) if g is not None else 0, float(angle_step), object_name.encode(),
+                                  C.byref(txt), C.byref(n)), "vltr_pymol_graphics")
+    return _take_string(txt, n.value)

@@ -129,9 +129,12 @@ def main():
         t1 = pyoracle.time_tessellation(balls, args.probe, threads=threads, reps=1)
         reps = max(1, int(args.cpu_seconds / max(t1, 1e-3)))
         tcpu = pyoracle.time_tessellation(balls, args.probe, threads=threads, reps=reps)
+        t_single = pyoracle.time_tessellation(balls, args.probe, threads=1, reps=1)
         cpu = {"value": args.n / tcpu, "unit": "balls/s", "cores": threads, "kind": "port",
                "sample": f"full compute_tessellation of the same {args.n}-ball input, {reps} run(s) "
-                         f"after one warm-up, C++ restatement (-O3, OpenMP over balls and pairs)"}
+                         f"after one warm-up, C++ restatement (-O3, OpenMP over balls and pairs)",
+               "single_thread_balls_per_s": round(args.n / t_single, 1),
+               "calibration": cpu_calibration(pyoracle, threads)}
 
     out = {
         "metric": METRIC,
@@ -168,6 +171,7 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms": ph_full,
         "counts": counts,
+        "grid_build": grid_build(ph_full["grid"], counts),
         "contacts_only": {"balls_per_s": round(args.n / (ms_co / 1e3), 1), "ms_per_step": round(ms_co, 4),
                           "phases_ms": ph_co, "contacts": counts_co["contacts"], "pairs": counts_co["pairs"],
                           "groups": "1 for x < 0, else 0 (inter-group contacts only)"},
@@ -181,6 +185,44 @@ def main():
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
     print(json.dumps(out), flush=True)
+
+
+# The reference's published times (README.md:375-381, Apple M4, 10 cores,
+# Rust/Rayon): balls_2zsk 12 ms, balls_3dlb 30 ms, 5.9x over one thread.
+PUBLISHED_M4_MS = {"balls_2zsk": 12.0, "balls_3dlb": 30.0}
+
+
+def cpu_calibration(pyoracle, threads):
+    """The oracle on the reference's benchmark inputs, all cores and one core,
+    beside the published numbers, to place this host's CPU path."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import load_xyzr
+
+    out = {}
+    for name, ms in PUBLISHED_M4_MS.items():
+        b = load_xyzr(name + ".xyzr")
+        pyoracle.time_tessellation(b, 1.4, threads=threads, reps=1)
+        t_all = pyoracle.time_tessellation(b, 1.4, threads=threads, reps=5)
+        t_one = pyoracle.time_tessellation(b, 1.4, threads=1, reps=2)
+        out[name] = {"balls": len(b), "ms_all_cores": round(t_all * 1e3, 2), "ms_one_core": round(t_one * 1e3, 2),
+                     "published_rayon_m4_10_cores_ms": ms}
+    return out
+
+
+# Algorithmic bytes of the cell-list build: each ball's sphere read (32 B),
+# its sorted copy (32 B) and permutation entry (4 B) written, one 4-byte
+# offset per cell-hash bucket.
+GRID_BYTES_PER_BALL = 68.0
+GRID_BYTES_PER_BUCKET = 4.0
+
+
+def grid_build(grid_ms, counts):
+    """HBM roofline of the grid phase (prepare, keys, counting sort, scatter)
+    from its HIP-event time."""
+    nbytes = GRID_BYTES_PER_BALL * counts["spheres"] + GRID_BYTES_PER_BUCKET * counts["buckets"]
+    gbs = nbytes / (grid_ms * 1e-3) / 1e9
+    return {"ms": round(grid_ms, 4), "bytes": int(nbytes), "achieved_GBs": round(gbs, 1), "peak_GBs": HBM_PEAK_GBS,
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "buckets": counts["buckets"]}
 
 
 def batched_leg(args, lib, ctx):
@@ -218,6 +260,7 @@ def batched_leg(args, lib, ctx):
     _hip.synchronize()
     t_batched = (time.perf_counter() - t0) / reps
     counts = _native.last_counts()
+    grid = grid_build(_native.last_timings()["grid"], counts)
 
     base = dbuf.ptr.value
     for k in range(min(8, len(sizes))):  # warm the per-call path
@@ -234,7 +277,7 @@ def batched_leg(args, lib, ctx):
     return {"structures": len(sizes), "balls": total, "pairs": counts["pairs"], "contacts": counts["contacts"],
             "batched_s": round(t_batched, 4), "batched_balls_per_s": round(total / t_batched, 1),
             "plain_s": round(t_plain, 4), "plain_balls_per_s": round(total / t_plain, 1),
-            "batched_over_plain": round(t_plain / t_batched, 3),
+            "batched_over_plain": round(t_plain / t_batched, 3), "grid_build": grid,
             "note": "full compute_tessellation of every structure, balls resident in HBM; plain = one "
                     "vltr_compute_tessellation_device call per structure, batched = one "
                     "vltr_compute_tessellation_batched_device call"}
@@ -266,7 +309,7 @@ def adjoint_leg(args, lib, ctx):
         _native.check(lib.vltr_compute_tessellation_device(ctx, db.ptr, n, args.probe, None, 0, 0), "forward")
 
     def fwd_adj():
-        _native.check(lib.vltr_energy_gradient_device(ctx, db.ptr, n, args.probe, dt.ptr, wp, 1.0, 1.0, dg.ptr,
+        _native.check(lib.vltr_energy_gradient_device(ctx, db.ptr, n, args.probe, None, dt.ptr, wp, 1.0, 1.0, dg.ptr,
                                                       C.byref(e)), "forward + adjoint")
 
     def timeit(fn, reps=10):

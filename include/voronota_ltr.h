@@ -100,18 +100,18 @@ void vltr_context_destroy(vltr_context* ctx);
 const char* vltr_last_error(void);
 
 /* Per-phase device time of the last pipeline run on ctx, in milliseconds,
- * measured with HIP events on the context stream. Order: prepare+grid,
- * neighbors, pairs, clip (classification and contour clipping of every
- * candidate pair), slow (the pairs of balls with more than 64 neighbors and
- * contours beyond 16 points),
- * cells+compaction, total. Returns the count written. */
+ * measured with HIP events on the context stream. Order: grid (prepare,
+ * cell keys, counting sort and scatter of the spheres), neighbors, pairs,
+ * clip (classification and contour clipping of every candidate pair), slow
+ * (the pairs of balls with more than 64 neighbors and contours beyond 8
+ * points), cells+compaction, total. Returns the count written. */
 int vltr_context_last_timings(const vltr_context* ctx, double* ms, int max_entries);
 
 /* Work counts of the last pipeline run on ctx: out[0] balls, [1] searched
  * spheres (27 n when periodic), [2] candidate pairs clipped, [3] contacts,
  * [4] cells, [5] pairs clipped outside the main clip kernel (first ball with
- * more than 64 neighbors, or contour beyond 16 points), [6] neighbor slots
- * per ball. Returns the count written. */
+ * more than 64 neighbors, or contour beyond 8 points), [6] neighbor slots
+ * per ball, [7] cell-hash buckets. Returns the count written. */
 int vltr_context_last_counts(const vltr_context* ctx, uint64_t* out, int max_entries);
 
 /* ------------------------------------------------------- periodic box --- */
@@ -190,16 +190,19 @@ int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64
  * + volume_weight * sum_cells volume of compute_tessellation
  * (tessellation.rs:37-48) and its gradient with respect to the ball centres
  * (radii and probe fixed), holding the forward pass's contour combinatorics
- * fixed. types[i] in 0..3, w16 = row-major 4x4 W. Non-periodic, no groups.
+ * fixed. types[i] in 0..3, w16 = row-major 4x4 W; periodic_box may be NULL
+ * (with a box the contact sum runs over the deduplicated contact list, as
+ * compute_tessellation reports it). No groups.
  * grad receives 3 n doubles (x, y, z per ball). No reference counterpart:
  * the reference has no derivatives. */
-int vltr_energy_gradient(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe, const int32_t* types,
-                         const double* w16, double sas_weight, double volume_weight, double* energy, double* grad);
+int vltr_energy_gradient(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe,
+                         const vltr_periodic_box* periodic_box, const int32_t* types, const double* w16,
+                         double sas_weight, double volume_weight, double* energy, double* grad);
 /* Same on device memory: d_balls, d_types, d_grad are device pointers; w16 and
  * energy are host memory. */
 int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
-                                const int32_t* d_types, const double* w16, double sas_weight, double volume_weight,
-                                double* d_grad, double* energy);
+                                const vltr_periodic_box* periodic_box, const int32_t* d_types, const double* w16,
+                                double sas_weight, double volume_weight, double* d_grad, double* energy);
 
 /* ----------------------------------------------------- solvent spheres --- */
 

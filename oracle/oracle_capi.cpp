@@ -271,26 +271,30 @@ int oracle_count_flops(const double* balls, size_t n, double probe, const double
   return 0;
 }
 
-// Energy of the differentiation workload (non-periodic, no groups).
-int oracle_energy(const double* balls, size_t n, double probe, const int32_t* types, const double* W16, int threads,
-                  double* energy) {
+// Energy of the differentiation workload (no groups; pbox9 may be NULL).
+int oracle_energy(const double* balls, size_t n, double probe, const double* pbox9, const int32_t* types,
+                  const double* W16, int threads, double* energy) {
   auto b = load_balls<double>(balls, n);
-  auto t = compute_tessellation<double>(b, probe, nullptr, nullptr, 0, false, threads);
+  PeriodicBox<double> pb;
+  if (pbox9) pb = load_pbox<double>(pbox9);
+  auto t = compute_tessellation<double>(b, probe, pbox9 ? &pb : nullptr, nullptr, 0, false, threads);
   *energy = energy_of(t, types, W16);
   return 0;
 }
 
 // Forward-mode derivative of the energy along coordinate `axis` (0,1,2) of
 // ball `ball_id`, with the contour combinatorics of the unperturbed input.
-int oracle_energy_dual(const double* balls, size_t n, double probe, const int32_t* types, const double* W16,
-                       size_t ball_id, int axis, double* energy, double* denergy) {
+int oracle_energy_dual(const double* balls, size_t n, double probe, const double* pbox9, const int32_t* types,
+                       const double* W16, size_t ball_id, int axis, double* energy, double* denergy) {
   auto b = load_balls<Dual>(balls, n);
+  PeriodicBox<Dual> pb;
+  if (pbox9) pb = load_pbox<Dual>(pbox9);
   if (ball_id < n) {
     if (axis == 0) b[ball_id].x.d = 1.0;
     if (axis == 1) b[ball_id].y.d = 1.0;
     if (axis == 2) b[ball_id].z.d = 1.0;
   }
-  auto t = compute_tessellation<Dual>(b, Dual(probe), nullptr, nullptr, 0, false, 1);
+  auto t = compute_tessellation<Dual>(b, Dual(probe), pbox9 ? &pb : nullptr, nullptr, 0, false, 1);
   const Dual e = energy_of(t, types, W16);
   *energy = e.v;
   *denergy = e.d;

@@ -49,8 +49,8 @@ def _load():
     L.oracle_pair_records.argtypes = [_f64p, sz, C.c_double, _f64p, _i32p, sz, C.c_int, C.c_int, _f64p,
                                       C.POINTER(C.c_uint32)]
     L.oracle_count_flops.argtypes = [_f64p, sz, C.c_double, _f64p, _i32p, sz, C.c_int, _u64p, _u64p]
-    L.oracle_energy.argtypes = [_f64p, sz, C.c_double, _i32p, _f64p, C.c_int, _f64p]
-    L.oracle_energy_dual.argtypes = [_f64p, sz, C.c_double, _i32p, _f64p, sz, C.c_int, _f64p, _f64p]
+    L.oracle_energy.argtypes = [_f64p, sz, C.c_double, _f64p, _i32p, _f64p, C.c_int, _f64p]
+    L.oracle_energy_dual.argtypes = [_f64p, sz, C.c_double, _f64p, _i32p, _f64p, sz, C.c_int, _f64p, _f64p]
     L.oracle_energy_terms.argtypes = [_f64p, sz, C.c_double, _i32p, _f64p, C.c_int, C.POINTER(_vp)]
     L.oracle_updateable_new.argtypes = [C.c_int, C.c_int]
     L.oracle_updateable_new.restype = _vp
@@ -225,25 +225,28 @@ def count_flops(balls, probe, pbox=None, groups=None, mode=0):
     return fl.value, tp.value
 
 
-def energy(balls, probe, types, W, threads=0) -> float:
+def energy(balls, probe, types, W, threads=0, pbox=None) -> float:
     L = _load()
     b = _balls(balls)
+    pb = _pbox(pbox)
     t = np.ascontiguousarray(np.asarray(types, np.int32))
     w = np.ascontiguousarray(np.asarray(W, np.float64).reshape(16))
     e = C.c_double()
-    L.oracle_energy(_ptr(b, _f64p), len(b), float(probe), _ptr(t, _i32p), _ptr(w, _f64p), int(threads), C.byref(e))
+    L.oracle_energy(_ptr(b, _f64p), len(b), float(probe), _ptr(pb, _f64p), _ptr(t, _i32p), _ptr(w, _f64p),
+                    int(threads), C.byref(e))
     return e.value
 
 
-def energy_dual(balls, probe, types, W, ball_id, axis):
+def energy_dual(balls, probe, types, W, ball_id, axis, pbox=None):
     L = _load()
     b = _balls(balls)
+    pb = _pbox(pbox)
     t = np.ascontiguousarray(np.asarray(types, np.int32))
     w = np.ascontiguousarray(np.asarray(W, np.float64).reshape(16))
     e = C.c_double()
     de = C.c_double()
-    L.oracle_energy_dual(_ptr(b, _f64p), len(b), float(probe), _ptr(t, _i32p), _ptr(w, _f64p), int(ball_id),
-                         int(axis), C.byref(e), C.byref(de))
+    L.oracle_energy_dual(_ptr(b, _f64p), len(b), float(probe), _ptr(pb, _f64p), _ptr(t, _i32p), _ptr(w, _f64p),
+                         int(ball_id), int(axis), C.byref(e), C.byref(de))
     return e.value, de.value
 
 

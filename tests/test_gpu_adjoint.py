@@ -73,6 +73,71 @@ def test_gradient_matches_oracle_dual(oracle):
     print(f"[gpu] adjoint vs dual: worst {worst:.2e}", flush=True)
 
 
+def _self_image_box():
+    """Four balls at seeded random positions in a 5.3 x 5.9 x 6.1 box: every
+    ball touches its own images (4 self-contacts), generic geometry."""
+    rng = np.random.default_rng(7)
+    L = np.array([5.3, 5.9, 6.1])
+    balls = np.c_[rng.uniform(0, 1, (4, 3)) * L, rng.choice([1.52, 1.55, 1.7, 1.8], 4)]
+    return balls, [L[0], 0, 0, 0, L[1], 0, 0, 0, L[2]]
+
+
+@pytest.mark.parametrize("case", ["rsa_box", "self_images"])
+def test_periodic_gradient_matches_oracle_dual(oracle, case):
+    """Periodic box: contacts through images, the deduplicated contact list
+    (one W * area term per canonical pair among the contacts through images,
+    the first in pair order), cells of canonical balls only; energy and
+    gradient against the oracle's dual numbers."""
+    from conftest import corners_box
+    from voronota_ltr import _native, workload
+
+    if case == "rsa_box":
+        L = 30.0
+        balls, _ = workload.rsa_box(1500, L, seed=8)
+        box = corners_box((0, 0, 0), (L, L, L))
+        picks = np.random.default_rng(9).choice(len(balls), 40, replace=False)
+    else:
+        balls, box = _self_image_box()
+        picks = np.arange(len(balls))
+    types = np.arange(len(balls), dtype=np.int32) % 4
+    W = workload.type_matrix(44)
+    print(f"\n[gpu] periodic adjoint {case}", flush=True)
+    e, g = _native.energy_gradient(balls, 1.4, types, W, pbox=box)
+    assert e == pytest.approx(oracle.energy(balls, 1.4, types, W, pbox=box), rel=1e-10)
+    worst = 0.0
+    for ball in picks:
+        for axis in range(3):
+            _, de = oracle.energy_dual(balls, 1.4, types, W, int(ball), axis, pbox=box)
+            err = abs(g[ball, axis] - de) / max(1.0, abs(de))
+            worst = max(worst, err)
+            assert err < 1e-7, (int(ball), axis, g[ball, axis], de)
+    print(f"[gpu] periodic adjoint {case}: worst {worst:.2e}", flush=True)
+
+
+def test_periodic_degenerate_ring17():
+    """Ring-17 in the reference's small periodic box (self-contacts through
+    several images, exact mirror symmetry, coincident vertices): the energy
+    equals the oracle's and the gradient stays finite; its y and x components
+    of the top ball vanish by the mirror symmetry. (The oracle's dual follows
+    one branch of the reference's arithmetic at these non-differentiable
+    points, so it is not a reference here.)"""
+    from conftest import RING17, corners_box
+    from voronota_ltr import _native, workload
+
+    import pyoracle
+
+    box = corners_box((-1.6, -1.6, -0.6), (1.6, 1.6, 3.1))
+    types = np.arange(17, dtype=np.int32) % 4
+    W = workload.type_matrix(44)
+    e, g = _native.energy_gradient(RING17, 1.0, types, np.zeros((4, 4)), pbox=box)
+    assert e == pytest.approx(pyoracle.energy(RING17, 1.0, types, np.zeros((4, 4)), pbox=box), rel=1e-10)
+    assert np.all(np.isfinite(g))
+    assert abs(g[0, 0]) < 1e-9 and abs(g[0, 1]) < 1e-9
+    e, g = _native.energy_gradient(RING17, 1.0, types, W, pbox=box)
+    assert e == pytest.approx(pyoracle.energy(RING17, 1.0, types, W, pbox=box), rel=1e-10)
+    assert np.all(np.isfinite(g))
+
+
 @pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.0, 1.0)])
 def test_sas_and_volume_gradients(alpha, beta):
     from voronota_ltr import _native

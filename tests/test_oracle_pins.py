@@ -230,6 +230,30 @@ def test_dual_matches_central_differences(oracle):
             assert de == pytest.approx(fd, rel=1e-6, abs=1e-6)
 
 
+def test_periodic_dual_matches_central_differences(oracle):
+    """The same in a periodic box (contacts through images, the deduplicated
+    contact list, cells of canonical balls only): the dual derivative of the
+    reference computation equals central differences of the energy."""
+    from conftest import corners_box
+    from voronota_ltr import workload
+
+    L = 13.0
+    balls, _ = workload.rsa_box(120, L, seed=5)
+    box = corners_box((0, 0, 0), (L, L, L))
+    types = np.arange(len(balls), dtype=np.int32) % 4
+    W = np.array([[0.3, -0.2, 0.5, 0.1], [-0.2, 0.7, -0.4, 0.2], [0.5, -0.4, -0.6, 0.9], [0.1, 0.2, 0.9, -0.3]])
+    h = 1e-6
+    for ball in (0, 7, 50, 119):
+        for axis in range(3):
+            e0, de = oracle.energy_dual(balls, 1.4, types, W, ball, axis, pbox=box)
+            bp, bm = balls.copy(), balls.copy()
+            bp[ball, axis] += h
+            bm[ball, axis] -= h
+            fd = (oracle.energy(bp, 1.4, types, W, pbox=box) - oracle.energy(bm, 1.4, types, W, pbox=box)) / (2 * h)
+            assert e0 == pytest.approx(oracle.energy(balls, 1.4, types, W, pbox=box), rel=1e-14)
+            assert de == pytest.approx(fd, rel=1e-6, abs=1e-6), (ball, axis)
+
+
 def test_counted_flops_are_deterministic(oracle):
     balls = _rsa_cluster(500, 13)
     f1 = oracle.count_flops(balls, 1.4, mode=0)

@@ -36,15 +36,21 @@ struct V3acc {
   VLTR_HD V3 v() const { return V3{x, y, z}; }
 };
 
+// At the measure-zero points where a derivative below is unbounded (a zero
+// vector normalised, a singular 3x3 system: tangent circle points, vertices
+// where more than three planes meet) the contribution is taken as zero.
+
 // u = v / |v|: returns v_bar for u_bar.
 VLTR_HD V3 vjp_unit(V3 v, V3 ubar) {
   const double n = norm(v);
+  if (!(n > 0.0)) return V3{0.0, 0.0, 0.0};
   const V3 u = v / n;
   return (ubar - u * dot(u, ubar)) / n;
 }
 // s = |v|
 VLTR_HD V3 vjp_norm(V3 v, double sbar) {
   const double n = norm(v);
+  if (!(n > 0.0)) return V3{0.0, 0.0, 0.0};
   return v * (sbar / n);
 }
 // acos(clamp(c)): d/dc (0 where the clamp is active)
@@ -84,6 +90,10 @@ VLTR_HD void vjp_directed_angle(V3 o, V3 p, V3 q, double sgn, double thbar, V3* 
   const V3 a = p - o, b = q - o;
   const V3 v1 = unit(a), v2 = unit(b);
   const double cbar = thbar * sgn * dacos(dot(v1, v2));
+  if (!(cbar != 0.0 && std::isfinite(cbar))) {
+    *obar = *pbar = *qbar = V3{0.0, 0.0, 0.0};
+    return;
+  }
   const V3 abar = vjp_unit(a, v2 * cbar);
   const V3 bbar = vjp_unit(b, v1 * cbar);
   *pbar = abar;
@@ -99,6 +109,10 @@ VLTR_HD void vjp_min_dihedral(V3 o, V3 p, V3 b1, V3 b2, double mbar, V3* obar, V
   const V3 d2 = w2 - oa * dot(oa, w2);
   const V3 u1 = unit(d1), u2 = unit(d2);
   const double cbar = mbar * dacos(dot(u1, u2));
+  if (!(cbar != 0.0 && std::isfinite(cbar))) {
+    *obar = *pbar = *b1bar = *b2bar = V3{0.0, 0.0, 0.0};
+    return;
+  }
   const V3 d1bar = vjp_unit(d1, u2 * cbar);
   const V3 d2bar = vjp_unit(d2, u1 * cbar);
   // d = w - oa (oa . w): w_bar = d_bar - oa (oa . d_bar); oa_bar = -((oa . w) d_bar + (oa . d_bar) w)
@@ -148,7 +162,9 @@ VLTR_HD V3 triple_point(const RelPlane& p1, const RelPlane& p2, const RelPlane& 
 VLTR_HD V3 solve_transposed(V3 r1, V3 r2, V3 r3, V3 ybar) {
   const V3 c23 = cross(r2, r3), c31 = cross(r3, r1), c12 = cross(r1, r2);
   const double det = dot(r1, c23);
-  return V3{dot(c23, ybar), dot(c31, ybar), dot(c12, ybar)} / det;
+  const V3 lam = V3{dot(c23, ybar), dot(c31, ybar), dot(c12, ybar)} / det;
+  if (!(std::isfinite(lam.x) && std::isfinite(lam.y) && std::isfinite(lam.z))) return V3{0.0, 0.0, 0.0};
+  return lam;
 }
 
 // ------------------------------------------------------ group backward ----
@@ -175,6 +191,12 @@ VLTR_HD V3 gsum3(G& g, V3 v) {
 //   w * area + sum_{i = a, b} (g_i * solid_i + beta * pyramid_volume_i),
 // added to acc (acc.add(sphere id, V3)). g_a, g_b: cell seeds; sgn_a, sgn_b:
 // signs of the forward solid angles (they fix the pyramid volume signs).
+// Adds v to sphere id unless it is not finite (a non-differentiable point).
+template <class Acc>
+VLTR_HD void add_finite(Acc& acc, int32_t id, V3 v) {
+  if (std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z)) acc.add(id, v);
+}
+
 template <class G, class Acc>
 VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t a_id, int32_t b_id,
                                    const ContactTopo& topo, double w, double g_a, double g_b, double sgn_a,
@@ -358,15 +380,15 @@ VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t 
         const V3 gb = (e - y) * lam.x, gc = (ec - y) * lam.y;
         cbar[1] = cbar[1] + gb;
         cbar[0] = cbar[0] - gb - gc;
-        acc.add(c, gc);
+        add_finite(acc, c, gc);
       } else {
         const V3 el = load_sph(sph4, l).c - A.c, er = load_sph(sph4, rr).c - A.c;
         const V3 lam = solve_transposed(e, el, er, xb);
         const V3 gb = (e - y) * lam.x, gl = (el - y) * lam.y, gr = (er - y) * lam.z;
         cbar[1] = cbar[1] + gb;
         cbar[0] = cbar[0] - gb - gl - gr;
-        acc.add(l, gl);
-        acc.add(rr, gr);
+        add_finite(acc, l, gl);
+        add_finite(acc, rr, gr);
       }
     }
   }
@@ -389,8 +411,8 @@ VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t 
   const V3 ga_tot = gsum3(g, cbar[0]);
   const V3 gb_tot = gsum3(g, cbar[1]);
   if (j == 0) {
-    acc.add(a_id, ga_tot);
-    acc.add(b_id, gb_tot);
+    add_finite(acc, a_id, ga_tot);
+    add_finite(acc, b_id, gb_tot);
   }
 }
 

@@ -5,6 +5,14 @@
 // with FP64 atomics (image ids fold onto their ball). The same kernel sums
 // the energy: contact terms by the groups, cell terms by all lanes, one
 // partial per block.
+//
+// Periodic box: a pair whose second end is an image adds nothing to that
+// ball's cell (tessellation.rs:624-631), so its second-side cell terms are
+// zero; and among the contacts through an image the list keeps only the
+// first of each canonical pair (tessellation.rs:556-601; contacts between two
+// canonical balls are always kept), so the W * area term of such a record
+// counts only if it is that first one, found through a hash table of
+// canonical pairs filled by k_dedup_insert (smallest pair index per key).
 #include "adjoint.h"
 #include "vltr_internal.h"
 #include "wave_group.h"
@@ -31,6 +39,62 @@ struct DevAcc {
   }
 };
 
+// Open-addressing table of canonical pairs: key lo * n + hi + 1 (0 = empty),
+// first = smallest pair index with that key. Sized to at least twice the
+// candidate pairs, so every probe sequence reaches its key or an empty slot.
+struct DedupTable {
+  unsigned long long* keys;
+  uint32_t* first;
+  uint64_t mask;
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long pair_key(const Pipeline& p, int64_t t) {
+  const int64_t ca = p.pair_a[t] % p.n, cb = p.pair_b[t] % p.n;
+  const int64_t lo = ca < cb ? ca : cb, hi = ca < cb ? cb : ca;
+  return static_cast<unsigned long long>(lo) * static_cast<unsigned long long>(p.n) + hi + 1ull;
+}
+
+__global__ void k_dedup_clear(DedupTable T) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i <= T.mask;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    T.keys[i] = 0ull;
+    T.first[i] = 0xffffffffu;
+  }
+}
+
+__global__ void k_dedup_insert(Pipeline p, DedupTable T) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= p.n_pairs) return;
+  const PairRec rec = p.rec[t];
+  if (!(rec.flags & kRecValid) || !(rec.area > 0.0) || p.pair_b[t] < p.n) return;
+  const unsigned long long key = pair_key(p, t);
+  uint64_t slot = mix64(key) & T.mask;
+  for (;;) {
+    const unsigned long long prev = atomicCAS(T.keys + slot, 0ull, key);
+    if (prev == 0ull || prev == key) {
+      atomicMin(T.first + slot, static_cast<uint32_t>(t));
+      return;
+    }
+    slot = (slot + 1) & T.mask;
+  }
+}
+
+__device__ bool dedup_first(const Pipeline& p, const DedupTable& T, int64_t t) {
+  const unsigned long long key = pair_key(p, t);
+  uint64_t slot = mix64(key) & T.mask;
+  while (T.keys[slot] != key) slot = (slot + 1) & T.mask;
+  return T.first[slot] == static_cast<uint32_t>(t);
+}
+
 __global__ void k_zero(double* a, int64_t n) {
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -38,7 +102,7 @@ __global__ void k_zero(double* a, int64_t n) {
 }
 
 __global__ __launch_bounds__(kAdjBlock) void k_adjoint(Pipeline p, const int32_t* types, W16 W, double* grad,
-                                                       double* part) {
+                                                       double* part, DedupTable T) {
   __shared__ AdjLds lds[kAdjGroups];
   __shared__ double red[kAdjBlock / 64];
   const int gl = threadIdx.x / kGroup;
@@ -55,10 +119,13 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(Pipeline p, const int32_t
     if (topo.m < 0) continue;
     const int32_t a = p.pair_a[t], b = p.pair_b[t];
     const int64_t ca = a % p.n, cb = b % p.n;
-    const double w = W.w[4 * types[ca] + types[cb]];
+    const bool counted = !p.periodic || b < p.n || dedup_first(p, T, t);
+    const double w = counted ? W.w[4 * types[ca] + types[cb]] : 0.0;
+    const bool b_cell = b < p.n;  // an image end has no cell share
     if (ln == 0) e += w * rec.area;
-    group_contact_adjoint(G, &lds[gl], sph4, a, b, topo, w, p.seed[ca], p.seed[cb], rec.solid_a < 0.0 ? -1.0 : 1.0,
-                          rec.solid_b < 0.0 ? -1.0 : 1.0, acc, p.volume_weight);
+    group_contact_adjoint(G, &lds[gl], sph4, a, b, topo, w, p.seed[ca], b_cell ? p.seed[cb] : 0.0,
+                          rec.solid_a < 0.0 ? -1.0 : 1.0, b_cell ? (rec.solid_b < 0.0 ? -1.0 : 1.0) : 0.0, acc,
+                          p.volume_weight);
   }
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < p.n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -79,12 +146,24 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(Pipeline p, const int32_t
 
 int64_t adjoint_blocks() { return kAdjBlocks; }
 
+uint64_t dedup_slots(int64_t n_pairs) {
+  uint64_t h = 1024;
+  while (h < 2 * static_cast<uint64_t>(n_pairs > 0 ? n_pairs : 1)) h <<= 1;
+  return h;
+}
+
 hipError_t launch_adjoint(const Pipeline& p, const int32_t* types, const double* w16, double* grad, double* part,
-                          hipStream_t s) {
+                          unsigned long long* dedup_keys, uint32_t* dedup_first, hipStream_t s) {
   W16 W;
   for (int k = 0; k < 16; ++k) W.w[k] = w16[k];
+  DedupTable T{dedup_keys, dedup_first, dedup_slots(p.n_pairs) - 1};
+  if (p.periodic) {
+    hipLaunchKernelGGL(k_dedup_clear, dim3(1024), dim3(256), 0, s, T);
+    if (p.n_pairs > 0)
+      hipLaunchKernelGGL(k_dedup_insert, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, T);
+  }
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, grad, 3 * p.n);
-  hipLaunchKernelGGL(k_adjoint, dim3(kAdjBlocks), dim3(kAdjBlock), 0, s, p, types, W, grad, part);
+  hipLaunchKernelGGL(k_adjoint, dim3(kAdjBlocks), dim3(kAdjBlock), 0, s, p, types, W, grad, part, T);
   return hipGetLastError();
 }
 

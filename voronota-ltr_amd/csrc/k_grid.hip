@@ -526,8 +526,12 @@ hipError_t launch_grid(const Pipeline& p, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_neighbors(const Pipeline& p, hipStream_t s) {
+hipError_t launch_scatter(const Pipeline& p, hipStream_t s) {
   if (p.n_pop > 0) hipLaunchKernelGGL(k_scatter, dim3(blocks_for(p.n_pop, 256)), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_neighbors(const Pipeline& p, hipStream_t s) {
   if (p.n > 0) {
     hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.n, kNbWaves)), dim3(kNbBlock), 0, s, p);
     const int64_t g = blocks_for(p.n, 64) < kBigGrid ? blocks_for(p.n, 64) : kBigGrid;

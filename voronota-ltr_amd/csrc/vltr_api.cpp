@@ -109,14 +109,15 @@ struct vltr_context {
   DevBuf balls, groups, sph, seg_of, segs, bkey, brank, bcnt, sorted_id, sorted_sph, small, nb, nb_cnt, excluded,
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
       in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad, net, net_out, solv_cnt, solv_comb, solv_parent,
-      solv_sih, solv_ioff, solv_cur, solv_img, nb_big;
+      solv_sih, solv_ioff, solv_cur, solv_img, nb_big,
+      dedup_keys, dedup_first;
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
   size_t last_n = 0;
   size_t last_structs = 0;  // structures of the last batched run (0: not batched)
   int last_nb_cap = 0;
-  uint64_t counts[7] = {};
+  uint64_t counts[8] = {};
 };
 
 struct vltr_solvent_result {
@@ -287,6 +288,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     VLTR_HIP(launch_prepare(p, s));
     VLTR_HIP(launch_grid(p, s));
     VLTR_HIP(launch_scan_u32(p.bucket_cnt, p.n_buckets, ctx->scan_tmp.as<uint32_t>(), s));
+    VLTR_HIP(launch_scatter(p, s));
     VLTR_HIP(hipEventRecord(ctx->ev[1], s));
     VLTR_HIP(launch_neighbors(p, s));
     VLTR_HIP(launch_scan_u32(p.pair_cnt, n, ctx->scan_tmp.as<uint32_t>(), s));
@@ -375,6 +377,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   ctx->counts[4] = *n_cells;
   ctx->counts[5] = static_cast<uint64_t>(host_words[6]) + host_words[7];
   ctx->counts[6] = static_cast<uint64_t>(p.nb_cap);
+  ctx->counts[7] = static_cast<uint64_t>(p.n_buckets);
   float ms = 0.0f;
   for (int k = 0; k < kPhases - 1; ++k) {
     if (hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess) ctx->timings[k] = ms;
@@ -902,8 +905,8 @@ void vltr_solvent_free(vltr_solvent_result* r) { delete r; }
 
 // Energy and its gradient (forward + adjoint) on device-resident balls.
 int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
-                                const int32_t* d_types, const double* w16, double sas_weight, double volume_weight,
-                                double* d_grad, double* energy) {
+                                const vltr_periodic_box* periodic_box, const int32_t* d_types, const double* w16,
+                                double sas_weight, double volume_weight, double* d_grad, double* energy) {
   if (!ctx || !w16 || !energy || (n > 0 && (!d_balls || !d_types || !d_grad)))
     return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
   *energy = 0.0;
@@ -923,10 +926,21 @@ int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, siz
   double* part = ctx->energy_part.as<double>();
   std::vector<double> w(w16, w16 + 16);
   ex.post_cells = [=](const Pipeline& p, hipStream_t st) -> hipError_t {
-    return launch_adjoint(p, d_types, w.data(), d_grad, part, st);
+    unsigned long long* keys = nullptr;
+    uint32_t* first = nullptr;
+    if (p.periodic) {
+      const uint64_t slots = dedup_slots(p.n_pairs);
+      hipError_t e = ctx->dedup_keys.ensure(slots * sizeof(unsigned long long));
+      if (e == hipSuccess) e = ctx->dedup_first.ensure(slots * sizeof(uint32_t));
+      if (e != hipSuccess) return e;
+      keys = ctx->dedup_keys.as<unsigned long long>();
+      first = ctx->dedup_first.as<uint32_t>();
+    }
+    return launch_adjoint(p, d_types, w.data(), d_grad, part, keys, first, st);
   };
   size_t nc = 0, ncell = 0;
-  const int rc = run_pipeline(ctx, static_cast<int64_t>(n), probe, nullptr, nullptr, 0, 0, &nc, &ncell, nullptr, 0, &ex);
+  const int rc =
+      run_pipeline(ctx, static_cast<int64_t>(n), probe, periodic_box, nullptr, 0, 0, &nc, &ncell, nullptr, 0, &ex);
   if (rc != VLTR_OK) return rc;
   std::vector<double> parts(static_cast<size_t>(adjoint_blocks()));
   VLTR_HIP(hipMemcpy(parts.data(), part, parts.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -938,8 +952,9 @@ int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, siz
   return VLTR_OK;
 }
 
-int vltr_energy_gradient(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe, const int32_t* types,
-                         const double* w16, double sas_weight, double volume_weight, double* energy, double* grad) {
+int vltr_energy_gradient(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe,
+                         const vltr_periodic_box* periodic_box, const int32_t* types, const double* w16,
+                         double sas_weight, double volume_weight, double* energy, double* grad) {
   if (!ctx || !w16 || !energy || (n > 0 && (!balls || !types || !grad)))
     return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
   *energy = 0.0;
@@ -954,8 +969,8 @@ int vltr_energy_gradient(vltr_context* ctx, const vltr_ball* balls, size_t n, do
   VLTR_HIP(ctx->grad.ensure(3 * n * sizeof(double)));
   VLTR_HIP(hipMemcpyAsync(db.ptr, balls, n * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
   VLTR_HIP(hipMemcpyAsync(ctx->types.ptr, types, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  int rc = vltr_energy_gradient_device(ctx, db.as<vltr_ball>(), n, probe, ctx->types.as<int32_t>(), w16, sas_weight,
-                                       volume_weight, ctx->grad.as<double>(), energy);
+  int rc = vltr_energy_gradient_device(ctx, db.as<vltr_ball>(), n, probe, periodic_box, ctx->types.as<int32_t>(), w16,
+                                       sas_weight, volume_weight, ctx->grad.as<double>(), energy);
   if (rc == VLTR_OK) {
     if (hipMemcpy(grad, ctx->grad.ptr, 3 * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
       rc = fail(VLTR_ERR_DEVICE, "gradient download");
@@ -1025,7 +1040,7 @@ void vltr_context_destroy(vltr_context* ctx) {
                     &ctx->in_cnt, &ctx->incoming, &ctx->seg_out, &ctx->topo, &ctx->seed, &ctx->energy_part,
                     &ctx->types, &ctx->grad, &ctx->net, &ctx->net_out, &ctx->solv_cnt, &ctx->solv_comb,
                     &ctx->solv_parent, &ctx->solv_sih, &ctx->solv_ioff, &ctx->solv_cur, &ctx->solv_img,
-                    &ctx->nb_big})
+                    &ctx->nb_big, &ctx->dedup_keys, &ctx->dedup_first})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)
@@ -1043,7 +1058,7 @@ int vltr_context_last_timings(const vltr_context* ctx, double* ms, int max_entri
 
 int vltr_context_last_counts(const vltr_context* ctx, uint64_t* out, int max_entries) {
   if (!ctx || !out) return 0;
-  const int k = std::min(max_entries, 7);
+  const int k = std::min(max_entries, 8);
   for (int i = 0; i < k; ++i) out[i] = ctx->counts[i];
   return k;
 }

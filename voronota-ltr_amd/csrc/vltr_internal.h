@@ -114,6 +114,7 @@ __device__ __forceinline__ bool pair_qualifies(const Pipeline& p, int64_t a, int
 // Host-side launchers (k_grid.hip).
 hipError_t launch_prepare(const Pipeline& p, hipStream_t s);
 hipError_t launch_grid(const Pipeline& p, hipStream_t s);
+hipError_t launch_scatter(const Pipeline& p, hipStream_t s);
 hipError_t launch_neighbors(const Pipeline& p, hipStream_t s);
 hipError_t launch_pairs(const Pipeline& p, hipStream_t s);
 // Exclusive scan of counts[0..n) in place into counts[0..n], counts[n] = total.
@@ -134,8 +135,10 @@ hipError_t upload_hexagon_constants(hipStream_t s);
 // E = sum_contacts W[t_a][t_b] area + alpha sum sas + beta sum volume, from the
 // records, topologies and seeds of the last full run; partial energy sums
 // per block into part[0 .. adjoint_blocks()).
+// Slots of the canonical-pair table (periodic runs): a power of two >= 2 n_pairs.
+uint64_t dedup_slots(int64_t n_pairs);
 hipError_t launch_adjoint(const Pipeline& p, const int32_t* types, const double* w16, double* grad, double* part,
-                          hipStream_t s);
+                          unsigned long long* dedup_keys, uint32_t* dedup_first, hipStream_t s);
 int64_t adjoint_blocks();
 
 // k_solvent.hip: emit == 0 counts the kept icosahedron samples (K unit

@@ -83,10 +83,10 @@ SIGNATURES = {
     "vltr_batch_free": (None, [_vp]),
     "vltr_compute_tessellation_batched_device": (C.c_int, [_vp, _vp, _u64p, _sz, C.c_double, C.c_int]),
     "vltr_context_batch_offsets": (C.c_int, [_vp, _sz, _u64p, _u64p]),
-    "vltr_energy_gradient": (C.c_int, [_vp, _vp, _sz, C.c_double, _i32p, _f64p, C.c_double, C.c_double, _f64p,
-                                       _f64p]),
-    "vltr_energy_gradient_device": (C.c_int, [_vp, _vp, _sz, C.c_double, _vp, _f64p, C.c_double, C.c_double, _vp,
-                                              _f64p]),
+    "vltr_energy_gradient": (C.c_int, [_vp, _vp, _sz, C.c_double, C.POINTER(VltrPeriodicBox), _i32p, _f64p,
+                                       C.c_double, C.c_double, _f64p, _f64p]),
+    "vltr_energy_gradient_device": (C.c_int, [_vp, _vp, _sz, C.c_double, C.POINTER(VltrPeriodicBox), _vp, _f64p,
+                                              C.c_double, C.c_double, _vp, _f64p]),
     "vltr_radii_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "vltr_radii_load_text": (C.c_int, [_vp, C.c_char_p]),
     "vltr_radii_get": (C.c_double, [_vp, C.c_char_p, C.c_char_p]),
@@ -188,9 +188,9 @@ def last_timings(device: int = 0):
 
 def last_counts(device: int = 0):
     lib = load_library()
-    buf = (C.c_uint64 * 7)()
-    k = lib.vltr_context_last_counts(context(device), buf, 7)
-    names = ["balls", "spheres", "pairs", "contacts", "cells", "slow_pairs", "nb_cap"]
+    buf = (C.c_uint64 * 8)()
+    k = lib.vltr_context_last_counts(context(device), buf, 8)
+    names = ["balls", "spheres", "pairs", "contacts", "cells", "slow_pairs", "nb_cap", "buckets"]
     return {names[i]: int(buf[i]) for i in range(k)}
 
 
@@ -431,9 +431,10 @@ class Updateable:
         return {names[i]: buf[i] for i in range(k)}
 
 
-def energy_gradient(balls, probe, types, W, sas_weight=1.0, volume_weight=1.0, device=0):
+def energy_gradient(balls, probe, types, W, sas_weight=1.0, volume_weight=1.0, pbox=None, device=0):
     """(E, dE/dcentres (n, 3)) for E = sum_contacts W[t_a][t_b] area +
-    sas_weight * sum SAS + volume_weight * sum volume."""
+    sas_weight * sum SAS + volume_weight * sum volume (periodic box: over the
+    deduplicated contact list)."""
     lib = load_library()
     ctx = context(device)
     b = _balls_array(balls)
@@ -443,7 +444,9 @@ def energy_gradient(balls, probe, types, W, sas_weight=1.0, volume_weight=1.0, d
         raise ValueError("one type per ball")
     e = C.c_double()
     g = np.zeros((len(b), 3), dtype=np.float64)
+    box = make_pbox(pbox)
     rc = lib.vltr_energy_gradient(ctx, b.ctypes.data_as(_vp) if len(b) else None, len(b), float(probe),
+                                  C.byref(box) if box is not None else None,
                                   t.ctypes.data_as(_i32p) if len(t) else None, w.ctypes.data_as(_f64p),
                                   float(sas_weight), float(volume_weight), C.byref(e),
                                   g.ctypes.data_as(_f64p) if len(b) else None)

@@ -34,7 +34,7 @@ class TessellationEnergy(torch.autograd.Function):
         torch.cuda.current_stream(centers.device).synchronize()
         e = C.c_double()
         rc = lib.vltr_energy_gradient_device(_native.context(centers.device.index or 0), C.c_void_p(balls.data_ptr()),
-                                             n, float(probe), C.c_void_p(t32.data_ptr()),
+                                             n, float(probe), None, C.c_void_p(t32.data_ptr()),
                                              w.ctypes.data_as(C.POINTER(C.c_double)), float(sas_weight),
                                              float(volume_weight), C.c_void_p(grad.data_ptr()), C.byref(e))
         _native.check(rc, "vltr_energy_gradient_device")

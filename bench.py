@@ -365,11 +365,27 @@ def dynamics_leg(args):
     t_gpu = (time.perf_counter() - t0) / len(moves)
     ph = u.last_timings()
     u.close()
+    # the same steps with the contact energy's adjoint in every pass
+    # (energy of the differentiation workload: W seeded, type = radius index)
+    types = np.searchsorted(np.array([1.52, 1.55, 1.70, 1.80]), balls[:, 3]).astype(np.int32)
+    u = _native.Updateable()
+    u.set_energy(types, workload.type_matrix(20000))
+    assert u.init(balls, args.probe, box)
+    u.energy_device()
+    t0 = time.perf_counter()
+    for b, ch in moves:
+        assert u.update_with_changed(b, ch)
+        e_last, _ = u.energy_device()
+    t_adj = (time.perf_counter() - t0) / len(moves)
+    u.close()
     out = {"balls": n, "box_edge_A": round(L, 3), "steps": len(moves), "moved_per_step": 500,
            "ms_per_step": round(t_gpu * 1e3, 3), "steps_per_s": round(1.0 / t_gpu, 1),
+           "ms_per_step_with_adjoint": round(t_adj * 1e3, 3), "last_energy": e_last,
            "mean_affected": round(affected / len(moves), 1), "full_reinits": reinit,
            "last_step_phases_ms": {k: round(v, 4) for k, v in ph.items()},
-           "note": "host API (the moved balls are sent each step; summary not downloaded)"}
+           "note": "host API (the moved balls are sent each step; summary not downloaded); with_adjoint: the "
+                   "energy of the differentiation workload and its gradient (left on the device) in every step's "
+                   "pass"}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle

@@ -241,6 +241,20 @@ int vltr_updateable_set_exclusion(vltr_updateable* u, size_t id, int excluded, i
 /* restore (updateable.rs:277-322): back to the state before the last
  * init / update / set_exclusion; *ok = 0 without backup */
 int vltr_updateable_restore(vltr_updateable* u, int* ok);
+/* Energy of the updateable's state, E = sum_contacts W[t_a][t_b] area +
+ * sas_weight * sum SAS + volume_weight * sum volume (vltr_energy_gradient's
+ * energy, with the state's box and exclusions), and its gradient with respect
+ * to the centres, computed in the same device pass as every later state
+ * change (init, update, set_exclusion, restore). types: one entry in 0..3 per
+ * ball (the state's ball count must match). No reference counterpart: the
+ * reference has no derivatives. */
+int vltr_updateable_set_energy(vltr_updateable* u, const int32_t* types, size_t n, const double* w16,
+                               double sas_weight, double volume_weight);
+/* E and dE/dcentres (3 n doubles, host) of the current state. */
+int vltr_updateable_energy_gradient(vltr_updateable* u, double* energy, double* grad);
+/* Same, the gradient left on the device (valid until the next call on u). */
+int vltr_updateable_energy_gradient_device(vltr_updateable* u, double* energy, const double** d_grad);
+
 /* summary (updateable.rs:333-376) as a result handle (free with
  * vltr_result_free); contacts listed per first ball, periodic ids canonical
  * and ordered */

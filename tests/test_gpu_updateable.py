@@ -191,3 +191,52 @@ def test_periodic_box_moves(oracle):
         balls[ch, :3] = np.mod(balls[ch, :3] + rng.normal(scale=0.05, size=(40, 3)), L)
         print(f"[gpu] periodic step {step}", flush=True)
         assert p.update(balls, ch)
+
+
+def test_energy_gradient_with_updates(oracle):
+    """The energy and gradient computed in the updateable's own pass equal a
+    standalone forward + adjoint of the same balls (periodic box), after
+    init, incremental updates, set_energy after the fact and restore; and the
+    oracle's dual numbers on a few coordinates."""
+    from voronota_ltr import _native, workload
+
+    L = 32.0
+    balls, _ = workload.rsa_box(1500, L, seed=12)
+    box = corners_box((0, 0, 0), (L, L, L))
+    types = np.arange(len(balls), dtype=np.int32) % 4
+    W = workload.type_matrix(3)
+    u = _native.Updateable(with_backup=True)
+    u.set_energy(types, W)
+    assert u.init(balls, 1.4, box)
+    rng = np.random.default_rng(6)
+
+    def check(b):
+        e, g = u.energy_gradient()
+        e2, g2 = _native.energy_gradient(b, 1.4, types, W, pbox=box)
+        assert e == pytest.approx(e2, rel=1e-12)
+        assert np.allclose(g, g2, rtol=1e-9, atol=1e-9)
+        return g
+
+    check(balls)
+    for step in range(3):
+        ch = np.sort(rng.choice(len(balls), 30, replace=False))
+        balls[ch, :3] = np.mod(balls[ch, :3] + rng.normal(scale=0.05, size=(30, 3)), L)
+        print(f"[gpu] updateable energy step {step}", flush=True)
+        assert u.update_with_changed(balls, ch)
+        g = check(balls)
+    for ball in rng.choice(len(balls), 4, replace=False):
+        _, de = oracle.energy_dual(balls, 1.4, types, W, int(ball), 0, pbox=box)
+        assert abs(g[ball, 0] - de) <= 1e-7 * max(1.0, abs(de))
+    prev = balls.copy()
+    ch = np.arange(10)
+    balls[ch, 0] = np.mod(balls[ch, 0] + 0.1, L)
+    assert u.update_with_changed(balls, ch)
+    assert u.restore()
+    check(prev)
+    # switched on after the state exists: computed on request
+    v = _native.Updateable()
+    assert v.init(prev, 1.4, box)
+    v.set_energy(types, W)
+    e, g = v.energy_gradient()
+    e2, g2 = _native.energy_gradient(prev, 1.4, types, W, pbox=box)
+    assert e == pytest.approx(e2, rel=1e-12) and np.allclose(g, g2, rtol=1e-9, atol=1e-9)

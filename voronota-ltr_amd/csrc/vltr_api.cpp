@@ -1211,6 +1211,12 @@ struct vltr_updateable {
   // cells and exclusion statuses of the current state and of the backup
   DevBuf cells_state, cells_backup, excl, excl_backup, affected_mask, ids, rows, rows_cnt, scatter;
   std::vector<uint8_t> mark;  // host scratch of the affected-set construction
+  // energy and gradient computed in the pipeline pass of every state change
+  bool energy_on = false, grad_valid = false;
+  double energy = 0.0, sas_weight = 1.0, volume_weight = 1.0;
+  std::vector<double> w16;
+  size_t n_types = 0;
+  DevBuf types, grad, part, topo, seed, dedup_keys, dedup_first;
 };
 
 namespace {
@@ -1311,17 +1317,55 @@ int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, c
   uint8_t* am = u->affected_mask.as<uint8_t>();
   const int32_t* more_ids = u->ids.as<int32_t>();
   const int64_t n_more = static_cast<int64_t>(more32.size());
+  // the energy's adjoint rides on the same pass (cut topologies and cell
+  // seeds written by the clip and cell kernels)
+  const bool adj = u->energy_on && u->n_types == n;
+  u->grad_valid = false;
+  const int32_t* d_types = nullptr;
+  double *d_grad = nullptr, *d_part = nullptr;
+  if (adj) {
+    VLTR_HIP(u->seed.ensure(n * sizeof(double)));
+    VLTR_HIP(u->grad.ensure(3 * n * sizeof(double)));
+    VLTR_HIP(u->part.ensure(adjoint_blocks() * sizeof(double)));
+    ex.topo_buf = &u->topo;
+    ex.seed = u->seed.as<double>();
+    ex.sas_weight = u->sas_weight;
+    ex.volume_weight = u->volume_weight;
+    d_types = u->types.as<int32_t>();
+    d_grad = u->grad.as<double>();
+    d_part = u->part.as<double>();
+  }
+  const double* w16 = u->w16.data();
   ex.post_cells = [=](const Pipeline& p, hipStream_t st2) -> hipError_t {
     if (n_more > 0) {
       const hipError_t e = launch_mark_more(p, more_ids, n_more, am, st2);
       if (e != hipSuccess) return e;
     }
-    return launch_updateable_cells(p, state, excl, full ? nullptr : am, full ? 1 : 0, refresh ? 1 : 0, st2);
+    hipError_t e = launch_updateable_cells(p, state, excl, full ? nullptr : am, full ? 1 : 0, refresh ? 1 : 0, st2);
+    if (e != hipSuccess || !adj) return e;
+    unsigned long long* keys = nullptr;
+    uint32_t* first = nullptr;
+    if (p.periodic) {
+      const uint64_t slots = dedup_slots(p.n_pairs);
+      if ((e = u->dedup_keys.ensure(slots * sizeof(unsigned long long))) != hipSuccess) return e;
+      if ((e = u->dedup_first.ensure(slots * sizeof(uint32_t))) != hipSuccess) return e;
+      keys = u->dedup_keys.as<unsigned long long>();
+      first = u->dedup_first.as<uint32_t>();
+    }
+    return launch_adjoint(p, d_types, w16, d_grad, d_part, keys, first, st2);
   };
   size_t nc = 0, ncell = 0;
   const int rc = run_pipeline(ctx, static_cast<int64_t>(n), st.probe, st.periodic ? &st.pbox : nullptr, nullptr, 0, 0,
                               &nc, &ncell, nullptr, 0, &ex);
   if (rc != VLTR_OK) return rc;
+  if (adj) {
+    std::vector<double> parts(static_cast<size_t>(adjoint_blocks()));
+    VLTR_HIP(hipMemcpy(parts.data(), d_part, parts.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double e = 0.0;
+    for (double v : parts) e += v;
+    u->energy = e;
+    u->grad_valid = true;
+  }
   st.n_contacts = nc;
   st.n_cells = ncell;
   st.empty = ncell == 0;
@@ -1570,6 +1614,61 @@ int vltr_updateable_restore(vltr_updateable* u, int* ok) {
   u->st.affected = kept.affected;
   u->st.was_full_reinit = kept.was_full_reinit;
   *ok = 1;
+  return VLTR_OK;
+}
+
+int vltr_updateable_set_energy(vltr_updateable* u, const int32_t* types, size_t n, const double* w16,
+                               double sas_weight, double volume_weight) {
+  if (!u || !w16 || (n > 0 && !types)) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  for (size_t i = 0; i < n; ++i)
+    if (types[i] < 0 || types[i] > 3) return fail(VLTR_ERR_INVALID_ARGUMENT, "types must be in 0..3");
+  VLTR_HIP(hipSetDevice(u->ctx->device));
+  VLTR_HIP(u->types.ensure(std::max<size_t>(n, 1) * sizeof(int32_t)));
+  if (n > 0) VLTR_HIP(hipMemcpy(u->types.ptr, types, n * sizeof(int32_t), hipMemcpyHostToDevice));
+  u->w16.assign(w16, w16 + 16);
+  u->sas_weight = sas_weight;
+  u->volume_weight = volume_weight;
+  u->n_types = n;
+  u->energy_on = true;
+  u->grad_valid = false;
+  return VLTR_OK;
+}
+
+namespace {
+// The energy of the current state: from the last pass, or from a pass over
+// the unchanged state (contacts recomputed, cells kept) when the energy was
+// switched on after it.
+int upd_energy(vltr_updateable* u) {
+  if (!u->energy_on) return fail(VLTR_ERR_INVALID_ARGUMENT, "no energy set (vltr_updateable_set_energy)");
+  if (u->n_types != u->st.balls.size()) return fail(VLTR_ERR_INVALID_ARGUMENT, "one type per ball required");
+  if (!u->grad_valid && !u->st.balls.empty()) {
+    const auto kept = u->st;
+    const int rc = upd_run(u, false, false, {}, {});
+    if (rc != VLTR_OK) return rc;
+    u->st.changed = kept.changed;
+    u->st.affected = kept.affected;
+    u->st.was_full_reinit = kept.was_full_reinit;
+  }
+  return VLTR_OK;
+}
+}  // namespace
+
+int vltr_updateable_energy_gradient(vltr_updateable* u, double* energy, double* grad) {
+  if (!u || !energy || (!grad && !u->st.balls.empty())) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  const int rc = upd_energy(u);
+  if (rc != VLTR_OK) return rc;
+  const size_t n = u->st.balls.size();
+  *energy = n ? u->energy : 0.0;
+  if (n) VLTR_HIP(hipMemcpy(grad, u->grad.ptr, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  return VLTR_OK;
+}
+
+int vltr_updateable_energy_gradient_device(vltr_updateable* u, double* energy, const double** d_grad) {
+  if (!u || !energy || !d_grad) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  const int rc = upd_energy(u);
+  if (rc != VLTR_OK) return rc;
+  *energy = u->st.balls.empty() ? 0.0 : u->energy;
+  *d_grad = u->st.balls.empty() ? nullptr : u->grad.as<double>();
   return VLTR_OK;
 }
 

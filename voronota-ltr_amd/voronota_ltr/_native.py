@@ -128,6 +128,9 @@ SIGNATURES = {
     "vltr_updateable_last_update_was_full_reinit": (C.c_int, [_vp]),
     "vltr_updateable_backup_enabled": (C.c_int, [_vp]),
     "vltr_updateable_last_timings": (C.c_int, [_vp, _f64p, C.c_int]),
+    "vltr_updateable_set_energy": (C.c_int, [_vp, _i32p, _sz, _f64p, C.c_double, C.c_double]),
+    "vltr_updateable_energy_gradient": (C.c_int, [_vp, _f64p, _f64p]),
+    "vltr_updateable_energy_gradient_device": (C.c_int, [_vp, _f64p, C.POINTER(_vp)]),
 }
 
 _lock = threading.Lock()
@@ -396,6 +399,32 @@ class Updateable:
 
     def restore(self):
         return self._call("vltr_updateable_restore")
+
+    def set_energy(self, types, W, sas_weight=1.0, volume_weight=1.0):
+        """E = sum_contacts W[t_a][t_b] area + sas_weight sum SAS +
+        volume_weight sum volume and its gradient, computed with every later
+        state change."""
+        t = np.ascontiguousarray(np.asarray(types, dtype=np.int32).reshape(-1))
+        w = np.ascontiguousarray(np.asarray(W, dtype=np.float64).reshape(16))
+        check(self._lib.vltr_updateable_set_energy(self._h, t.ctypes.data_as(_i32p) if len(t) else None, len(t),
+                                                   w.ctypes.data_as(_f64p), float(sas_weight), float(volume_weight)),
+              "vltr_updateable_set_energy")
+        self._n_types = len(t)
+
+    def energy_gradient(self):
+        """(E, dE/dcentres (n, 3)) of the current state."""
+        e = C.c_double()
+        g = np.zeros((getattr(self, "_n_types", 0), 3))
+        check(self._lib.vltr_updateable_energy_gradient(self._h, C.byref(e), g.ctypes.data_as(_f64p)),
+              "vltr_updateable_energy_gradient")
+        return e.value, g
+
+    def energy_device(self):
+        """(E, device pointer to dE/dcentres, 3 n doubles) of the current state."""
+        e, ptr = C.c_double(), C.c_void_p()
+        check(self._lib.vltr_updateable_energy_gradient_device(self._h, C.byref(e), C.byref(ptr)),
+              "vltr_updateable_energy_gradient_device")
+        return e.value, ptr.value
 
     def summary(self) -> Result:
         h = C.c_void_p()

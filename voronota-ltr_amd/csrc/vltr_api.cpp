@@ -1433,23 +1433,39 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
   // |changed| + |new ids| > threshold (spheres_container.rs:105-124)
   std::vector<uint8_t>& mark = u->mark;
   mark.assign(n, 0);
-  for (uint64_t id : changed) mark[id] = 1;
+  bool repeats = false;  // a changed id listed twice stays twice until the final sort + unique
+  for (uint64_t id : changed) {
+    repeats = repeats || mark[id] != 0;
+    mark[id] = 3;  // 3 changed, 1 old neighbor, 2 new neighbor
+  }
   std::vector<std::vector<uint64_t>> old_rows;
   rc = upd_rows(u, changed, &old_rows);
   if (rc != VLTR_OK) return rc;
-  std::vector<uint64_t> affected = changed;
+  size_t n_affected = changed.size();
   for (const auto& row : old_rows)
     for (uint64_t cid : row)
       if (!mark[cid]) {
         mark[cid] = 1;
-        affected.push_back(cid);
+        ++n_affected;
       }
-  if (affected.size() > threshold) {
+  if (n_affected > threshold) {
     rc = upd_full(u, balls, nb);
     if (rc == VLTR_OK) *ok = 1;
     return rc;
   }
-  std::sort(affected.begin(), affected.end());
+  // the sorted affected list: the marked ids in order (a linear scan of the
+  // marks, not a sort)
+  std::vector<uint64_t> affected;
+  affected.reserve(n_affected);
+  if (repeats) {
+    affected = changed;
+    for (size_t i = 0; i < n; ++i)
+      if (mark[i] == 1) affected.push_back(i);
+    std::sort(affected.begin(), affected.end());
+  } else {
+    for (size_t i = 0; i < n; ++i)
+      if (mark[i]) affected.push_back(i);
+  }
   for (uint64_t id : changed) st.balls[id] = balls[id];
   rc = upd_run(u, false, true, affected, changed, &changed);
   if (rc != VLTR_OK) return rc;
@@ -1462,12 +1478,13 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
     for (uint64_t cid : row)
       if (!mark[cid]) {
         mark[cid] = 2;
-        affected.push_back(cid);
         more = true;
       }
   if (more) {
-    std::sort(affected.begin(), affected.end());
-    affected.erase(std::unique(affected.begin(), affected.end()), affected.end());
+    // sorted and without repeats: the marked ids in order
+    affected.clear();
+    for (size_t i = 0; i < n; ++i)
+      if (mark[i]) affected.push_back(i);
   }
   st.changed = changed;
   st.affected = affected;

@@ -21,6 +21,12 @@
 // outgrows that goes to k_cut_slow: one lane per pair, the one-lane code of
 // contour.h with the contour in a global workspace. All paths produce
 // identical records.
+//
+// Updateable steps (p.dirty set) clip only the pairs whose two balls are
+// both affected; every other pair keeps the record of the previous run
+// (updateable.rs:476-512). A wavefront whose ball is not affected copies its
+// records as one block, and an affected ball looks up each kept pair in its
+// previous pair list.
 #include "contour.h"
 #include "group_clip.h"
 #include "vltr_internal.h"
@@ -87,6 +93,77 @@ __device__ __forceinline__ void push_wide(const Pipeline& p, int64_t pidx) {
   p.wide_list[k] = static_cast<int32_t>(pidx);
 }
 
+// ---- updateable steps: pairs that keep their previous record
+constexpr int kRecWords = static_cast<int>(sizeof(PairRec) / 8);
+constexpr int kTopoWords = static_cast<int>(sizeof(ContactTopo) / 8);
+static_assert(sizeof(PairRec) % 8 == 0 && sizeof(ContactTopo) % 8 == 0, "records are copied as 8-byte words");
+static_assert(kRecWords + kTopoWords <= kWave, "one pair's words fit one wavefront");
+
+// Position of partner b in the previous run's pair list [ob, ob + on) of the
+// same ball, or -1. Every lane of the wavefront takes part (b uniform).
+__device__ int64_t prev_find_wave(const Pipeline& p, int64_t ob, int on, int32_t b, int lane) {
+  for (int j0 = 0; j0 < on; j0 += kWave) {
+    const int j = j0 + lane;
+    const uint64_t hit = __ballot(j < on && p.prev_pair_b[ob + j] == b);
+    if (hit != 0ull) return ob + j0 + __builtin_ctzll(hit);
+  }
+  return -1;
+}
+__device__ int64_t prev_find_lane(const Pipeline& p, int64_t ob, int on, int32_t b) {
+  for (int j = 0; j < on; ++j)
+    if (p.prev_pair_b[ob + j] == b) return ob + j;
+  return -1;
+}
+
+// Pair `to` takes the previous run's record (and topology) of pair `from`,
+// or an empty record when from < 0; the wavefront's lanes copy one word each.
+__device__ void keep_pair_wave(const Pipeline& p, int64_t from, int64_t to, int lane) {
+  if (from < 0) {
+    if (lane == 0) p.rec[to] = empty_record();
+    return;
+  }
+  if (lane < kRecWords) {
+    reinterpret_cast<uint64_t*>(p.rec + to)[lane] = reinterpret_cast<const uint64_t*>(p.prev_rec + from)[lane];
+  } else if (p.topo != nullptr && lane < kRecWords + kTopoWords) {
+    reinterpret_cast<uint64_t*>(p.topo + to)[lane - kRecWords] =
+        reinterpret_cast<const uint64_t*>(p.prev_topo + from)[lane - kRecWords];
+  }
+}
+__device__ void keep_pair_lane(const Pipeline& p, int64_t from, int64_t to) {
+  if (from < 0) {
+    p.rec[to] = empty_record();
+    return;
+  }
+  p.rec[to] = p.prev_rec[from];
+  if (p.topo != nullptr) p.topo[to] = p.prev_topo[from];
+}
+
+// Ball a is not dirty: all its pairs keep their previous records. When its
+// pair list is the previous one (the usual case: no neighbor of a moved),
+// the records are copied as one block.
+__device__ void keep_ball(const Pipeline& p, int64_t base, int npairs, int64_t ob, int on, int lane) {
+  bool diff = on != npairs;
+  if (!diff)
+    for (int j = lane; j < npairs; j += kWave) diff = diff || p.pair_b[base + j] != p.prev_pair_b[ob + j];
+  if (__ballot(diff) == 0ull) {
+    const uint64_t* rs = reinterpret_cast<const uint64_t*>(p.prev_rec + ob);
+    uint64_t* rd = reinterpret_cast<uint64_t*>(p.rec + base);
+    for (int t = lane; t < npairs * kRecWords; t += kWave) rd[t] = rs[t];
+    if (p.topo != nullptr) {
+      const uint64_t* ts = reinterpret_cast<const uint64_t*>(p.prev_topo + ob);
+      uint64_t* td = reinterpret_cast<uint64_t*>(p.topo + base);
+      for (int t = lane; t < npairs * kTopoWords; t += kWave) td[t] = ts[t];
+    }
+    return;
+  }
+  for (int j = 0; j < npairs; ++j)
+    keep_pair_wave(p, prev_find_wave(p, ob, on, p.pair_b[base + j], lane), base + j, lane);
+}
+
+__device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
+  return p.dirty[p.periodic ? id % p.n : id] != 0;
+}
+
 __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_eu(VLTR_CLIP_WAVES_PER_EU, VLTR_CLIP_WAVES_PER_EU))) void k_clip(Pipeline p) {
   __shared__ ClipLds lds[kClipWaves];
   const int wid = threadIdx.x / kWave;
@@ -101,8 +178,25 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
   const int cnt = p.nb_cnt[a];
   const int32_t* nbl = p.nb + a * static_cast<int64_t>(p.nb_cap);
   const double* sph4 = reinterpret_cast<const double*>(p.sph);
+  const bool incremental = p.dirty != nullptr;
+  int64_t ob = 0;
+  int on = 0;
+  if (incremental) {
+    ob = p.prev_pair_cnt[a];
+    on = static_cast<int>(static_cast<int64_t>(p.prev_pair_cnt[a + 1]) - ob);
+    if (!p.dirty[a]) {
+      keep_ball(p, base, npairs, ob, on, lane);
+      return;
+    }
+  }
   if (cnt > kClipSlots) {
-    for (int j = lane; j < npairs; j += kWave) push_wide(p, base + j);
+    for (int j = lane; j < npairs; j += kWave) {
+      const int32_t b = p.pair_b[base + j];
+      if (incremental && !is_dirty(p, b))
+        keep_pair_lane(p, prev_find_lane(p, ob, on, b), base + j);
+      else
+        push_wide(p, base + j);
+    }
     return;
   }
 
@@ -137,6 +231,10 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     const int32_t b = __builtin_amdgcn_readlane(c0, k);
     if (!pair_qualifies(p, a, b)) continue;
     const int jj = jp++;
+    if (incremental && !is_dirty(p, b)) {
+      keep_pair_wave(p, prev_find_wave(p, ob, on, b, lane), base + jj, lane);
+      continue;
+    }
     if (!((ok_mask >> k) & 1ull)) {
       if (lane == 0) p.rec[base + jj] = empty_record();
       continue;

@@ -214,6 +214,15 @@ struct RunExtras {
   DevBuf* net_out = nullptr;
   double* seed = nullptr;
   double sas_weight = 1.0, volume_weight = 1.0;
+  // updateable steps: a step run once the neighbor lists are known (before
+  // the clip), and the dirty mask with the previous run's pair lists,
+  // records and topologies (null dirty: every pair is clipped)
+  std::function<hipError_t(const Pipeline&, hipStream_t)> pre_clip;
+  const uint8_t* dirty = nullptr;
+  const uint32_t* prev_pair_cnt = nullptr;
+  const int32_t* prev_pair_b = nullptr;
+  const PairRec* prev_rec = nullptr;
+  const ContactTopo* prev_topo = nullptr;
 };
 
 int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic_box* pbox, const int32_t* d_groups,
@@ -332,6 +341,14 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     p.net = extras->net_buf->as<ContactNet>();
   }
   VLTR_HIP(launch_pairs(p, s));
+  if (extras && extras->pre_clip) VLTR_HIP(extras->pre_clip(p, s));
+  if (extras && extras->dirty) {
+    p.dirty = extras->dirty;
+    p.prev_pair_cnt = extras->prev_pair_cnt;
+    p.prev_pair_b = extras->prev_pair_b;
+    p.prev_rec = extras->prev_rec;
+    p.prev_topo = extras->prev_topo;
+  }
   VLTR_HIP(hipEventRecord(ctx->ev[3], s));
   VLTR_HIP(launch_clip(p, s));
   VLTR_HIP(hipEventRecord(ctx->ev[4], s));
@@ -1217,6 +1234,11 @@ struct vltr_updateable {
   std::vector<double> w16;
   size_t n_types = 0;
   DevBuf types, grad, part, topo, seed, dedup_keys, dedup_first;
+  // pair lists, records and topologies of the last run: the next step's
+  // pairs that are not clipped again (one end not affected) take them
+  DevBuf prev_pair_cnt, prev_pair_b, prev_rec, prev_topo;
+  bool prev_ok = false, prev_topo_ok = false;
+  size_t prev_n = 0;
 };
 
 namespace {
@@ -1260,12 +1282,17 @@ int upd_rows(Upd* u, const std::vector<uint64_t>& ids, std::vector<std::vector<u
 // One tessellation of the current balls. full: init / full reinit rules;
 // otherwise only balls in aff take new cells. refresh: affected balls get
 // their hidden status back as exclusion status (collision update). more:
-// balls whose new neighbors join the affected set.
+// balls whose new neighbors join the affected set. incremental: only pairs
+// with both balls affected are clipped; the others keep the records of the
+// last run (updateable.rs:476-512), which must be of the state this step
+// starts from.
 int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, const std::vector<uint64_t>& more,
-            const std::vector<uint64_t>* moved = nullptr) {
+            const std::vector<uint64_t>* moved = nullptr, bool incremental = false) {
   vltr_context* ctx = u->ctx;
   auto& st = u->st;
   const size_t n = st.balls.size();
+  const bool keep = incremental && !full && u->prev_ok && u->prev_n == n;
+  u->prev_ok = false;
   st.n_contacts = st.n_cells = 0;
   if (n == 0) {
     st.empty = true;
@@ -1336,11 +1363,19 @@ int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, c
     d_part = u->part.as<double>();
   }
   const double* w16 = u->w16.data();
+  // the changed balls' new neighbors join the affected set before the clip
+  if (n_more > 0)
+    ex.pre_clip = [=](const Pipeline& p, hipStream_t st2) -> hipError_t {
+      return launch_mark_more(p, more_ids, n_more, am, st2);
+    };
+  if (keep && (!adj || u->prev_topo_ok)) {
+    ex.dirty = am;
+    ex.prev_pair_cnt = u->prev_pair_cnt.as<uint32_t>();
+    ex.prev_pair_b = u->prev_pair_b.as<int32_t>();
+    ex.prev_rec = u->prev_rec.as<PairRec>();
+    ex.prev_topo = adj ? u->prev_topo.as<ContactTopo>() : nullptr;
+  }
   ex.post_cells = [=](const Pipeline& p, hipStream_t st2) -> hipError_t {
-    if (n_more > 0) {
-      const hipError_t e = launch_mark_more(p, more_ids, n_more, am, st2);
-      if (e != hipSuccess) return e;
-    }
     hipError_t e = launch_updateable_cells(p, state, excl, full ? nullptr : am, full ? 1 : 0, refresh ? 1 : 0, st2);
     if (e != hipSuccess || !adj) return e;
     unsigned long long* keys = nullptr;
@@ -1358,6 +1393,15 @@ int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, c
   const int rc = run_pipeline(ctx, static_cast<int64_t>(n), st.probe, st.periodic ? &st.pbox : nullptr, nullptr, 0, 0,
                               &nc, &ncell, nullptr, 0, &ex);
   if (rc != VLTR_OK) return rc;
+  // this run's pair lists and records become the previous run's (the
+  // context's buffers are scratch of the next run)
+  std::swap(ctx->pair_cnt, u->prev_pair_cnt);
+  std::swap(ctx->pair_b, u->prev_pair_b);
+  std::swap(ctx->rec, u->prev_rec);
+  if (adj) std::swap(u->topo, u->prev_topo);
+  u->prev_ok = true;
+  u->prev_topo_ok = adj;
+  u->prev_n = n;
   if (adj) {
     std::vector<double> parts(static_cast<size_t>(adjoint_blocks()));
     VLTR_HIP(hipMemcpy(parts.data(), d_part, parts.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1467,7 +1511,7 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
       if (mark[i]) affected.push_back(i);
   }
   for (uint64_t id : changed) st.balls[id] = balls[id];
-  rc = upd_run(u, false, true, affected, changed, &changed);
+  rc = upd_run(u, false, true, affected, changed, &changed, true);
   if (rc != VLTR_OK) return rc;
   // spheres_container.rs:150-170: new neighbors of the changed balls join
   std::vector<std::vector<uint64_t>> new_rows;
@@ -1601,7 +1645,7 @@ int vltr_updateable_set_exclusion(vltr_updateable* u, size_t id, int excluded, i
   st.changed = {static_cast<uint64_t>(id)};
   st.affected = affected;
   st.was_full_reinit = false;
-  rc = upd_run(u, false, false, affected, {});
+  rc = upd_run(u, false, false, affected, {}, nullptr, true);
   if (rc != VLTR_OK) return rc;
   *ok = 1;
   return VLTR_OK;
@@ -1660,7 +1704,7 @@ int upd_energy(vltr_updateable* u) {
   if (u->n_types != u->st.balls.size()) return fail(VLTR_ERR_INVALID_ARGUMENT, "one type per ball required");
   if (!u->grad_valid && !u->st.balls.empty()) {
     const auto kept = u->st;
-    const int rc = upd_run(u, false, false, {}, {});
+    const int rc = upd_run(u, false, false, {}, {}, nullptr, true);
     if (rc != VLTR_OK) return rc;
     u->st.changed = kept.changed;
     u->st.affected = kept.affected;

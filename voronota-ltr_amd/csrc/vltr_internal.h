@@ -94,6 +94,17 @@ struct Pipeline {
   double sas_weight, volume_weight;  // alpha, beta of the adjoint's energy
   uint32_t* in_cnt;        // n: pairs (a, b), a < b, received by ball b
   int32_t* incoming;       // n * nb_cap: their pair indices (any order)
+
+  // Updateable steps (updateable.rs:476-512): a pair is clipped only when
+  // both of its balls are marked in dirty; every other pair keeps its record
+  // (and cut topology) of the previous run, found by its partner in that
+  // run's pair list of the same ball, or gets an empty record when it was no
+  // pair then. Null dirty: every pair is clipped.
+  const uint8_t* dirty;           // n
+  const uint32_t* prev_pair_cnt;  // n + 1: pair bases of the previous run
+  const int32_t* prev_pair_b;     // its pair partners
+  const PairRec* prev_rec;        // its records
+  const ContactTopo* prev_topo;   // its topologies (read when topo is set)
 };
 
 // tessellation.rs:323-328

@@ -188,10 +188,10 @@ class UpdateableTessellation:
                 "total_volume": float(r.total_volume), "total_contact_area": float(r.total_contact_area)}
 
     def changed_ids(self):
-        return self._u.changed_ids()
+        return self._u.changed_ids().tolist()
 
     def affected_ids(self):
-        return self._u.affected_ids()
+        return self._u.affected_ids().tolist()
 
     def last_update_was_full_reinit(self):
         return self._u.last_update_was_full_reinit()

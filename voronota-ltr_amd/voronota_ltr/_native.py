@@ -439,7 +439,7 @@ class Updateable:
         k = fn(self._h, None, 0)
         out = np.zeros(max(k, 1), dtype=np.uint64)
         fn(self._h, out.ctypes.data_as(_u64p), k)
-        return [int(x) for x in out[:k]]
+        return out[:k]
 
     def changed_ids(self):
         return self._ids("vltr_updateable_changed_ids")

@@ -236,15 +236,27 @@ __global__ void k_updateable_cells(Pipeline p, CellRec* state, uint8_t* excl, co
   state[i] = out;
 }
 
-// spheres_container.rs:150-170: the changed balls' new colliding neighbors
-// (canonical ids) join the affected set.
-__global__ void k_mark_more(Pipeline p, const int32_t* changed, int64_t k, uint8_t* affected) {
+// spheres_container.rs:105-124 and 150-170: the colliding neighbors
+// (canonical ids) of the changed balls join the affected set; a ball not yet
+// marked takes the mark `value` (every writer of one ball writes the same).
+__global__ void k_mark_neighbors(Pipeline p, const int32_t* changed, int64_t k, uint8_t* affected, uint8_t value) {
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (t >= k * p.nb_cap) return;
   const int64_t r = t / p.nb_cap, j = t - r * p.nb_cap;
   const int32_t id = changed[r];
   if (id < 0 || id >= p.n || j >= p.nb_cnt[id]) return;
-  affected[p.nb[static_cast<int64_t>(id) * p.nb_cap + j] % p.n] = 1;
+  uint8_t* m = affected + p.nb[static_cast<int64_t>(id) * p.nb_cap + j] % p.n;
+  if (*m == 0) *m = value;
+}
+
+__global__ void k_mark_ids(uint8_t* mask, int64_t n, const int32_t* ids, int64_t k, uint8_t value) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t < k && ids[t] >= 0 && ids[t] < n) mask[ids[t]] = value;
+}
+
+__global__ void k_fill_u8(uint8_t* mask, int64_t n, uint8_t value) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) mask[i] = value;
 }
 
 __global__ void k_scatter_balls(vltr_ball* balls, const int32_t* ids, const vltr_ball* vals, int64_t count) {
@@ -291,9 +303,19 @@ hipError_t launch_updateable_cells(const Pipeline& p, CellRec* state, uint8_t* e
                        refresh);
   return hipGetLastError();
 }
-hipError_t launch_mark_more(const Pipeline& p, const int32_t* changed, int64_t k, uint8_t* affected, hipStream_t s) {
+hipError_t launch_mark_neighbors(const Pipeline& p, const int32_t* changed, int64_t k, uint8_t* affected,
+                                 uint8_t value, hipStream_t s) {
   if (k > 0 && p.n > 0)
-    hipLaunchKernelGGL(k_mark_more, dim3(blocks_for(k * p.nb_cap, 256)), dim3(256), 0, s, p, changed, k, affected);
+    hipLaunchKernelGGL(k_mark_neighbors, dim3(blocks_for(k * p.nb_cap, 256)), dim3(256), 0, s, p, changed, k, affected,
+                       value);
+  return hipGetLastError();
+}
+hipError_t launch_mark_ids(uint8_t* mask, int64_t n, const int32_t* ids, int64_t k, uint8_t value, hipStream_t s) {
+  if (k > 0) hipLaunchKernelGGL(k_mark_ids, dim3(blocks_for(k, 256)), dim3(256), 0, s, mask, n, ids, k, value);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u8(uint8_t* mask, int64_t n, uint8_t value, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(n, 256)), dim3(256), 0, s, mask, n, value);
   return hipGetLastError();
 }
 hipError_t launch_scatter_balls(vltr_ball* balls, const int32_t* ids, const vltr_ball* vals, int64_t count,

@@ -1226,8 +1226,8 @@ struct vltr_updateable {
   State st, bk;
   bool has_backup = false;
   // cells and exclusion statuses of the current state and of the backup
-  DevBuf cells_state, cells_backup, excl, excl_backup, affected_mask, ids, rows, rows_cnt, scatter;
-  std::vector<uint8_t> mark;  // host scratch of the affected-set construction
+  DevBuf cells_state, cells_backup, excl, excl_backup, affected_mask, ids, rows, rows_cnt, moved_vals;
+  HostPinned stage;  // staging of moved balls and affected marks
   // energy and gradient computed in the pipeline pass of every state change
   bool energy_on = false, grad_valid = false;
   double energy = 0.0, sas_weight = 1.0, volume_weight = 1.0;
@@ -1279,41 +1279,49 @@ int upd_rows(Upd* u, const std::vector<uint64_t>& ids, std::vector<std::vector<u
   return VLTR_OK;
 }
 
-// One tessellation of the current balls. full: init / full reinit rules;
-// otherwise only balls in aff take new cells. refresh: affected balls get
-// their hidden status back as exclusion status (collision update). more:
-// balls whose new neighbors join the affected set. incremental: only pairs
-// with both balls affected are clipped; the others keep the records of the
-// last run (updateable.rs:476-512), which must be of the state this step
-// starts from.
-int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, const std::vector<uint64_t>& more,
-            const std::vector<uint64_t>* moved = nullptr, bool incremental = false) {
+// Options of one updateable run.
+struct RunOpts {
+  bool full = false;         // init / full reinit rules: every ball takes a new cell
+  bool refresh = false;      // affected balls get their hidden status back as exclusion status
+  bool incremental = false;  // clip only pairs with both balls affected; the others keep the
+                             // records of the last run (updateable.rs:476-512), which must be
+                             // of the state this run starts from
+  const std::vector<uint64_t>* aff = nullptr;  // affected balls, uploaded as the mask
+  bool mask_ready = false;   // the affected mask is on the device already
+  int64_t n_moved = 0;       // balls rewritten in the device copy of the previous balls
+                             // (ids in u->ids, values in u->moved_vals, on the device)
+  bool more = false;         // the moved balls' new neighbors join the affected set (mark 2)
+  uint8_t* mask_out = nullptr;  // pinned host copy of the final affected mask (n bytes)
+};
+
+// Pinned staging of at least `bytes`, after the stream is idle when it grows.
+int upd_stage(Upd* u, size_t bytes) {
+  if (u->stage.bytes < bytes) VLTR_HIP(hipStreamSynchronize(u->ctx->stream));
+  VLTR_HIP(u->stage.ensure(bytes));
+  return VLTR_OK;
+}
+
+// One tessellation of the current balls; without full, only affected balls
+// take new cells.
+int upd_run(Upd* u, const RunOpts& o) {
   vltr_context* ctx = u->ctx;
   auto& st = u->st;
   const size_t n = st.balls.size();
-  const bool keep = incremental && !full && u->prev_ok && u->prev_n == n;
+  const bool keep = o.incremental && !o.full && u->prev_ok && u->prev_n == n;
+  const bool scatter = o.n_moved > 0 && ctx->last_n == n;
   u->prev_ok = false;
+  ctx->last_n = 0;  // until this run completes
   st.n_contacts = st.n_cells = 0;
   if (n == 0) {
     st.empty = true;
-    ctx->last_n = 0;
     return VLTR_OK;
   }
   hipStream_t s = ctx->stream;
   VLTR_HIP(hipSetDevice(ctx->device));
-  if (moved != nullptr && ctx->last_n == n) {
+  if (scatter) {
     // the device copy holds the previous balls: write only the moved ones
-    std::vector<int32_t> ids(moved->begin(), moved->end());
-    std::vector<vltr_ball> vals;
-    vals.reserve(ids.size());
-    for (uint64_t id : *moved) vals.push_back(st.balls[id]);
-    const size_t vals_off = (ids.size() * sizeof(int32_t) + 255) / 256 * 256;
-    VLTR_HIP(u->scatter.ensure(vals_off + vals.size() * sizeof(vltr_ball)));
-    int32_t* d_ids = u->scatter.as<int32_t>();
-    vltr_ball* d_vals = reinterpret_cast<vltr_ball*>(u->scatter.as<unsigned char>() + vals_off);
-    VLTR_HIP(hipMemcpyAsync(d_vals, vals.data(), vals.size() * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
-    VLTR_HIP(hipMemcpyAsync(d_ids, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-    VLTR_HIP(launch_scatter_balls(ctx->balls.as<vltr_ball>(), d_ids, d_vals, static_cast<int64_t>(ids.size()), s));
+    VLTR_HIP(launch_scatter_balls(ctx->balls.as<vltr_ball>(), u->ids.as<int32_t>(), u->moved_vals.as<vltr_ball>(),
+                                  o.n_moved, s));
   } else {
     VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
     VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, st.balls.data(), n * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
@@ -1321,29 +1329,30 @@ int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, c
   VLTR_HIP(u->cells_state.ensure(n * sizeof(CellRec)));
   VLTR_HIP(u->excl.ensure(n));
   VLTR_HIP(u->affected_mask.ensure(n));
-  std::vector<uint8_t> mask;
-  if (full) {
-    VLTR_HIP(hipMemsetAsync(u->excl.ptr, 0, n, s));
+  uint8_t* am = u->affected_mask.as<uint8_t>();
+  if (o.full) {
+    VLTR_HIP(launch_fill_u8(u->excl.as<uint8_t>(), static_cast<int64_t>(n), 0, s));
   } else {
-    mask.assign(n, 0);
-    for (uint64_t id : aff)
-      if (id < n) mask[id] = 1;
-    VLTR_HIP(hipMemcpyAsync(u->affected_mask.ptr, mask.data(), n, hipMemcpyHostToDevice, s));
-    if (refresh) VLTR_HIP(launch_mask_clear(u->excl.as<uint8_t>(), u->affected_mask.as<uint8_t>(),
-                                            static_cast<int64_t>(n), s));
-  }
-  std::vector<int32_t> more32(more.begin(), more.end());
-  if (!more32.empty()) {
-    VLTR_HIP(u->ids.ensure(more32.size() * sizeof(int32_t)));
-    VLTR_HIP(hipMemcpyAsync(u->ids.ptr, more32.data(), more32.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (!o.mask_ready) {
+      const int rc = upd_stage(u, n);
+      if (rc != VLTR_OK) return rc;
+      uint8_t* hm = static_cast<uint8_t*>(u->stage.ptr);
+      std::memset(hm, 0, n);
+      if (o.aff)
+        for (uint64_t id : *o.aff)
+          if (id < n) hm[id] = 1;
+      VLTR_HIP(hipMemcpyAsync(am, hm, n, hipMemcpyHostToDevice, s));
+    }
+    if (o.refresh) VLTR_HIP(launch_mask_clear(u->excl.as<uint8_t>(), am, static_cast<int64_t>(n), s));
   }
   RunExtras ex;
-  ex.user_excl = full ? nullptr : u->excl.as<uint8_t>();
+  ex.user_excl = o.full ? nullptr : u->excl.as<uint8_t>();
   CellRec* state = u->cells_state.as<CellRec>();
   uint8_t* excl = u->excl.as<uint8_t>();
-  uint8_t* am = u->affected_mask.as<uint8_t>();
-  const int32_t* more_ids = u->ids.as<int32_t>();
-  const int64_t n_more = static_cast<int64_t>(more32.size());
+  const int32_t* moved_ids = u->ids.as<int32_t>();
+  const int64_t n_more = o.more ? o.n_moved : 0;
+  const bool full = o.full, refresh = o.refresh;
+  uint8_t* mask_out = o.mask_out;
   // the energy's adjoint rides on the same pass (cut topologies and cell
   // seeds written by the clip and cell kernels)
   const bool adj = u->energy_on && u->n_types == n;
@@ -1363,10 +1372,10 @@ int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, c
     d_part = u->part.as<double>();
   }
   const double* w16 = u->w16.data();
-  // the changed balls' new neighbors join the affected set before the clip
+  // the moved balls' new neighbors join the affected set before the clip
   if (n_more > 0)
     ex.pre_clip = [=](const Pipeline& p, hipStream_t st2) -> hipError_t {
-      return launch_mark_more(p, more_ids, n_more, am, st2);
+      return launch_mark_neighbors(p, moved_ids, n_more, am, 2, st2);
     };
   if (keep && (!adj || u->prev_topo_ok)) {
     ex.dirty = am;
@@ -1377,7 +1386,11 @@ int upd_run(Upd* u, bool full, bool refresh, const std::vector<uint64_t>& aff, c
   }
   ex.post_cells = [=](const Pipeline& p, hipStream_t st2) -> hipError_t {
     hipError_t e = launch_updateable_cells(p, state, excl, full ? nullptr : am, full ? 1 : 0, refresh ? 1 : 0, st2);
-    if (e != hipSuccess || !adj) return e;
+    if (e != hipSuccess) return e;
+    // the final marks travel back with the run's own synchronisation
+    if (mask_out != nullptr && (e = hipMemcpyAsync(mask_out, am, n, hipMemcpyDeviceToHost, st2)) != hipSuccess)
+      return e;
+    if (!adj) return hipSuccess;
     unsigned long long* keys = nullptr;
     uint32_t* first = nullptr;
     if (p.periodic) {
@@ -1439,7 +1452,10 @@ int upd_full(Upd* u, const vltr_ball* balls, size_t n) {
   u->st.changed.clear();
   u->st.affected.clear();
   u->st.was_full_reinit = true;
-  return upd_run(u, true, true, {}, {});
+  RunOpts o;
+  o.full = true;
+  o.refresh = true;
+  return upd_run(u, o);
 }
 
 // updateable.rs:422-447 with spheres_container.rs:71-182
@@ -1448,7 +1464,9 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
   int rc = upd_prepare(u);
   if (rc != VLTR_OK) return rc;
   auto& st = u->st;
-  if (nb != st.balls.size()) {
+  vltr_context* ctx = u->ctx;
+  // a changed ball count, or no complete last run to take neighbors from
+  if (nb != st.balls.size() || ctx->last_n != nb) {
     rc = upd_full(u, balls, nb);
     if (rc == VLTR_OK) *ok = 1;
     return rc;
@@ -1471,64 +1489,86 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
     if (rc == VLTR_OK) *ok = 1;
     return rc;
   }
-  // affected = changed (sorted, repeats kept) plus the changed balls' old
-  // neighbors; the reference inserts them one by one into a sorted vector and
-  // gives up (full reinit) when one more would pass the threshold, i.e. when
-  // |changed| + |new ids| > threshold (spheres_container.rs:105-124)
-  std::vector<uint8_t>& mark = u->mark;
-  mark.assign(n, 0);
-  bool repeats = false;  // a changed id listed twice stays twice until the final sort + unique
-  for (uint64_t id : changed) {
-    repeats = repeats || mark[id] != 0;
-    mark[id] = 3;  // 3 changed, 1 old neighbor, 2 new neighbor
+  // Affected marks on the device: 3 changed, 1 old colliding neighbor (the
+  // last run's lists), 2 new colliding neighbor (marked in the run). The
+  // changed ids and their new balls go up once, through pinned staging.
+  const size_t k = changed.size();
+  const size_t vals_off = (k * sizeof(int32_t) + 255) / 256 * 256;
+  const size_t mask_off = vals_off + (k * sizeof(vltr_ball) + 255) / 256 * 256;
+  if ((rc = upd_stage(u, mask_off + n)) != VLTR_OK) return rc;
+  unsigned char* stage = static_cast<unsigned char*>(u->stage.ptr);
+  int32_t* h_ids = reinterpret_cast<int32_t*>(stage);
+  vltr_ball* h_vals = reinterpret_cast<vltr_ball*>(stage + vals_off);
+  const uint8_t* h_mask = stage + mask_off;
+  for (size_t j = 0; j < k; ++j) {
+    h_ids[j] = static_cast<int32_t>(changed[j]);
+    h_vals[j] = balls[changed[j]];
   }
-  std::vector<std::vector<uint64_t>> old_rows;
-  rc = upd_rows(u, changed, &old_rows);
-  if (rc != VLTR_OK) return rc;
-  size_t n_affected = changed.size();
-  for (const auto& row : old_rows)
-    for (uint64_t cid : row)
-      if (!mark[cid]) {
-        mark[cid] = 1;
-        ++n_affected;
-      }
-  if (n_affected > threshold) {
+  hipStream_t s = ctx->stream;
+  VLTR_HIP(hipSetDevice(ctx->device));
+  VLTR_HIP(u->ids.ensure(k * sizeof(int32_t)));
+  VLTR_HIP(u->moved_vals.ensure(k * sizeof(vltr_ball)));
+  VLTR_HIP(u->affected_mask.ensure(n));
+  int32_t* d_ids = u->ids.as<int32_t>();
+  uint8_t* am = u->affected_mask.as<uint8_t>();
+  VLTR_HIP(hipMemcpyAsync(d_ids, h_ids, k * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  VLTR_HIP(hipMemcpyAsync(u->moved_vals.ptr, h_vals, k * sizeof(vltr_ball), hipMemcpyHostToDevice, s));
+  VLTR_HIP(launch_fill_u8(am, static_cast<int64_t>(n), 0, s));
+  VLTR_HIP(launch_mark_ids(am, static_cast<int64_t>(n), d_ids, static_cast<int64_t>(k), 3, s));
+  Pipeline last{};
+  last.n = static_cast<int64_t>(n);
+  last.nb = ctx->nb.as<int32_t>();
+  last.nb_cnt = ctx->nb_cnt.as<int32_t>();
+  last.nb_cap = ctx->last_nb_cap;
+  VLTR_HIP(launch_mark_neighbors(last, d_ids, static_cast<int64_t>(k), am, 1, s));
+  VLTR_HIP(hipMemcpyAsync(stage + mask_off, am, n, hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipStreamSynchronize(s));
+  // the reference inserts the old neighbors one by one into the sorted
+  // changed list and gives up (full reinit) when one more would pass the
+  // threshold, i.e. when |changed| + |new ids| > threshold
+  // (spheres_container.rs:105-124); a changed id listed twice stays twice
+  // until the final sort + unique
+  size_t n_old = 0, n_changed_unique = 0;
+  for (size_t i = 0; i < n; ++i) {
+    n_old += h_mask[i] == 1;
+    n_changed_unique += h_mask[i] == 3;
+  }
+  const bool repeats = n_changed_unique != k;
+  if (k + n_old > threshold) {
     rc = upd_full(u, balls, nb);
     if (rc == VLTR_OK) *ok = 1;
     return rc;
   }
-  // the sorted affected list: the marked ids in order (a linear scan of the
-  // marks, not a sort)
+  // the sorted affected list: the marked ids in order
   std::vector<uint64_t> affected;
-  affected.reserve(n_affected);
+  affected.reserve(k + n_old);
   if (repeats) {
     affected = changed;
     for (size_t i = 0; i < n; ++i)
-      if (mark[i] == 1) affected.push_back(i);
+      if (h_mask[i] == 1) affected.push_back(i);
     std::sort(affected.begin(), affected.end());
   } else {
     for (size_t i = 0; i < n; ++i)
-      if (mark[i]) affected.push_back(i);
+      if (h_mask[i]) affected.push_back(i);
   }
   for (uint64_t id : changed) st.balls[id] = balls[id];
-  rc = upd_run(u, false, true, affected, changed, &changed, true);
+  RunOpts o;
+  o.refresh = true;
+  o.incremental = true;
+  o.mask_ready = true;
+  o.n_moved = static_cast<int64_t>(k);
+  o.more = true;
+  o.mask_out = stage + mask_off;
+  rc = upd_run(u, o);
   if (rc != VLTR_OK) return rc;
-  // spheres_container.rs:150-170: new neighbors of the changed balls join
-  std::vector<std::vector<uint64_t>> new_rows;
-  rc = upd_rows(u, changed, &new_rows);
-  if (rc != VLTR_OK) return rc;
+  // spheres_container.rs:150-170: new neighbors of the changed balls join;
+  // the list is then sorted and without repeats
   bool more = false;
-  for (const auto& row : new_rows)
-    for (uint64_t cid : row)
-      if (!mark[cid]) {
-        mark[cid] = 2;
-        more = true;
-      }
+  for (size_t i = 0; i < n && !more; ++i) more = h_mask[i] == 2;
   if (more) {
-    // sorted and without repeats: the marked ids in order
     affected.clear();
     for (size_t i = 0; i < n; ++i)
-      if (mark[i]) affected.push_back(i);
+      if (h_mask[i]) affected.push_back(i);
   }
   st.changed = changed;
   st.affected = affected;
@@ -1558,8 +1598,10 @@ void vltr_updateable_destroy(vltr_updateable* u) {
     (void)hipSetDevice(u->ctx->device);
     (void)hipStreamSynchronize(u->ctx->stream);
     for (DevBuf* b : {&u->cells_state, &u->cells_backup, &u->excl, &u->excl_backup, &u->affected_mask, &u->ids, &u->rows,
-                      &u->rows_cnt, &u->scatter})
+                      &u->rows_cnt, &u->moved_vals, &u->types, &u->grad, &u->part, &u->topo, &u->seed, &u->dedup_keys,
+                      &u->dedup_first, &u->prev_pair_cnt, &u->prev_pair_b, &u->prev_rec, &u->prev_topo})
       b->release();
+    u->stage.release();
     vltr_context_destroy(u->ctx);
   }
   delete u;
@@ -1645,7 +1687,10 @@ int vltr_updateable_set_exclusion(vltr_updateable* u, size_t id, int excluded, i
   st.changed = {static_cast<uint64_t>(id)};
   st.affected = affected;
   st.was_full_reinit = false;
-  rc = upd_run(u, false, false, affected, {}, nullptr, true);
+  RunOpts o;
+  o.incremental = true;
+  o.aff = &affected;
+  rc = upd_run(u, o);
   if (rc != VLTR_OK) return rc;
   *ok = 1;
   return VLTR_OK;
@@ -1669,7 +1714,7 @@ int vltr_updateable_restore(vltr_updateable* u, int* ok) {
   const auto kept = u->st;
   // contacts are recomputed from the restored balls and exclusions; every
   // cell keeps its restored record
-  const int rc = upd_run(u, false, false, {}, {});
+  const int rc = upd_run(u, RunOpts{});
   if (rc != VLTR_OK) return rc;
   u->st.changed = kept.changed;
   u->st.affected = kept.affected;
@@ -1704,7 +1749,9 @@ int upd_energy(vltr_updateable* u) {
   if (u->n_types != u->st.balls.size()) return fail(VLTR_ERR_INVALID_ARGUMENT, "one type per ball required");
   if (!u->grad_valid && !u->st.balls.empty()) {
     const auto kept = u->st;
-    const int rc = upd_run(u, false, false, {}, {}, nullptr, true);
+    RunOpts o;
+    o.incremental = true;
+    const int rc = upd_run(u, o);
     if (rc != VLTR_OK) return rc;
     u->st.changed = kept.changed;
     u->st.affected = kept.affected;

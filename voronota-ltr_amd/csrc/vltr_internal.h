@@ -186,8 +186,13 @@ hipError_t launch_updateable_cells(const Pipeline& p, CellRec* state, uint8_t* e
 // balls[ids[k]] = vals[k], k < count.
 hipError_t launch_scatter_balls(vltr_ball* balls, const int32_t* ids, const vltr_ball* vals, int64_t count,
                                 hipStream_t s);
-// affected[nb % n] = 1 for every neighbor nb of the balls changed[0..k).
-hipError_t launch_mark_more(const Pipeline& p, const int32_t* changed, int64_t k, uint8_t* affected, hipStream_t s);
+// affected[nb % n] = value, where still 0, for every neighbor nb of the balls
+// changed[0..k) in p's neighbor lists.
+hipError_t launch_mark_neighbors(const Pipeline& p, const int32_t* changed, int64_t k, uint8_t* affected,
+                                 uint8_t value, hipStream_t s);
+// mask[ids[t]] = value, t < k (ids outside [0, n) skipped).
+hipError_t launch_mark_ids(uint8_t* mask, int64_t n, const int32_t* ids, int64_t k, uint8_t value, hipStream_t s);
+hipError_t launch_fill_u8(uint8_t* mask, int64_t n, uint8_t value, hipStream_t s);
 // mask[i] = 0 where where[i] != 0.
 hipError_t launch_mask_clear(uint8_t* mask, const uint8_t* where, int64_t n, hipStream_t s);
 // rows[k * cap + j] = neighbor j of ball ids[k] (cnt[k] entries).

@@ -184,6 +184,24 @@ int vltr_compute_tessellation_batched_device(vltr_context* ctx, const vltr_ball*
                                              size_t n_structs, double probe, int contacts_only);
 int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64_t* contact_off, uint64_t* cell_off);
 
+/* Launch shapes of the contact clip kernel and their autotuner. No reference
+ * counterpart: the tuning surface of this implementation. Every shape gives
+ * the same records; shapes differ in lanes per contact (8 or 16),
+ * wavefronts per block and occupancy target. vltr_clip_variant_name returns
+ * e.g. "g8w4o4" (NULL out of range); shape 0 is the default of a new
+ * context. */
+int vltr_clip_variant_count(void);
+const char* vltr_clip_variant_name(int variant);
+int vltr_context_set_clip_variant(vltr_context* ctx, int variant);
+int vltr_context_clip_variant(const vltr_context* ctx);
+/* Times every shape on a device-resident batch (arguments as
+ * vltr_compute_tessellation_batched_device; one structure is a batch of one):
+ * one warm-up run, then `repeats` runs whose mean pipeline time (HIP events,
+ * ms) goes to ms[variant] (max_entries entries, ms may be NULL). The fastest
+ * shape stays set on the context and is returned in *chosen. */
+int vltr_context_autotune(vltr_context* ctx, const vltr_ball* d_balls, const uint64_t* offsets, size_t n_structs,
+                          double probe, int contacts_only, int repeats, double* ms, int max_entries, int* chosen);
+
 /* ------------------------------------------------------------ adjoint --- */
 
 /* Energy E = sum_contacts W[t_a][t_b] * area + sas_weight * sum_cells sas_area

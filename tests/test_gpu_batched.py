@@ -85,3 +85,38 @@ def test_batched_device_offsets():
                                                  loff.ctypes.data_as(C.POINTER(C.c_uint64))), "batch offsets")
     assert list(np.diff(coff)) == [len(p.contacts) for p in parts]
     assert list(np.diff(loff)) == [len(p.cells) for p in parts]
+
+
+def test_clip_variants_give_identical_records(oracle):
+    """Every launch shape of the clip kernel gives the same records (the
+    autotuner only picks among equals); the autotuner keeps one of them set
+    on the context."""
+    from voronota_ltr import _native, workload
+
+    names = _native.clip_variants()
+    assert len(names) >= 2 and names[0] == "g8w1o4"
+    structs = [s for _, s, _ in _structures()] + [workload.rsa_sphere(20000, seed=20000)[0]]
+    before = _native.clip_variant()
+    try:
+        ref_parts = None
+        for v, name in enumerate(names):
+            print(f"\n[gpu] clip variant {name}", flush=True)
+            _native.set_clip_variant(v)
+            parts = _native.run_batched(structs, 1.4)
+            plain = _native.run(structs[-1], 1.4, pbox=[100, 0, 0, 0, 100, 0, 0, 0, 100])
+            if ref_parts is None:
+                ref_parts, ref_plain = parts, plain
+                for part, balls in zip(parts[-2:], structs[-2:]):
+                    compare(part, oracle.compute_tessellation(balls, 1.4), balls, 1.4)
+                continue
+            for a, b in zip(parts, ref_parts):
+                _same(a, b)
+            _same(plain, ref_plain)
+        print("[gpu] autotune", flush=True)
+        chosen, ms = _native.autotune(structs, 1.4, repeats=1)
+        assert chosen in names and set(ms) == set(names) and all(t > 0 for t in ms.values())
+        assert _native.clip_variant() == names.index(chosen)
+        with pytest.raises(_native.VltrError, match="clip variant out of range"):
+            _native.set_clip_variant(len(names))
+    finally:
+        _native.set_clip_variant(before)

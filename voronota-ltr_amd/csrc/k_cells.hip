@@ -32,7 +32,7 @@ struct Acc {
 };
 
 constexpr int kWave = 64;
-constexpr int kCellsBlock = 256;
+constexpr int kCellsBlock = 64;
 constexpr int kCellsWaves = kCellsBlock / kWave;
 
 __device__ __forceinline__ void wave_sync() {

@@ -45,25 +45,17 @@ double h_hex_s[5], h_hex_c[5];
 
 constexpr int kWave = 64;
 constexpr int kClipSlots = 64;  // neighbor slots handled by k_clip
-constexpr int kClipWaves = 4;   // wavefronts per k_clip block
-// Lanes per pair in k_clip (8: eight pairs side by side; contours beyond 8
-// points, about 1% of the clipped pairs, go to k_clip_wide's 16-lane groups).
-#ifndef VLTR_CLIP_GROUP
-#define VLTR_CLIP_GROUP 8
-#endif
-constexpr int kClipGroup = VLTR_CLIP_GROUP;
-using ClipGroup = WaveGroupT<kClipGroup>;
-using ClipGroupLds = GroupLdsT<2 * kClipGroup>;
-// Occupancy target of k_clip (wavefronts per SIMD). Its shared memory
-// (38.9 KB per 4-wavefront block) allows four blocks per CU; 4 caps registers
-// at 128 per lane, which spills some phase-1 state but measures faster than
-// 3 wavefronts with 168 registers (clip 3.20 vs 3.49 ms at 100k balls).
-#ifndef VLTR_CLIP_WAVES_PER_EU
-#define VLTR_CLIP_WAVES_PER_EU 4
-#endif
+// Launch shape of k_clip, picked per context by the autotuner among the
+// variants of kClipVariants: lanes per pair in phase 2 (8: eight pairs side
+// by side, contours beyond 8 points, about 1% of the clipped pairs, go to
+// k_clip_wide's 16-lane groups; 16: four pairs side by side), wavefronts per
+// block, and the occupancy target in wavefronts per SIMD (4 caps registers at
+// 128 per lane, which spills some phase-1 state; 3 allows 168). The shared
+// memory of a 4-wavefront block (38.9 KB) allows four blocks per CU.
 
 // Shared memory of one k_clip wavefront.
-struct ClipLds {
+template <int G>
+struct ClipLdsT {
   double pcx[kClipSlots], pcy[kClipSlots], pcz[kClipSlots];
   double pnx[kClipSlots], pny[kClipSlots], pnz[kClipSlots];
   double thr[kClipSlots];  // squared distance from a's centre to the plane point
@@ -72,7 +64,7 @@ struct ClipLds {
   // from the front (with their cutting masks), uncut pairs from the back
   int32_t item[kClipSlots];
   uint64_t mask[kClipSlots];
-  ClipGroupLds grp[kWave / kClipGroup];
+  GroupLdsT<2 * G> grp[kWave / G];
 };
 
 // Value of v in lane k (k uniform over the wavefront).
@@ -164,7 +156,11 @@ __device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
   return p.dirty[p.periodic ? id % p.n : id] != 0;
 }
 
-__global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_eu(VLTR_CLIP_WAVES_PER_EU, VLTR_CLIP_WAVES_PER_EU))) void k_clip(Pipeline p) {
+template <int kClipGroup, int kClipWaves, int kWavesPerEu>
+__global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu))) void k_clip(Pipeline p) {
+  using ClipGroup = WaveGroupT<kClipGroup>;
+  using ClipGroupLds = GroupLdsT<2 * kClipGroup>;
+  using ClipLds = ClipLdsT<kClipGroup>;
   __shared__ ClipLds lds[kClipWaves];
   const int wid = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
@@ -514,10 +510,35 @@ hipError_t upload_hexagon_constants(hipStream_t s) {
   return hipStreamSynchronize(s);
 }
 
+namespace {
+struct ClipVariant {
+  const char* name;
+  int waves;  // wavefronts per block
+  void (*kernel)(Pipeline);
+};
+// The first is the default: the fastest on the 100k-ball plain workload and
+// on the mixed batch of 200-20000-ball structures (pipeline 3.64 ms against
+// 3.94 for g8w2o4 and 4.31 for g8w4o4 at 100k). One wavefront per block lets
+// each ball's wavefront release its shared memory as soon as it ends.
+const ClipVariant kClipVariants[] = {
+    {"g8w1o4", 1, k_clip<8, 1, 4>},
+    {"g8w2o4", 2, k_clip<8, 2, 4>},
+    {"g8w4o4", 4, k_clip<8, 4, 4>},
+    {"g8w4o3", 4, k_clip<8, 4, 3>},
+    {"g16w2o4", 2, k_clip<16, 2, 4>},
+    {"g16w4o4", 4, k_clip<16, 4, 4>},
+};
+constexpr int kNumClipVariants = static_cast<int>(sizeof(kClipVariants) / sizeof(kClipVariants[0]));
+}  // namespace
+
+int clip_variants() { return kNumClipVariants; }
+const char* clip_variant_name(int v) { return v >= 0 && v < kNumClipVariants ? kClipVariants[v].name : nullptr; }
+
 hipError_t launch_clip(const Pipeline& p, hipStream_t s) {
   if (p.n <= 0 || p.n_pairs <= 0) return hipSuccess;
-  const int64_t blocks = (p.n + kClipWaves - 1) / kClipWaves;
-  hipLaunchKernelGGL(k_clip, dim3(blocks), dim3(kClipWaves * kWave), 0, s, p);
+  const ClipVariant& v = kClipVariants[p.clip_variant >= 0 && p.clip_variant < kNumClipVariants ? p.clip_variant : 0];
+  const int64_t blocks = (p.n + v.waves - 1) / v.waves;
+  hipLaunchKernelGGL(v.kernel, dim3(blocks), dim3(v.waves * kWave), 0, s, p);
   return hipGetLastError();
 }
 

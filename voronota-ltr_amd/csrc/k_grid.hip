@@ -17,7 +17,7 @@ namespace vltr {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kNbBlock = 256;                 // 4 wavefronts per block
+constexpr int kNbBlock = 64;                  // one wavefront per block
 constexpr int kNbWaves = kNbBlock / kWave;
 constexpr int kListCap = 512;                 // per-wave LDS candidate list
 

@@ -118,6 +118,7 @@ struct vltr_context {
   size_t last_structs = 0;  // structures of the last batched run (0: not batched)
   int last_nb_cap = 0;
   uint64_t counts[8] = {};
+  int clip_variant = 0;  // launch shape of k_clip (set by the autotuner or the caller)
 };
 
 struct vltr_solvent_result {
@@ -250,6 +251,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   p.seed = extras ? extras->seed : nullptr;
   p.sas_weight = extras ? extras->sas_weight : 1.0;
   p.volume_weight = extras ? extras->volume_weight : 1.0;
+  p.clip_variant = ctx->clip_variant;
 
   // One cell-hash segment per structure: its own bucket range, sized for it.
   std::vector<GridSeg> segs(static_cast<size_t>(p.n_segs));
@@ -868,6 +870,48 @@ int vltr_compute_tessellation_batched_device(vltr_context* ctx, const vltr_ball*
   if (rc != VLTR_OK) return rc;
   ctx->last_contacts = nc;
   ctx->last_cells = ncell;
+  return VLTR_OK;
+}
+
+int vltr_clip_variant_count(void) { return vltr::clip_variants(); }
+const char* vltr_clip_variant_name(int variant) { return vltr::clip_variant_name(variant); }
+int vltr_context_set_clip_variant(vltr_context* ctx, int variant) {
+  if (!ctx) return fail(VLTR_ERR_INVALID_ARGUMENT, "null context");
+  if (variant < 0 || variant >= vltr::clip_variants())
+    return fail(VLTR_ERR_INVALID_ARGUMENT, "clip variant out of range");
+  ctx->clip_variant = variant;
+  return VLTR_OK;
+}
+int vltr_context_clip_variant(const vltr_context* ctx) { return ctx ? ctx->clip_variant : -1; }
+
+int vltr_context_autotune(vltr_context* ctx, const vltr_ball* d_balls, const uint64_t* offsets, size_t n_structs,
+                          double probe, int contacts_only, int repeats, double* ms, int max_entries, int* chosen) {
+  if (!ctx || !chosen) return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  if (repeats < 1) repeats = 1;
+  const int nv = vltr::clip_variants();
+  const int before = ctx->clip_variant;
+  double best = 0.0;
+  int best_v = before;
+  for (int v = 0; v < nv; ++v) {
+    ctx->clip_variant = v;
+    double sum = 0.0;
+    for (int r = 0; r <= repeats; ++r) {
+      const int rc = vltr_compute_tessellation_batched_device(ctx, d_balls, offsets, n_structs, probe, contacts_only);
+      if (rc != VLTR_OK) {
+        ctx->clip_variant = before;
+        return rc;
+      }
+      if (r > 0) sum += ctx->timings[kPhases - 1];
+    }
+    const double mean = sum / repeats;
+    if (ms && v < max_entries) ms[v] = mean;
+    if (v == 0 || mean < best) {
+      best = mean;
+      best_v = v;
+    }
+  }
+  ctx->clip_variant = best_v;
+  *chosen = best_v;
   return VLTR_OK;
 }
 

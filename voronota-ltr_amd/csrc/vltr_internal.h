@@ -105,6 +105,8 @@ struct Pipeline {
   const int32_t* prev_pair_b;     // its pair partners
   const PairRec* prev_rec;        // its records
   const ContactTopo* prev_topo;   // its topologies (read when topo is set)
+
+  int32_t clip_variant;  // launch shape of k_clip (index into its variant table)
 };
 
 // tessellation.rs:323-328
@@ -134,6 +136,9 @@ int64_t scan_tmp_words(int64_t n);
 
 // k_contact.hip
 hipError_t launch_clip(const Pipeline& p, hipStream_t s);
+// Launch shapes of k_clip (same records from each): their number and names.
+int clip_variants();
+const char* clip_variant_name(int v);
 // k_clip_wide and k_cut_slow over the pairs k_clip listed (device-side
 // counts; no host synchronisation). p.slow_ws must hold slow_lanes(nb_cap)
 // regions.

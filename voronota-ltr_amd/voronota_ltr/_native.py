@@ -82,6 +82,12 @@ SIGNATURES = {
     "vltr_batch_cells": (_vp, [_vp, _sz, C.POINTER(_sz)]),
     "vltr_batch_free": (None, [_vp]),
     "vltr_compute_tessellation_batched_device": (C.c_int, [_vp, _vp, _u64p, _sz, C.c_double, C.c_int]),
+    "vltr_clip_variant_count": (C.c_int, []),
+    "vltr_clip_variant_name": (C.c_char_p, [C.c_int]),
+    "vltr_context_set_clip_variant": (C.c_int, [_vp, C.c_int]),
+    "vltr_context_clip_variant": (C.c_int, [_vp]),
+    "vltr_context_autotune": (C.c_int, [_vp, _vp, _u64p, _sz, C.c_double, C.c_int, C.c_int, _f64p, C.c_int,
+                                        C.POINTER(C.c_int)]),
     "vltr_context_batch_offsets": (C.c_int, [_vp, _sz, _u64p, _u64p]),
     "vltr_energy_gradient": (C.c_int, [_vp, _vp, _sz, C.c_double, C.POINTER(VltrPeriodicBox), _i32p, _f64p,
                                        C.c_double, C.c_double, _f64p, _f64p]),
@@ -322,6 +328,53 @@ def offsets_of(structures):
     off = np.zeros(len(sizes) + 1, dtype=np.uint64)
     off[1:] = np.cumsum(sizes)
     return off
+
+
+def clip_variants():
+    """Names of the clip kernel's launch shapes (index = shape)."""
+    lib = load_library()
+    return [lib.vltr_clip_variant_name(v).decode() for v in range(lib.vltr_clip_variant_count())]
+
+
+def set_clip_variant(variant, device=0):
+    lib = load_library()
+    check(lib.vltr_context_set_clip_variant(context(device), int(variant)), "vltr_context_set_clip_variant")
+
+
+def clip_variant(device=0):
+    return int(load_library().vltr_context_clip_variant(context(device)))
+
+
+def autotune(structures, probe, contacts_only=False, repeats=2, device=0, d_balls=None):
+    """Times every clip launch shape on the batch (a list of structures, or
+    one structure) and keeps the fastest on the device's context; returns
+    (name, {name: mean pipeline ms}). d_balls: optional device pointer that
+    already holds the concatenated balls."""
+    from . import _hip
+
+    lib = load_library()
+    ctx = context(device)
+    if len(structures) and np.ndim(structures[0]) == 1:
+        structures = [structures]
+    off = offsets_of(structures)
+    buf = None
+    if d_balls is None:
+        allb = np.ascontiguousarray(np.concatenate([_balls_array(s) for s in structures]))
+        buf = _hip.DeviceBuffer(allb.nbytes, device)
+        buf.upload(allb)
+        d_balls = buf.ptr
+    try:
+        nv = lib.vltr_clip_variant_count()
+        ms = np.zeros(nv)
+        chosen = C.c_int(0)
+        check(lib.vltr_context_autotune(ctx, d_balls, off.ctypes.data_as(_u64p), len(structures), float(probe),
+                                        int(bool(contacts_only)), int(repeats), ms.ctypes.data_as(_f64p), nv,
+                                        C.byref(chosen)), "vltr_context_autotune")
+    finally:
+        if buf is not None:
+            buf.free()
+    names = clip_variants()
+    return names[chosen.value], {names[v]: float(ms[v]) for v in range(nv)}
 
 
 def run_batched(structures, probe, contacts_only=False, device=0):

@@ -290,6 +290,31 @@ def batched_leg(args, lib, ctx):
     counts = _native.last_counts()
     grid = grid_build(_native.last_timings()["grid"], counts)
 
+    def graph_vs_direct(k_structs, reps):
+        """The first k_structs structures as one batch: direct runs against
+        replays of the captured HIP graph (same device buffer and layout)."""
+        sub = np.ascontiguousarray(off[:k_structs + 1])
+        subp = sub.ctypes.data_as(C.POINTER(C.c_uint64))
+        times = {}
+        for graphs in (False, True):
+            _native.set_graphs(graphs)
+            for _ in range(2):
+                _native.check(lib.vltr_compute_tessellation_batched_device(ctx, dbuf.ptr, subp, k_structs, args.probe,
+                                                                           0), "batched device")
+            _hip.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _native.check(lib.vltr_compute_tessellation_batched_device(ctx, dbuf.ptr, subp, k_structs, args.probe,
+                                                                           0), "batched device")
+            _hip.synchronize()
+            times[graphs] = (time.perf_counter() - t0) / reps
+        _native.set_graphs(False)
+        return {"structures": k_structs, "balls": int(sub[-1]), "direct_ms": round(times[False] * 1e3, 4),
+                "graph_replay_ms": round(times[True] * 1e3, 4),
+                "direct_over_graph": round(times[False] / times[True], 3)}
+
+    graphs = {"all": graph_vs_direct(len(sizes), reps), "first_16": graph_vs_direct(min(16, len(sizes)), 50)}
+
     base = dbuf.ptr.value
     for k in range(min(8, len(sizes))):  # warm the per-call path
         lib.vltr_compute_tessellation_device(ctx, C.c_void_p(base + int(off[k]) * 32), int(sizes[k]), args.probe,
@@ -306,6 +331,7 @@ def batched_leg(args, lib, ctx):
             "batched_s": round(t_batched, 4), "batched_balls_per_s": round(total / t_batched, 1),
             "plain_s": round(t_plain, 4), "plain_balls_per_s": round(total / t_plain, 1),
             "batched_over_plain": round(t_plain / t_batched, 3), "grid_build": grid, "autotune": tune,
+            "hip_graph": graphs,
             "note": "full compute_tessellation of every structure, balls resident in HBM; plain = one "
                     "vltr_compute_tessellation_device call per structure, batched = one "
                     "vltr_compute_tessellation_batched_device call"}

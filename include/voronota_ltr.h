@@ -184,6 +184,19 @@ int vltr_compute_tessellation_batched_device(vltr_context* ctx, const vltr_ball*
                                              size_t n_structs, double probe, int contacts_only);
 int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64_t* contact_off, uint64_t* cell_off);
 
+/* HIP-graph replays of the device-resident batched run. No reference
+ * counterpart. With graphs on, vltr_compute_tessellation_batched_device
+ * captures its whole pipeline in a HIP graph (pair buffers sized 25% above
+ * the pair count of a direct run) and replays it, one launch and one
+ * synchronisation per call, while later calls keep the same d_balls pointer,
+ * offsets, probe, mode and clip launch shape and no other run has used the
+ * context in between (the balls' values may change); otherwise, or when a
+ * replay finds more pairs or longer neighbor lists than the capture holds,
+ * it runs directly and captures again. Results are identical either way;
+ * vltr_context_last_timings then reports only the total. */
+int vltr_context_set_graphs(vltr_context* ctx, int enable);
+uint64_t vltr_context_graph_replays(const vltr_context* ctx);
+
 /* Launch shapes of the contact clip kernel and their autotuner. No reference
  * counterpart: the tuning surface of this implementation. Every shape gives
  * the same records; shapes differ in lanes per contact (8 or 16),

@@ -120,3 +120,83 @@ def test_clip_variants_give_identical_records(oracle):
             _native.set_clip_variant(len(names))
     finally:
         _native.set_clip_variant(before)
+
+
+def _device_outputs(lib, ctx, n_structs):
+    import ctypes as C
+
+    from voronota_ltr import _hip, _native
+
+    dc, nc, dl, nl = C.c_void_p(), C.c_size_t(), C.c_void_p(), C.c_size_t()
+    _native.check(lib.vltr_context_device_outputs(ctx, C.byref(dc), C.byref(nc), C.byref(dl), C.byref(nl)), "outputs")
+    con = np.zeros(nc.value, dtype=_native.CONTACT_DTYPE)
+    cel = np.zeros(nl.value, dtype=_native.CELL_DTYPE)
+    h = _hip._load()
+    if nc.value:
+        assert h.hipMemcpy(con.ctypes.data_as(C.c_void_p), dc, con.nbytes, _hip.D2H) == 0
+    if nl.value:
+        assert h.hipMemcpy(cel.ctypes.data_as(C.c_void_p), dl, cel.nbytes, _hip.D2H) == 0
+    coff = np.zeros(n_structs + 1, np.uint64)
+    loff = np.zeros(n_structs + 1, np.uint64)
+    _native.check(lib.vltr_context_batch_offsets(ctx, n_structs, coff.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                 loff.ctypes.data_as(C.POINTER(C.c_uint64))), "batch offsets")
+    return con, cel, coff, loff
+
+
+def test_graph_replays():
+    """The batched device run captured in a HIP graph: replays give exactly
+    the direct run's outputs, follow new ball values at the same address, and
+    a replay that finds more pairs than the capture holds falls back to a
+    direct run (then captures again)."""
+    import ctypes as C
+
+    from voronota_ltr import _hip, _native, workload
+
+    lib = _native.load_library()
+    ctx = _native.context(0)
+    sizes = (300, 1200, 50, 2000, 5000)
+
+    def batch(seed, scale=1.0):
+        structs = [workload.rsa_sphere(n, seed=seed + n)[0] for n in sizes]
+        for s in structs:
+            s[:, :3] *= scale
+        return np.ascontiguousarray(np.concatenate(structs))
+
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    offp = off.ctypes.data_as(C.POINTER(C.c_uint64))
+    first = batch(1)
+    buf = _hip.DeviceBuffer(first.nbytes)
+
+    def run(balls, graphs):
+        buf.upload(balls)
+        _native.set_graphs(graphs)
+        _native.check(lib.vltr_compute_tessellation_batched_device(ctx, buf.ptr, offp, len(sizes), 1.4, 0), "batched")
+        return _device_outputs(lib, ctx, len(sizes))
+
+    def same(a, b):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+    try:
+        for label, balls in (("first", first), ("second", batch(2)), ("denser", batch(3, scale=0.8))):
+            print(f"\n[gpu] graph replays: {label}", flush=True)
+            ref = run(balls, False)
+            assert len(ref[0]) > 0
+            r0 = _native.graph_replays()
+            for k in range(3):
+                same(run(balls, True), ref)
+            assert _native.graph_replays() - r0 == 3
+        # captured on a sparse batch, replayed on a denser one at the same
+        # address: about twice the pairs, past the capture's capacity
+        sparse, dense = batch(4), batch(3, scale=0.8)
+        ref_sparse, ref_dense = run(sparse, False), run(dense, False)
+        same(run(sparse, True), ref_sparse)
+        r0 = _native.graph_replays()
+        same(run(dense, True), ref_dense)  # overflow: direct run, no replay
+        assert _native.graph_replays() == r0
+        same(run(dense, True), ref_dense)  # captured again, replayed
+        assert _native.graph_replays() == r0 + 1
+    finally:
+        _native.set_graphs(False)
+        buf.free()

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
   const int64_t b = xcd_remap(blockIdx.x, gridDim.x) * kCellsWaves + w;
-  if (b >= p.n) return;  // wave-private LDS, no block barrier below
+  if (b >= p.n || pairs_overflowed(p)) return;  // wave-private LDS, no block barrier below
   const int cap = p.nb_cap;
   const int nb = p.nb_cnt[b];
   const int nin = min(static_cast<int>(p.in_cnt[b]), cap);
@@ -128,9 +128,14 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   if (p.seed != nullptr) p.seed[b] = seed;
 }
 
-__global__ void k_flag_contacts(const PairRec* rec, int64_t n, uint32_t* flag) {
+__global__ void k_flag_contacts(const PairRec* rec, int64_t n, const uint32_t* n_dev, uint32_t* flag) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
+  // graph replays: pairs past the device-side total hold no record of this run
+  if (n_dev != nullptr && i >= static_cast<int64_t>(*n_dev)) {
+    flag[i] = 0u;
+    return;
+  }
   flag[i] = (rec[i].flags & kRecValid) && rec[i].area > 0.0 ? 1u : 0u;
 }
 
@@ -196,8 +201,10 @@ __global__ void k_seg_offsets(Pipeline p, const uint32_t* pos, int what, uint32_
   const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (k > p.n_segs) return;
   const int64_t ball = k < p.n_segs ? p.segs[k].ball_off : p.n;
-  // contacts: the first pair of the structure's first ball
-  out[k] = what == 0 ? pos[p.pair_cnt[ball]] : pos[ball];
+  // contacts: the first pair of the structure's first ball (within the
+  // scanned range; an overflowed graph replay is rerun by the host)
+  const int64_t first = static_cast<int64_t>(p.pair_cnt[ball]);
+  out[k] = what == 0 ? pos[first < p.n_pairs ? first : p.n_pairs] : pos[ball];
 }
 
 // updateable.rs:633-727 with the container's exclusion statuses excl
@@ -280,7 +287,8 @@ hipError_t launch_cells(const Pipeline& p, hipStream_t s) {
 
 hipError_t launch_flag_contacts(const Pipeline& p, uint32_t* flag, hipStream_t s) {
   if (p.n_pairs > 0)
-    hipLaunchKernelGGL(k_flag_contacts, dim3(blocks_for(p.n_pairs, 256)), dim3(256), 0, s, p.rec, p.n_pairs, flag);
+    hipLaunchKernelGGL(k_flag_contacts, dim3(blocks_for(p.n_pairs, 256)), dim3(256), 0, s, p.rec, p.n_pairs,
+                       p.n_pairs_dev, flag);
   return hipGetLastError();
 }
 hipError_t launch_emit_contacts(const Pipeline& p, const uint32_t* pos, vltr_contact* out, hipStream_t s) {

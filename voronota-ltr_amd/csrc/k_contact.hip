@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
   const int lane = threadIdx.x & (kWave - 1);
   ClipLds& w = lds[wid];
   const int64_t q = xcd_remap(blockIdx.x, gridDim.x) * kClipWaves + wid;
-  if (q >= p.n) return;
+  if (q >= p.n || pairs_overflowed(p)) return;
   const int64_t a = p.periodic ? q : static_cast<int64_t>(p.sorted_id[q]);
   const int64_t base = p.pair_cnt[a];
   const int npairs = static_cast<int>(static_cast<int64_t>(p.pair_cnt[a + 1]) - base);

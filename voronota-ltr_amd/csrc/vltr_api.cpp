@@ -119,6 +119,18 @@ struct vltr_context {
   int last_nb_cap = 0;
   uint64_t counts[8] = {};
   int clip_variant = 0;  // launch shape of k_clip (set by the autotuner or the caller)
+  // Batched device runs captured in a HIP graph (vltr_context_set_graphs):
+  // replayed while the layout, input pointer, probe, mode, launch shape and
+  // buffers stay those of the capture.
+  bool use_graphs = false;
+  hipGraphExec_t graph = nullptr;
+  std::vector<uint64_t> graph_offsets;
+  const void* graph_src = nullptr;
+  double graph_probe = 0.0;
+  int graph_mode = 0, graph_variant = 0, graph_nb_cap = 0;
+  int64_t graph_cap = 0, graph_buckets = 0;
+  std::vector<const void*> graph_bufs;
+  uint64_t graph_replays = 0;
 };
 
 struct vltr_solvent_result {
@@ -226,10 +238,131 @@ struct RunExtras {
   const ContactTopo* prev_topo = nullptr;
 };
 
+// Capture of a batched run into a HIP graph (see run_batched): the pair
+// capacity it is captured with and the caller's device balls, which the
+// graph's first node copies into the context.
+struct GraphCapture {
+  int64_t cap = 0;
+  const void* src = nullptr;
+  size_t src_bytes = 0;
+  hipGraphExec_t exec = nullptr;  // out
+};
+
+void drop_graph(vltr_context* ctx);
+
+// Back-half buffers for `cap` pairs (pairs, records, fallback lists, flags,
+// compacted outputs) and the extras' per-pair buffers.
+int ensure_back(vltr_context* ctx, Pipeline& p, int64_t cap, int64_t n, const RunExtras* extras) {
+  p.pair_cap = std::max<int64_t>(cap, 1);
+  VLTR_HIP(ctx->pair_a.ensure(p.pair_cap * sizeof(int32_t)));
+  VLTR_HIP(ctx->pair_b.ensure(p.pair_cap * sizeof(int32_t)));
+  VLTR_HIP(ctx->rec.ensure(p.pair_cap * sizeof(PairRec)));
+  VLTR_HIP(ctx->overflow_list.ensure(p.pair_cap * sizeof(int32_t)));
+  VLTR_HIP(ctx->wide_list.ensure(p.pair_cap * sizeof(int32_t)));
+  p.slow_ws_pairs = slow_lanes(p.nb_cap);
+  {
+    const int64_t cc = 6 + p.nb_cap, cr = 2 * cc;
+    VLTR_HIP(ctx->slow_ws.ensure(static_cast<size_t>(p.slow_ws_pairs) * (5 * cc + 6 * cr) * sizeof(double)));
+  }
+  VLTR_HIP(ctx->flags.ensure((std::max<int64_t>(cap, n) + 2) * sizeof(uint32_t)));
+  VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(cap, n) + 64) * sizeof(uint32_t) * 2));
+  VLTR_HIP(ctx->out_contacts.ensure(p.pair_cap * sizeof(vltr_contact)));
+  VLTR_HIP(ctx->out_cells.ensure(std::max<int64_t>(n, 1) * sizeof(vltr_cell)));
+  fill_pointers(ctx, p);
+  if (extras && extras->topo_buf) {
+    VLTR_HIP(extras->topo_buf->ensure(p.pair_cap * sizeof(ContactTopo)));
+    p.topo = extras->topo_buf->as<ContactTopo>();
+  }
+  if (extras && extras->net_buf && extras->net_out) {
+    VLTR_HIP(extras->net_buf->ensure(p.pair_cap * sizeof(ContactNet)));
+    VLTR_HIP(extras->net_out->ensure(p.pair_cap * sizeof(ContactNet)));
+    p.net = extras->net_buf->as<ContactNet>();
+  }
+  return VLTR_OK;
+}
+
+// Front half: cell-list build and neighbor lists with their scanned pair
+// counts (pair_cnt[n] = the pair total).
+int enqueue_front(vltr_context* ctx, const Pipeline& p, bool events, hipStream_t s) {
+  VLTR_HIP(launch_prepare(p, s));
+  VLTR_HIP(launch_grid(p, s));
+  VLTR_HIP(launch_scan_u32(p.bucket_cnt, p.n_buckets, ctx->scan_tmp.as<uint32_t>(), s));
+  VLTR_HIP(launch_scatter(p, s));
+  if (events) VLTR_HIP(hipEventRecord(ctx->ev[1], s));
+  VLTR_HIP(launch_neighbors(p, s));
+  VLTR_HIP(launch_scan_u32(p.pair_cnt, p.n, ctx->scan_tmp.as<uint32_t>(), s));
+  return VLTR_OK;
+}
+
+// Back half: pairs, clip, cells, compaction of contacts and cells; the
+// contact and cell totals, the status word and the slow-path counts go to
+// host_words[3..7].
+int enqueue_back(vltr_context* ctx, Pipeline& p, const RunExtras* extras, bool batched, bool events,
+                 uint32_t* host_words, hipStream_t s) {
+  const int64_t n = p.n;
+  VLTR_HIP(launch_pairs(p, s));
+  if (extras && extras->pre_clip) VLTR_HIP(extras->pre_clip(p, s));
+  if (extras && extras->dirty) {
+    p.dirty = extras->dirty;
+    p.prev_pair_cnt = extras->prev_pair_cnt;
+    p.prev_pair_b = extras->prev_pair_b;
+    p.prev_rec = extras->prev_rec;
+    p.prev_topo = extras->prev_topo;
+  }
+  if (events) VLTR_HIP(hipEventRecord(ctx->ev[3], s));
+  VLTR_HIP(launch_clip(p, s));
+  if (events) VLTR_HIP(hipEventRecord(ctx->ev[4], s));
+  VLTR_HIP(launch_fallbacks(p, s));
+  if (events) VLTR_HIP(hipEventRecord(ctx->ev[5], s));
+  if (p.mode == 0) VLTR_HIP(launch_cells(p, s));
+  if (p.mode == 0 && extras && extras->post_cells) VLTR_HIP(extras->post_cells(p, s));
+  uint32_t* flags = ctx->flags.as<uint32_t>();
+  VLTR_HIP(launch_flag_contacts(p, flags, s));
+  VLTR_HIP(launch_scan_u32(flags, p.n_pairs, ctx->scan_tmp.as<uint32_t>(), s));
+  VLTR_HIP(launch_emit_contacts(p, flags, ctx->out_contacts.as<vltr_contact>(), s));
+  VLTR_HIP(hipMemcpyAsync(host_words + 3, flags + p.n_pairs, 4, hipMemcpyDeviceToHost, s));
+  uint32_t* seg_out = ctx->seg_out.as<uint32_t>();
+  if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 0, seg_out, s));
+  if (p.net != nullptr) VLTR_HIP(launch_emit_net(p, flags, extras->net_out->as<ContactNet>(), s));
+  if (p.mode == 0) {
+    VLTR_HIP(launch_flag_cells(p, flags, s));
+    VLTR_HIP(launch_scan_u32(flags, n, ctx->scan_tmp.as<uint32_t>(), s));
+    VLTR_HIP(launch_emit_cells(p, flags, ctx->out_cells.as<vltr_cell>(), s));
+    VLTR_HIP(hipMemcpyAsync(host_words + 4, flags + n, 4, hipMemcpyDeviceToHost, s));
+    if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 1, seg_out + p.n_segs + 1, s));
+  }
+  VLTR_HIP(hipMemcpyAsync(host_words + 5, p.status, 4, hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipMemcpyAsync(host_words + 6, p.overflow_cnt, 8, hipMemcpyDeviceToHost, s));
+  if (events) VLTR_HIP(hipEventRecord(ctx->ev[6], s));
+  return VLTR_OK;
+}
+
+// Bookkeeping of a completed run (direct or graph replay).
+void finish_run(vltr_context* ctx, const Pipeline& p, int64_t n_pairs, size_t n_structs, size_t* n_contacts,
+                size_t* n_cells) {
+  const uint32_t* host_words = static_cast<const uint32_t*>(ctx->pinned.ptr);
+  *n_contacts = host_words[3];
+  *n_cells = p.mode == 0 ? host_words[4] : 0;
+  ctx->last_n = static_cast<size_t>(p.n);
+  ctx->last_structs = n_structs;
+  ctx->last_nb_cap = p.nb_cap;
+  ctx->counts[0] = static_cast<uint64_t>(p.n);
+  ctx->counts[1] = static_cast<uint64_t>(p.n_pop);
+  ctx->counts[2] = static_cast<uint64_t>(n_pairs);
+  ctx->counts[3] = *n_contacts;
+  ctx->counts[4] = *n_cells;
+  ctx->counts[5] = static_cast<uint64_t>(host_words[6]) + host_words[7];
+  ctx->counts[6] = static_cast<uint64_t>(p.nb_cap);
+  ctx->counts[7] = static_cast<uint64_t>(p.n_buckets);
+}
+
 int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic_box* pbox, const int32_t* d_groups,
                  int64_t n_groups, int mode, size_t* n_contacts, size_t* n_cells, const uint64_t* seg_offsets = nullptr,
-                 int64_t n_structs = 0, const RunExtras* extras = nullptr) {
+                 int64_t n_structs = 0, const RunExtras* extras = nullptr, GraphCapture* gc = nullptr) {
   hipStream_t s = ctx->stream;
+  const bool capture = gc != nullptr;
+  // a direct run may move or resize the buffers a captured graph uses
+  if (!capture) drop_graph(ctx);
   const bool periodic = pbox != nullptr;
   const int64_t n_pop = periodic ? 27 * n : n;
   Pipeline p{};
@@ -286,9 +419,42 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->in_cnt.ensure(n * sizeof(uint32_t)));
   VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(p.n_buckets, 1) + n + 64) * sizeof(uint32_t) * 2));
   VLTR_HIP(ctx->pinned.ensure(64));
-  VLTR_HIP(hipMemcpyAsync(ctx->segs.ptr, segs.data(), segs.size() * sizeof(GridSeg), hipMemcpyHostToDevice, s));
-
   uint32_t* host_words = static_cast<uint32_t*>(ctx->pinned.ptr);
+
+  if (capture) {
+    // every buffer at its capacity first (no allocation under capture); the
+    // segment table is the one the direct run of this layout uploaded
+    p.nb_cap = ctx->nb_cap;
+    VLTR_HIP(ctx->nb.ensure(static_cast<size_t>(n) * p.nb_cap * sizeof(int32_t)));
+    VLTR_HIP(ctx->incoming.ensure(static_cast<size_t>(n) * p.nb_cap * sizeof(int32_t)));
+    VLTR_HIP(ctx->balls.ensure(gc->src_bytes));
+    fill_pointers(ctx, p);
+    int rc = ensure_back(ctx, p, gc->cap, n, extras);
+    if (rc != VLTR_OK) return rc;
+    p.n_pairs = p.pair_cap;
+    p.n_pairs_dev = p.pair_cnt + n;
+    VLTR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = [&]() -> int {
+      VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, gc->src, gc->src_bytes, hipMemcpyDeviceToDevice, s));
+      int r = enqueue_front(ctx, p, false, s);
+      if (r != VLTR_OK) return r;
+      VLTR_HIP(hipMemcpyAsync(host_words, p.pair_cnt + n, 4, hipMemcpyDeviceToHost, s));
+      return enqueue_back(ctx, p, extras, batched, false, host_words, s);
+    }();
+    hipGraph_t graph = nullptr;
+    const hipError_t e_end = hipStreamEndCapture(s, &graph);
+    if (rc != VLTR_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    VLTR_HIP(e_end);
+    const hipError_t e_inst = hipGraphInstantiate(&gc->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    VLTR_HIP(e_inst);
+    return VLTR_OK;
+  }
+
+  VLTR_HIP(hipMemcpyAsync(ctx->segs.ptr, segs.data(), segs.size() * sizeof(GridSeg), hipMemcpyHostToDevice, s));
  restart:
   VLTR_HIP(hipEventRecord(ctx->ev[0], s));
   for (;;) {
@@ -296,13 +462,8 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     VLTR_HIP(ctx->nb.ensure(static_cast<size_t>(n) * p.nb_cap * sizeof(int32_t)));
     VLTR_HIP(ctx->incoming.ensure(static_cast<size_t>(n) * p.nb_cap * sizeof(int32_t)));
     fill_pointers(ctx, p);
-    VLTR_HIP(launch_prepare(p, s));
-    VLTR_HIP(launch_grid(p, s));
-    VLTR_HIP(launch_scan_u32(p.bucket_cnt, p.n_buckets, ctx->scan_tmp.as<uint32_t>(), s));
-    VLTR_HIP(launch_scatter(p, s));
-    VLTR_HIP(hipEventRecord(ctx->ev[1], s));
-    VLTR_HIP(launch_neighbors(p, s));
-    VLTR_HIP(launch_scan_u32(p.pair_cnt, n, ctx->scan_tmp.as<uint32_t>(), s));
+    const int rc = enqueue_front(ctx, p, true, s);
+    if (rc != VLTR_OK) return rc;
     VLTR_HIP(hipMemcpyAsync(host_words, p.pair_cnt + n, 4, hipMemcpyDeviceToHost, s));
     VLTR_HIP(hipMemcpyAsync(host_words + 1, p.status, 4, hipMemcpyDeviceToHost, s));
     VLTR_HIP(hipEventRecord(ctx->ev[2], s));
@@ -316,66 +477,11 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     break;
   }
   const int64_t P = host_words[0];
+  int rc = ensure_back(ctx, p, P, n, extras);
+  if (rc != VLTR_OK) return rc;
   p.n_pairs = P;
-  p.pair_cap = std::max<int64_t>(P, 1);
-  VLTR_HIP(ctx->pair_a.ensure(p.pair_cap * sizeof(int32_t)));
-  VLTR_HIP(ctx->pair_b.ensure(p.pair_cap * sizeof(int32_t)));
-  VLTR_HIP(ctx->rec.ensure(p.pair_cap * sizeof(PairRec)));
-  VLTR_HIP(ctx->overflow_list.ensure(p.pair_cap * sizeof(int32_t)));
-  VLTR_HIP(ctx->wide_list.ensure(p.pair_cap * sizeof(int32_t)));
-  p.slow_ws_pairs = slow_lanes(p.nb_cap);
-  {
-    const int64_t cc = 6 + p.nb_cap, cr = 2 * cc;
-    VLTR_HIP(ctx->slow_ws.ensure(static_cast<size_t>(p.slow_ws_pairs) * (5 * cc + 6 * cr) * sizeof(double)));
-  }
-  VLTR_HIP(ctx->flags.ensure((std::max<int64_t>(P, n) + 2) * sizeof(uint32_t)));
-  VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(P, n) + 64) * sizeof(uint32_t) * 2));
-  VLTR_HIP(ctx->out_contacts.ensure(p.pair_cap * sizeof(vltr_contact)));
-  VLTR_HIP(ctx->out_cells.ensure(std::max<int64_t>(n, 1) * sizeof(vltr_cell)));
-  fill_pointers(ctx, p);
-  if (extras && extras->topo_buf) {
-    VLTR_HIP(extras->topo_buf->ensure(p.pair_cap * sizeof(ContactTopo)));
-    p.topo = extras->topo_buf->as<ContactTopo>();
-  }
-  if (extras && extras->net_buf && extras->net_out) {
-    VLTR_HIP(extras->net_buf->ensure(p.pair_cap * sizeof(ContactNet)));
-    VLTR_HIP(extras->net_out->ensure(p.pair_cap * sizeof(ContactNet)));
-    p.net = extras->net_buf->as<ContactNet>();
-  }
-  VLTR_HIP(launch_pairs(p, s));
-  if (extras && extras->pre_clip) VLTR_HIP(extras->pre_clip(p, s));
-  if (extras && extras->dirty) {
-    p.dirty = extras->dirty;
-    p.prev_pair_cnt = extras->prev_pair_cnt;
-    p.prev_pair_b = extras->prev_pair_b;
-    p.prev_rec = extras->prev_rec;
-    p.prev_topo = extras->prev_topo;
-  }
-  VLTR_HIP(hipEventRecord(ctx->ev[3], s));
-  VLTR_HIP(launch_clip(p, s));
-  VLTR_HIP(hipEventRecord(ctx->ev[4], s));
-  VLTR_HIP(launch_fallbacks(p, s));
-  VLTR_HIP(hipEventRecord(ctx->ev[5], s));
-  if (mode == 0) VLTR_HIP(launch_cells(p, s));
-  if (mode == 0 && extras && extras->post_cells) VLTR_HIP(extras->post_cells(p, s));
-  uint32_t* flags = ctx->flags.as<uint32_t>();
-  VLTR_HIP(launch_flag_contacts(p, flags, s));
-  VLTR_HIP(launch_scan_u32(flags, P, ctx->scan_tmp.as<uint32_t>(), s));
-  VLTR_HIP(launch_emit_contacts(p, flags, ctx->out_contacts.as<vltr_contact>(), s));
-  VLTR_HIP(hipMemcpyAsync(host_words + 3, flags + P, 4, hipMemcpyDeviceToHost, s));
-  uint32_t* seg_out = ctx->seg_out.as<uint32_t>();
-  if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 0, seg_out, s));
-  if (p.net != nullptr) VLTR_HIP(launch_emit_net(p, flags, extras->net_out->as<ContactNet>(), s));
-  if (mode == 0) {
-    VLTR_HIP(launch_flag_cells(p, flags, s));
-    VLTR_HIP(launch_scan_u32(flags, n, ctx->scan_tmp.as<uint32_t>(), s));
-    VLTR_HIP(launch_emit_cells(p, flags, ctx->out_cells.as<vltr_cell>(), s));
-    VLTR_HIP(hipMemcpyAsync(host_words + 4, flags + n, 4, hipMemcpyDeviceToHost, s));
-    if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 1, seg_out + p.n_segs + 1, s));
-  }
-  VLTR_HIP(hipMemcpyAsync(host_words + 5, p.status, 4, hipMemcpyDeviceToHost, s));
-  VLTR_HIP(hipMemcpyAsync(host_words + 6, p.overflow_cnt, 8, hipMemcpyDeviceToHost, s));
-  VLTR_HIP(hipEventRecord(ctx->ev[6], s));
+  rc = enqueue_back(ctx, p, extras, batched, true, host_words, s);
+  if (rc != VLTR_OK) return rc;
   VLTR_HIP(hipStreamSynchronize(s));
   if (host_words[5] & kStatNbOverflow) {
     // an incoming-pair list outgrew nb_cap (a hidden ball with many
@@ -384,19 +490,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     ctx->nb_cap *= 2;
     goto restart;
   }
-  *n_contacts = host_words[3];
-  *n_cells = mode == 0 ? host_words[4] : 0;
-  ctx->last_n = static_cast<size_t>(n);
-  ctx->last_structs = batched ? static_cast<size_t>(n_structs) : 0;
-  ctx->last_nb_cap = p.nb_cap;
-  ctx->counts[0] = static_cast<uint64_t>(n);
-  ctx->counts[1] = static_cast<uint64_t>(n_pop);
-  ctx->counts[2] = static_cast<uint64_t>(P);
-  ctx->counts[3] = *n_contacts;
-  ctx->counts[4] = *n_cells;
-  ctx->counts[5] = static_cast<uint64_t>(host_words[6]) + host_words[7];
-  ctx->counts[6] = static_cast<uint64_t>(p.nb_cap);
-  ctx->counts[7] = static_cast<uint64_t>(p.n_buckets);
+  finish_run(ctx, p, P, batched ? static_cast<size_t>(n_structs) : 0, n_contacts, n_cells);
   float ms = 0.0f;
   for (int k = 0; k < kPhases - 1; ++k) {
     if (hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess) ctx->timings[k] = ms;
@@ -780,6 +874,90 @@ int check_offsets(const uint64_t* offsets, size_t n_structs) {
 
 // Batched run on ctx->balls (already on the device); per-structure offsets of
 // the outputs are left in ctx->seg_out.
+void drop_graph(vltr_context* ctx) {
+  if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+  ctx->graph = nullptr;
+}
+
+// Addresses of every buffer a captured pipeline reads or writes.
+std::vector<const void*> graph_buffers(const vltr_context* ctx) {
+  std::vector<const void*> v;
+  for (const DevBuf* b : {&ctx->balls, &ctx->sph, &ctx->seg_of, &ctx->segs, &ctx->seg_out, &ctx->bkey, &ctx->brank,
+                          &ctx->bcnt, &ctx->sorted_id, &ctx->sorted_sph, &ctx->small, &ctx->nb, &ctx->nb_cnt,
+                          &ctx->nb_big, &ctx->excluded, &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec,
+                          &ctx->cell, &ctx->overflow_list, &ctx->wide_list, &ctx->slow_ws, &ctx->flags, &ctx->scan_tmp,
+                          &ctx->out_contacts, &ctx->out_cells, &ctx->in_cnt, &ctx->incoming})
+    v.push_back(b->ptr);
+  v.push_back(ctx->pinned.ptr);
+  return v;
+}
+
+int run_batched(vltr_context* ctx, const uint64_t* offsets, size_t n_structs, double probe, int mode, size_t* nc,
+                size_t* ncell);
+
+// Batched run on device balls through the context's HIP graph: a call whose
+// layout matches the capture replays it (one launch, no host synchronisation
+// inside); otherwise a direct run sizes everything, the pipeline is captured
+// with a pair capacity 25% above the pairs seen, and the new graph runs.
+int run_batched_graph(vltr_context* ctx, const vltr_ball* d_balls, const uint64_t* offsets, size_t n_structs,
+                      double probe, int mode, size_t* nc, size_t* ncell) {
+  const size_t n = static_cast<size_t>(offsets[n_structs]);
+  const size_t bytes = n * sizeof(vltr_ball);
+  hipStream_t s = ctx->stream;
+  const bool match = ctx->graph != nullptr && ctx->graph_src == d_balls && ctx->graph_probe == probe &&
+                     ctx->graph_mode == mode && ctx->graph_variant == ctx->clip_variant &&
+                     ctx->graph_nb_cap == ctx->nb_cap && ctx->graph_offsets.size() == n_structs + 1 &&
+                     std::equal(offsets, offsets + n_structs + 1, ctx->graph_offsets.begin()) &&
+                     ctx->graph_bufs == graph_buffers(ctx);
+  if (!match) {
+    VLTR_HIP(ctx->balls.ensure(bytes));
+    VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, d_balls, bytes, hipMemcpyDeviceToDevice, s));
+    int rc = run_batched(ctx, offsets, n_structs, probe, mode, nc, ncell);
+    if (rc != VLTR_OK) return rc;
+    GraphCapture gc;
+    const int64_t P = static_cast<int64_t>(ctx->counts[2]);
+    gc.cap = P + P / 4 + 4096;
+    gc.src = d_balls;
+    gc.src_bytes = bytes;
+    size_t unused_c = 0, unused_l = 0;
+    rc = run_pipeline(ctx, static_cast<int64_t>(n), probe, nullptr, nullptr, 0, mode, &unused_c, &unused_l, offsets,
+                      static_cast<int64_t>(n_structs), nullptr, &gc);
+    if (rc != VLTR_OK) return rc;
+    ctx->graph = gc.exec;
+    ctx->graph_offsets.assign(offsets, offsets + n_structs + 1);
+    ctx->graph_src = d_balls;
+    ctx->graph_probe = probe;
+    ctx->graph_mode = mode;
+    ctx->graph_variant = ctx->clip_variant;
+    ctx->graph_nb_cap = ctx->nb_cap;
+    ctx->graph_cap = gc.cap;
+    ctx->graph_buckets = static_cast<int64_t>(ctx->counts[7]);
+    ctx->graph_bufs = graph_buffers(ctx);
+  }
+  VLTR_HIP(hipEventRecord(ctx->ev[0], s));
+  VLTR_HIP(hipGraphLaunch(ctx->graph, s));
+  VLTR_HIP(hipEventRecord(ctx->ev[kPhases - 1], s));
+  VLTR_HIP(hipStreamSynchronize(s));
+  const uint32_t* hw = static_cast<const uint32_t*>(ctx->pinned.ptr);
+  if (static_cast<int64_t>(hw[0]) > ctx->graph_cap || (hw[5] & (kStatNbOverflow | kStatNbTooLong | kStatPairOverflow))) {
+    // more pairs or longer neighbor lists than the capture holds: a direct
+    // run (which drops the graph; the next call captures again)
+    VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, d_balls, bytes, hipMemcpyDeviceToDevice, s));
+    return run_batched(ctx, offsets, n_structs, probe, mode, nc, ncell);
+  }
+  Pipeline p{};
+  p.n = p.n_pop = static_cast<int64_t>(n);
+  p.mode = mode;
+  p.nb_cap = ctx->graph_nb_cap;
+  p.n_buckets = ctx->graph_buckets;
+  finish_run(ctx, p, hw[0], n_structs, nc, ncell);
+  float ms = 0.0f;
+  for (int k = 0; k < kPhases - 1; ++k) ctx->timings[k] = 0.0;
+  if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[kPhases - 1]) == hipSuccess) ctx->timings[kPhases - 1] = ms;
+  ++ctx->graph_replays;
+  return VLTR_OK;
+}
+
 int run_batched(vltr_context* ctx, const uint64_t* offsets, size_t n_structs, double probe, int mode, size_t* nc,
                 size_t* ncell) {
   const int64_t n = static_cast<int64_t>(offsets[n_structs]);
@@ -863,15 +1041,31 @@ int vltr_compute_tessellation_batched_device(vltr_context* ctx, const vltr_ball*
     return VLTR_OK;
   }
   VLTR_HIP(hipSetDevice(ctx->device));
-  VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
-  VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, d_balls, n * sizeof(vltr_ball), hipMemcpyDeviceToDevice, ctx->stream));
   size_t nc = 0, ncell = 0;
-  const int rc = run_batched(ctx, offsets, n_structs, probe, contacts_only ? 1 : 0, &nc, &ncell);
+  int rc = VLTR_OK;
+  if (ctx->use_graphs) {
+    rc = run_batched_graph(ctx, d_balls, offsets, n_structs, probe, contacts_only ? 1 : 0, &nc, &ncell);
+  } else {
+    VLTR_HIP(ctx->balls.ensure(n * sizeof(vltr_ball)));
+    VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, d_balls, n * sizeof(vltr_ball), hipMemcpyDeviceToDevice, ctx->stream));
+    rc = run_batched(ctx, offsets, n_structs, probe, contacts_only ? 1 : 0, &nc, &ncell);
+  }
   if (rc != VLTR_OK) return rc;
   ctx->last_contacts = nc;
   ctx->last_cells = ncell;
   return VLTR_OK;
 }
+
+int vltr_context_set_graphs(vltr_context* ctx, int enable) {
+  if (!ctx) return fail(VLTR_ERR_INVALID_ARGUMENT, "null context");
+  ctx->use_graphs = enable != 0;
+  if (!ctx->use_graphs) {
+    VLTR_HIP(hipSetDevice(ctx->device));
+    drop_graph(ctx);
+  }
+  return VLTR_OK;
+}
+uint64_t vltr_context_graph_replays(const vltr_context* ctx) { return ctx ? ctx->graph_replays : 0; }
 
 int vltr_clip_variant_count(void) { return vltr::clip_variants(); }
 const char* vltr_clip_variant_name(int variant) { return vltr::clip_variant_name(variant); }
@@ -1094,6 +1288,7 @@ void vltr_context_destroy(vltr_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
   for (DevBuf* b : {&ctx->balls, &ctx->groups, &ctx->sph, &ctx->seg_of, &ctx->segs, &ctx->bkey, &ctx->brank, &ctx->bcnt,
                     &ctx->sorted_id, &ctx->sorted_sph, &ctx->small, &ctx->nb, &ctx->nb_cnt, &ctx->excluded,
                     &ctx->pair_cnt, &ctx->pair_a, &ctx->pair_b, &ctx->rec, &ctx->cell, &ctx->overflow_list, &ctx->wide_list,

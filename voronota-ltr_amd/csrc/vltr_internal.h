@@ -107,7 +107,17 @@ struct Pipeline {
   const ContactTopo* prev_topo;   // its topologies (read when topo is set)
 
   int32_t clip_variant;  // launch shape of k_clip (index into its variant table)
+  // Graph replays: the pair buffers hold pair_cap pairs (a capacity from an
+  // earlier run), kernels over pairs run to n_pairs = pair_cap and stop at
+  // the device-side total *n_pairs_dev. Null in direct runs (n_pairs exact).
+  const uint32_t* n_pairs_dev;
 };
+
+// A graph replay whose pair total outgrew the capacity it was captured with:
+// the pair kernels do nothing and the host reruns without the graph.
+__device__ __forceinline__ bool pairs_overflowed(const Pipeline& p) {
+  return p.n_pairs_dev != nullptr && static_cast<int64_t>(*p.n_pairs_dev) > p.pair_cap;
+}
 
 // tessellation.rs:323-328
 __device__ __forceinline__ bool same_group(const Pipeline& p, int64_t a, int64_t b) {

@@ -82,6 +82,8 @@ SIGNATURES = {
     "vltr_batch_cells": (_vp, [_vp, _sz, C.POINTER(_sz)]),
     "vltr_batch_free": (None, [_vp]),
     "vltr_compute_tessellation_batched_device": (C.c_int, [_vp, _vp, _u64p, _sz, C.c_double, C.c_int]),
+    "vltr_context_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "vltr_context_graph_replays": (C.c_uint64, [_vp]),
     "vltr_clip_variant_count": (C.c_int, []),
     "vltr_clip_variant_name": (C.c_char_p, [C.c_int]),
     "vltr_context_set_clip_variant": (C.c_int, [_vp, C.c_int]),
@@ -328,6 +330,17 @@ def offsets_of(structures):
     off = np.zeros(len(sizes) + 1, dtype=np.uint64)
     off[1:] = np.cumsum(sizes)
     return off
+
+
+def set_graphs(enable, device=0):
+    """HIP-graph capture and replay of device-resident batched runs on the
+    device's context."""
+    lib = load_library()
+    check(lib.vltr_context_set_graphs(context(device), int(bool(enable))), "vltr_context_set_graphs")
+
+
+def graph_replays(device=0):
+    return int(load_library().vltr_context_graph_replays(context(device)))
 
 
 def clip_variants():

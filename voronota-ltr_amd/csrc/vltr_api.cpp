@@ -1467,6 +1467,7 @@ struct vltr_updateable {
   // cells and exclusion statuses of the current state and of the backup
   DevBuf cells_state, cells_backup, excl, excl_backup, affected_mask, ids, rows, rows_cnt, moved_vals;
   HostPinned stage;  // staging of moved balls and affected marks
+  std::vector<uint64_t> aff_scratch;  // the affected list under construction
   // energy and gradient computed in the pipeline pass of every state change
   bool energy_on = false, grad_valid = false;
   double energy = 0.0, sas_weight = 1.0, volume_weight = 1.0;
@@ -1532,6 +1533,20 @@ struct RunOpts {
   bool more = false;         // the moved balls' new neighbors join the affected set (mark 2)
   uint8_t* mask_out = nullptr;  // pinned host copy of the final affected mask (n bytes)
 };
+
+// The ids i < n with mask[i] != 0, in order. Branch-free: about 40% of the
+// balls are marked in a dynamics step, which a data-dependent branch per id
+// mispredicts often enough to cost 100 us at 50k balls.
+void marked_ids(const uint8_t* mask, size_t n, std::vector<uint64_t>* out) {
+  out->resize(n);
+  uint64_t* o = out->data();
+  size_t c = 0;
+  for (size_t i = 0; i < n; ++i) {
+    o[c] = i;
+    c += mask[i] != 0;
+  }
+  out->resize(c);
+}
 
 // Pinned staging of at least `bytes`, after the stream is idle when it grows.
 int upd_stage(Upd* u, size_t bytes) {
@@ -1779,16 +1794,14 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
     return rc;
   }
   // the sorted affected list: the marked ids in order
-  std::vector<uint64_t> affected;
-  affected.reserve(k + n_old);
+  std::vector<uint64_t>& affected = u->aff_scratch;
   if (repeats) {
     affected = changed;
     for (size_t i = 0; i < n; ++i)
       if (h_mask[i] == 1) affected.push_back(i);
     std::sort(affected.begin(), affected.end());
   } else {
-    for (size_t i = 0; i < n; ++i)
-      if (h_mask[i]) affected.push_back(i);
+    marked_ids(h_mask, n, &affected);
   }
   for (uint64_t id : changed) st.balls[id] = balls[id];
   RunOpts o;
@@ -1804,13 +1817,9 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
   // the list is then sorted and without repeats
   bool more = false;
   for (size_t i = 0; i < n && !more; ++i) more = h_mask[i] == 2;
-  if (more) {
-    affected.clear();
-    for (size_t i = 0; i < n; ++i)
-      if (h_mask[i]) affected.push_back(i);
-  }
-  st.changed = changed;
-  st.affected = affected;
+  if (more) marked_ids(h_mask, n, &affected);
+  st.changed.swap(changed);
+  st.affected.swap(affected);  // the old list's storage is the next step's scratch
   st.was_full_reinit = false;
   *ok = 1;
   return VLTR_OK;

@@ -54,6 +54,36 @@ __device__ __forceinline__ double box_size(const unsigned long long* bits) {
 
 __device__ __forceinline__ int64_t xcd_block() { return xcd_remap(blockIdx.x, gridDim.x); }
 
+// Doubles as 64-bit keys in their numeric order (negative values reversed),
+// for atomic min / max of coordinates.
+__device__ __forceinline__ unsigned long long ordered_key(double v) {
+  const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+  return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+__device__ __forceinline__ double ordered_value(unsigned long long k) {
+  const unsigned long long u = (k >> 63) ? (k & ~(1ull << 63)) : ~k;
+  return __longlong_as_double(static_cast<long long>(u));
+}
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, kWave);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, kWave);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Bucket key of a periodic image that no central ball can touch: its centre
+// lies more than one box size (at least the largest 2R) outside the central
+// balls' bounding box on some axis. Such images stay out of the cell lists.
+constexpr uint32_t kCulled = 0xffffffffu;
+
 __device__ __forceinline__ int32_t seg_of_ball(const GridSeg* segs, int32_t n_segs, int64_t i) {
   int32_t lo = 0, hi = n_segs - 1;
   while (lo < hi) {
@@ -69,6 +99,10 @@ __device__ __forceinline__ int32_t seg_of_ball(const GridSeg* segs, int32_t n_se
 __global__ void k_init(Pipeline p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     *p.box_bits = static_cast<unsigned long long>(__double_as_longlong(1.0));
+    for (int d = 0; d < 3; ++d) {
+      p.bbox[d] = ~0ull;
+      p.bbox[3 + d] = 0ull;
+    }
     *p.status = 0u;
     *p.overflow_cnt = 0u;
     *p.wide_cnt = 0u;
@@ -83,15 +117,36 @@ __global__ void k_fill_u32(uint32_t* a, int64_t n, uint32_t v) {
 }
 
 // types.rs:58-63 (R = r + probe) and spheres_searcher.rs:74-77 (box size).
+// The box size (and, with periodic images, the central bounding box) is
+// reduced over the wavefront first: one atomic per wavefront.
 __global__ void k_prepare(Pipeline p) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (i >= p.n) return;
-  const double4 b = p.balls_in[i];
-  const double R = b.w + p.probe;
-  p.sph[i] = make_double4(b.x, b.y, b.z, R);
-  p.seg_of[i] = p.n_segs > 1 ? seg_of_ball(p.segs, p.n_segs, i) : 0;
-  const double bs = fma(R, 2.0, 0.25);
-  if (bs > 0.0) atomicMax(p.box_bits, static_cast<unsigned long long>(__double_as_longlong(bs)));
+  if (i - (threadIdx.x & (kWave - 1)) >= p.n) return;  // whole wavefront past the end
+  unsigned long long bs_bits = 0ull, lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
+  if (i < p.n) {
+    const double4 b = p.balls_in[i];
+    const double R = b.w + p.probe;
+    p.sph[i] = make_double4(b.x, b.y, b.z, R);
+    p.seg_of[i] = p.n_segs > 1 ? seg_of_ball(p.segs, p.n_segs, i) : 0;
+    const double bs = fma(R, 2.0, 0.25);
+    if (bs > 0.0) bs_bits = static_cast<unsigned long long>(__double_as_longlong(bs));
+    lo[0] = hi[0] = ordered_key(b.x);
+    lo[1] = hi[1] = ordered_key(b.y);
+    lo[2] = hi[2] = ordered_key(b.z);
+  }
+  // positive doubles order as their bits
+  bs_bits = wave_max_u64(bs_bits);
+  const bool lead = (threadIdx.x & (kWave - 1)) == 0;
+  if (lead && bs_bits != 0ull) atomicMax(p.box_bits, bs_bits);
+  if (p.periodic) {
+    for (int d = 0; d < 3; ++d) {
+      const unsigned long long l = wave_min_u64(lo[d]), h = wave_max_u64(hi[d]);
+      if (lead) {
+        atomicMin(p.bbox + d, l);
+        atomicMax(p.bbox + 3 + d, h);
+      }
+    }
+  }
 }
 
 // types.rs:362-375: image g of ball i is sphere (g + 1) n + i.
@@ -117,8 +172,20 @@ __global__ void k_keys(Pipeline p) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= p.n_pop) return;
   const double box = box_size(p.box_bits);
+  const double4 s = p.sph[i];
+  if (p.periodic && i >= p.n) {
+    // an image farther than one box size outside the central bounding box
+    // on some axis is farther than R_i + R_j from every central ball
+    const bool far = s.x < ordered_value(p.bbox[0]) - box || s.x > ordered_value(p.bbox[3]) + box ||
+                     s.y < ordered_value(p.bbox[1]) - box || s.y > ordered_value(p.bbox[4]) + box ||
+                     s.z < ordered_value(p.bbox[2]) - box || s.z > ordered_value(p.bbox[5]) + box;
+    if (far) {
+      p.bucket_key[i] = kCulled;
+      return;
+    }
+  }
   int32_t cx, cy, cz;
-  cell_of(p.sph[i], box, &cx, &cy, &cz);
+  cell_of(s, box, &cx, &cy, &cz);
   const GridSeg sg = p.segs[p.seg_of[i]];
   const uint32_t b = static_cast<uint32_t>(sg.tab_off) + morton(cx, cy, cz, sg.kbits);
   p.bucket_key[i] = b;
@@ -128,7 +195,9 @@ __global__ void k_keys(Pipeline p) {
 __global__ void k_scatter(Pipeline p) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= p.n_pop) return;
-  const uint32_t pos = p.bucket_cnt[p.bucket_key[i]] + p.bucket_rank[i];
+  const uint32_t key = p.bucket_key[i];
+  if (key == kCulled) return;
+  const uint32_t pos = p.bucket_cnt[key] + p.bucket_rank[i];
   p.sorted_id[pos] = static_cast<int32_t>(i);
   p.sorted_sph[pos] = p.sph[i];
 }

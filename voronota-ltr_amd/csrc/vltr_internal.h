@@ -69,6 +69,7 @@ struct Pipeline {
   int32_t* sorted_id;    // n_pop
   double4* sorted_sph;   // n_pop
   unsigned long long* box_bits;  // max over 2R + 0.25 as double bits
+  unsigned long long* bbox;      // 6: min x, y, z then max x, y, z of the central centres (order-preserving keys)
 
   int32_t* nb;           // n * nb_cap neighbor ids (sorted by key, id)
   int32_t* nb_cnt;       // n

@@ -1,7 +1,8 @@
 // Backward pass of the tessellation energy (adjoint.h): one 16-lane group per
-// contact, four per wavefront, over all candidate pairs of the last full
-// forward run (records, cut topologies and cell seeds written by k_clip,
-// k_clip_wide, k_cut_slow and k_cells). Gradients of the centres are added
+// contact, four per wavefront, over the contacts of the last full forward run
+// (records, cut topologies and cell seeds written by k_clip, k_clip_wide,
+// k_cut_slow and k_cells), listed first by a flag / scan / emit pass so that
+// no group idles on a candidate pair without a contact. Gradients of the centres are added
 // with FP64 atomics (image ids fold onto their ball). The same kernel sums
 // the energy: contact terms by the groups, cell terms by all lanes, one
 // partial per block.
@@ -95,14 +96,35 @@ __device__ bool dedup_first(const Pipeline& p, const DedupTable& T, int64_t t) {
   return T.first[slot] == static_cast<uint32_t>(t);
 }
 
+__global__ void k_zero_u32(uint32_t* a, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    a[i] = 0u;
+}
+
 __global__ void k_zero(double* a, int64_t n) {
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x)
     a[i] = 0.0;
 }
 
+// The contacts with a topology (valid record, positive area, m >= 0), listed
+// in pair order so that every group of the adjoint takes a real contact.
+__global__ void k_adj_flag(Pipeline p, uint32_t* flag) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= p.n_pairs) return;
+  const PairRec rec = p.rec[i];
+  flag[i] = (rec.flags & kRecValid) && rec.area > 0.0 && p.topo[i].m >= 0 ? 1u : 0u;
+}
+__global__ void k_adj_list(Pipeline p, const uint32_t* pos, int32_t* list) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= p.n_pairs || pos[i + 1] == pos[i]) return;
+  list[pos[i]] = static_cast<int32_t>(i);
+}
+
 __global__ __launch_bounds__(kAdjBlock) void k_adjoint(Pipeline p, const int32_t* types, W16 W, double* grad,
-                                                       double* part, DedupTable T) {
+                                                       double* part, DedupTable T, const int32_t* list,
+                                                       const uint32_t* count) {
   __shared__ AdjLds lds[kAdjGroups];
   __shared__ double red[kAdjBlock / 64];
   const int gl = threadIdx.x / kGroup;
@@ -112,11 +134,11 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(Pipeline p, const int32_t
   DevAcc acc{grad, p.n};
   double e = 0.0;
   const int64_t groups = static_cast<int64_t>(gridDim.x) * kAdjGroups;
-  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kAdjGroups + gl; t < p.n_pairs; t += groups) {
+  const int64_t n_list = *count;
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * kAdjGroups + gl; k < n_list; k += groups) {
+    const int64_t t = list[k];
     const PairRec rec = p.rec[t];
-    if (!(rec.flags & kRecValid) || !(rec.area > 0.0)) continue;
     const ContactTopo& topo = p.topo[t];
-    if (topo.m < 0) continue;
     const int32_t a = p.pair_a[t], b = p.pair_b[t];
     const int64_t ca = a % p.n, cb = b % p.n;
     const bool counted = !p.periodic || b < p.n || dedup_first(p, T, t);
@@ -153,17 +175,29 @@ uint64_t dedup_slots(int64_t n_pairs) {
 }
 
 hipError_t launch_adjoint(const Pipeline& p, const int32_t* types, const double* w16, double* grad, double* part,
-                          unsigned long long* dedup_keys, uint32_t* dedup_first, hipStream_t s) {
+                          unsigned long long* dedup_keys, uint32_t* dedup_first, const AdjointWork& work,
+                          hipStream_t s) {
   W16 W;
   for (int k = 0; k < 16; ++k) W.w[k] = w16[k];
   DedupTable T{dedup_keys, dedup_first, dedup_slots(p.n_pairs) - 1};
+  if (p.n_pairs > 0) {
+    hipLaunchKernelGGL(k_adj_flag, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, work.flags);
+    const hipError_t e = launch_scan_u32(work.flags, p.n_pairs, work.scan_tmp, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adj_list, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, work.flags, work.list);
+  }
   if (p.periodic) {
     hipLaunchKernelGGL(k_dedup_clear, dim3(1024), dim3(256), 0, s, T);
     if (p.n_pairs > 0)
       hipLaunchKernelGGL(k_dedup_insert, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, T);
   }
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, grad, 3 * p.n);
-  hipLaunchKernelGGL(k_adjoint, dim3(kAdjBlocks), dim3(kAdjBlock), 0, s, p, types, W, grad, part, T);
+  // no pairs: the scanned total flags[0] is not written; the list is empty
+  if (p.n_pairs <= 0) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, work.flags, 1);
+  }
+  hipLaunchKernelGGL(k_adjoint, dim3(kAdjBlocks), dim3(kAdjBlock), 0, s, p, types, W, grad, part, T, work.list,
+                     work.flags + (p.n_pairs > 0 ? p.n_pairs : 0));
   return hipGetLastError();
 }
 

@@ -110,7 +110,7 @@ struct vltr_context {
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
       in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad, net, net_out, solv_cnt, solv_comb, solv_parent,
       solv_sih, solv_ioff, solv_cur, solv_img, nb_big,
-      dedup_keys, dedup_first;
+      dedup_keys, dedup_first, adj_list;
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
@@ -1192,7 +1192,11 @@ int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, siz
       keys = ctx->dedup_keys.as<unsigned long long>();
       first = ctx->dedup_first.as<uint32_t>();
     }
-    return launch_adjoint(p, d_types, w.data(), d_grad, part, keys, first, st);
+    hipError_t e = ctx->adj_list.ensure(std::max<int64_t>(p.n_pairs, 1) * sizeof(int32_t));
+    if (e != hipSuccess) return e;
+    // the contact flags and their scan scratch are free until the contacts are compacted
+    const AdjointWork work{ctx->flags.as<uint32_t>(), ctx->scan_tmp.as<uint32_t>(), ctx->adj_list.as<int32_t>()};
+    return launch_adjoint(p, d_types, w.data(), d_grad, part, keys, first, work, st);
   };
   size_t nc = 0, ncell = 0;
   const int rc =
@@ -1297,7 +1301,7 @@ void vltr_context_destroy(vltr_context* ctx) {
                     &ctx->in_cnt, &ctx->incoming, &ctx->seg_out, &ctx->topo, &ctx->seed, &ctx->energy_part,
                     &ctx->types, &ctx->grad, &ctx->net, &ctx->net_out, &ctx->solv_cnt, &ctx->solv_comb,
                     &ctx->solv_parent, &ctx->solv_sih, &ctx->solv_ioff, &ctx->solv_cur, &ctx->solv_img,
-                    &ctx->nb_big, &ctx->dedup_keys, &ctx->dedup_first})
+                    &ctx->nb_big, &ctx->dedup_keys, &ctx->dedup_first, &ctx->adj_list})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)
@@ -1655,7 +1659,11 @@ int upd_run(Upd* u, const RunOpts& o) {
       keys = u->dedup_keys.as<unsigned long long>();
       first = u->dedup_first.as<uint32_t>();
     }
-    return launch_adjoint(p, d_types, w16, d_grad, d_part, keys, first, st2);
+    vltr_context* c = u->ctx;
+    if ((e = c->adj_list.ensure(std::max<int64_t>(p.n_pairs, 1) * sizeof(int32_t))) != hipSuccess) return e;
+    // the contact flags and their scan scratch are free until the contacts are compacted
+    const AdjointWork work{c->flags.as<uint32_t>(), c->scan_tmp.as<uint32_t>(), c->adj_list.as<int32_t>()};
+    return launch_adjoint(p, d_types, w16, d_grad, d_part, keys, first, work, st2);
   };
   size_t nc = 0, ncell = 0;
   const int rc = run_pipeline(ctx, static_cast<int64_t>(n), st.probe, st.periodic ? &st.pbox : nullptr, nullptr, 0, 0,

@@ -164,8 +164,17 @@ hipError_t upload_hexagon_constants(hipStream_t s);
 // per block into part[0 .. adjoint_blocks()).
 // Slots of the canonical-pair table (periodic runs): a power of two >= 2 n_pairs.
 uint64_t dedup_slots(int64_t n_pairs);
+// Scratch of the adjoint: flags (n_pairs + 1 words) and scan_tmp as for
+// launch_scan_u32 over n_pairs, list (n_pairs entries): the contacts the
+// adjoint visits.
+struct AdjointWork {
+  uint32_t* flags;
+  uint32_t* scan_tmp;
+  int32_t* list;
+};
 hipError_t launch_adjoint(const Pipeline& p, const int32_t* types, const double* w16, double* grad, double* part,
-                          unsigned long long* dedup_keys, uint32_t* dedup_first, hipStream_t s);
+                          unsigned long long* dedup_keys, uint32_t* dedup_first, const AdjointWork& work,
+                          hipStream_t s);
 int64_t adjoint_blocks();
 
 // k_solvent.hip: emit == 0 counts the kept icosahedron samples (K unit

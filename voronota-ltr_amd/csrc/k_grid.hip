@@ -100,8 +100,8 @@ __global__ void k_init(Pipeline p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     *p.box_bits = static_cast<unsigned long long>(__double_as_longlong(1.0));
     for (int d = 0; d < 3; ++d) {
-      p.bbox[d] = ~0ull;
-      p.bbox[3 + d] = 0ull;
+      p.bbox[d * kBboxStride] = ~0ull;
+      p.bbox[(3 + d) * kBboxStride] = 0ull;
     }
     *p.status = 0u;
     *p.overflow_cnt = 0u;
@@ -142,8 +142,8 @@ __global__ void k_prepare(Pipeline p) {
     for (int d = 0; d < 3; ++d) {
       const unsigned long long l = wave_min_u64(lo[d]), h = wave_max_u64(hi[d]);
       if (lead) {
-        atomicMin(p.bbox + d, l);
-        atomicMax(p.bbox + 3 + d, h);
+        atomicMin(p.bbox + d * kBboxStride, l);
+        atomicMax(p.bbox + (3 + d) * kBboxStride, h);
       }
     }
   }
@@ -176,9 +176,11 @@ __global__ void k_keys(Pipeline p) {
   if (p.periodic && i >= p.n) {
     // an image farther than one box size outside the central bounding box
     // on some axis is farther than R_i + R_j from every central ball
-    const bool far = s.x < ordered_value(p.bbox[0]) - box || s.x > ordered_value(p.bbox[3]) + box ||
-                     s.y < ordered_value(p.bbox[1]) - box || s.y > ordered_value(p.bbox[4]) + box ||
-                     s.z < ordered_value(p.bbox[2]) - box || s.z > ordered_value(p.bbox[5]) + box;
+    const unsigned long long* bb = p.bbox;
+    const bool far = s.x < ordered_value(bb[0]) - box || s.x > ordered_value(bb[3 * kBboxStride]) + box ||
+                     s.y < ordered_value(bb[kBboxStride]) - box || s.y > ordered_value(bb[4 * kBboxStride]) + box ||
+                     s.z < ordered_value(bb[2 * kBboxStride]) - box ||
+                     s.z > ordered_value(bb[5 * kBboxStride]) + box;
     if (far) {
       p.bucket_key[i] = kCulled;
       return;

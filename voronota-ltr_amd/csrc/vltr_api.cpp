@@ -195,7 +195,7 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.overflow_cnt = reinterpret_cast<uint32_t*>(small + 12);
   p.wide_cnt = reinterpret_cast<uint32_t*>(small + 16);
   p.nb_big_cnt = reinterpret_cast<uint32_t*>(small + 20);
-  p.bbox = reinterpret_cast<unsigned long long*>(small + 24);
+  p.bbox = reinterpret_cast<unsigned long long*>(small + 256);  // 6 words, kBboxStride apart
   p.nb_big = ctx->nb_big.as<int32_t>();
   p.nb = ctx->nb.as<int32_t>();
   p.nb_cnt = ctx->nb_cnt.as<int32_t>();
@@ -411,7 +411,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->bcnt.ensure((p.n_buckets + 1) * sizeof(uint32_t)));
   VLTR_HIP(ctx->sorted_id.ensure(n_pop * sizeof(int32_t)));
   VLTR_HIP(ctx->sorted_sph.ensure(n_pop * sizeof(double4)));
-  VLTR_HIP(ctx->small.ensure(128));
+  VLTR_HIP(ctx->small.ensure(256 + 6 * kBboxStride * sizeof(unsigned long long)));
   VLTR_HIP(ctx->nb_cnt.ensure(n * sizeof(int32_t)));
   VLTR_HIP(ctx->nb_big.ensure(n * sizeof(int32_t)));
   VLTR_HIP(ctx->excluded.ensure(n));

@@ -39,6 +39,7 @@ constexpr uint32_t kStatPairOverflow = 2u;  // more pairs than the pair buffers 
 constexpr uint32_t kStatContourOverflow = 4u;
 constexpr uint32_t kStatNbTooLong = 8u;        // a ball has more than kNbMax neighbors
 constexpr int kNbMax = 4096;
+constexpr int kBboxStride = 32;  // 256 bytes
 
 // Launch parameters of the tessellation pipeline for one call (one or many
 // structures). All pointers are device pointers.
@@ -69,7 +70,8 @@ struct Pipeline {
   int32_t* sorted_id;    // n_pop
   double4* sorted_sph;   // n_pop
   unsigned long long* box_bits;  // max over 2R + 0.25 as double bits
-  unsigned long long* bbox;      // 6: min x, y, z then max x, y, z of the central centres (order-preserving keys)
+  unsigned long long* bbox;      // min x, y, z then max x, y, z of the central centres (order-preserving
+                                 // keys), one every kBboxStride words so their atomics hit different lines
 
   int32_t* nb;           // n * nb_cap neighbor ids (sorted by key, id)
   int32_t* nb_cnt;       // n

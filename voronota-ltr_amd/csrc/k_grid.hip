@@ -254,6 +254,12 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
     r_start[w][lane] = rs;
     r_pre[w][lane] = inc - rc;
   }
+  // the 27 cell prefixes as wavefront-uniform values: a candidate's cell is
+  // the number of later cells whose prefix it has reached (prefixes never
+  // decrease; empty cells repeat their successor's)
+  uint32_t pre_s[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) pre_s[k] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(inc - rc), k));
   __syncthreads();
 
   int cnt = 0;
@@ -264,8 +270,8 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
     NbEntry e{0.0, 0};
     if (j < total) {
       int t = 0;
-      for (int k = 1; k < 27; ++k)
-        if (r_pre[w][k] <= j) t = k;
+#pragma unroll
+      for (int k = 1; k < 27; ++k) t += pre_s[k] <= j ? 1 : 0;
       const uint32_t pos = r_start[w][t] + (j - r_pre[w][t]);
       const int32_t c = p.sorted_id[pos];
       if (c != a) {

@@ -13,11 +13,13 @@
 //     means the uncut full circle, finished one pair per lane afterwards;
 //     otherwise the pair and its 64-bit cutting mask go to the wavefront's
 //     list.
-//  2. Clipping. The wavefront splits into eight groups of 8 lanes; each group
-//     clips one listed pair at a time with group_clip (contour points in
-//     lanes, planes read from the shared table), eight pairs side by side.
+//  2. Clipping. The wavefront splits into groups of 8 (or 16) lanes; each
+//     group clips one listed pair at a time with group_clip (contour points in
+//     lanes, planes read from the shared table), eight (four) pairs side by
+//     side. The group size and the wavefronts per block are template
+//     parameters of k_clip (its launch shapes, chosen by the autotuner).
 // Pairs of balls with more than 64 neighbors, and pairs whose contour
-// outgrows an 8-lane group, go to k_clip_wide (16 lanes per pair); what
+// outgrows its lane group, go to k_clip_wide (16 lanes per pair); what
 // outgrows that goes to k_cut_slow: one lane per pair, the one-lane code of
 // contour.h with the contour in a global workspace. All paths produce
 // identical records.

@@ -1085,6 +1085,9 @@ int vltr_context_autotune(vltr_context* ctx, const vltr_ball* d_balls, const uin
   if (repeats < 1) repeats = 1;
   const int nv = vltr::clip_variants();
   const int before = ctx->clip_variant;
+  // direct runs only: each shape would otherwise be captured in a graph of its own
+  const bool graphs = ctx->use_graphs;
+  ctx->use_graphs = false;
   double best = 0.0;
   int best_v = before;
   for (int v = 0; v < nv; ++v) {
@@ -1094,6 +1097,7 @@ int vltr_context_autotune(vltr_context* ctx, const vltr_ball* d_balls, const uin
       const int rc = vltr_compute_tessellation_batched_device(ctx, d_balls, offsets, n_structs, probe, contacts_only);
       if (rc != VLTR_OK) {
         ctx->clip_variant = before;
+        ctx->use_graphs = graphs;
         return rc;
       }
       if (r > 0) sum += ctx->timings[kPhases - 1];
@@ -1106,6 +1110,7 @@ int vltr_context_autotune(vltr_context* ctx, const vltr_ball* d_balls, const uin
     }
   }
   ctx->clip_variant = best_v;
+  ctx->use_graphs = graphs;
   *chosen = best_v;
   return VLTR_OK;
 }

@@ -107,6 +107,22 @@ def test_periodic(vltr, oracle):
     _run(vltr, oracle, RING17, 1.0, pbox=corners_box((-1.6, -1.6, -0.6), (1.6, 1.6, 3.1)), name="ring17 periodic")
 
 
+def test_periodic_balls_outside_the_box(vltr, oracle):
+    """Balls need not lie in the box: a 3,000-ball cluster of a periodic box
+    (RSA at 0.056 balls/A^3) moved 1.5 box lengths away from the box and a
+    box smaller than the cluster's span; the images that can touch a ball are
+    those near the balls, wherever the box is."""
+    from voronota_ltr import workload
+
+    n = 3000
+    L = (n / 0.056) ** (1.0 / 3.0)
+    b, _ = workload.rsa_box(n, L, seed=3000)
+    shifted = b.copy()
+    shifted[:, :3] += 1.5 * L
+    _run(vltr, oracle, shifted, 1.4, pbox=corners_box((0, 0, 0), (L, L, L)), name="rsa box shifted")
+    _run(vltr, oracle, b, 1.4, pbox=corners_box((0, 0, 0), (0.9 * L, 0.95 * L, L)), name="rsa box, smaller box")
+
+
 def test_rsa_2500(vltr, oracle):
     from voronota_ltr import workload
 

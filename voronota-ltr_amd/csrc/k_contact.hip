@@ -55,11 +55,31 @@ constexpr int kClipSlots = 64;  // neighbor slots handled by k_clip
 // 128 per lane, which spills some phase-1 state; 3 allows 168). The shared
 // memory of a 4-wavefront block (38.9 KB) allows four blocks per CU.
 
+// Unit normals of the planes: kept in shared memory (NN) or recomputed each
+// time a plane is applied (the shared memory then allows four wavefronts per
+// SIMD).
+template <bool NN>
+struct UnitNormals {
+  __device__ void put(int, V3) {}
+  __device__ V3 get(int, V3 pn) const { return unit(pn); }
+};
+template <>
+struct UnitNormals<true> {
+  double x[kClipSlots], y[kClipSlots], z[kClipSlots];
+  __device__ void put(int k, V3 nn) {
+    x[k] = nn.x;
+    y[k] = nn.y;
+    z[k] = nn.z;
+  }
+  __device__ V3 get(int k, V3) const { return V3{x[k], y[k], z[k]}; }
+};
+
 // Shared memory of one k_clip wavefront.
-template <int G>
+template <int G, bool NN>
 struct ClipLdsT {
   double pcx[kClipSlots], pcy[kClipSlots], pcz[kClipSlots];
   double pnx[kClipSlots], pny[kClipSlots], pnz[kClipSlots];
+  UnitNormals<NN> nn;
   double thr[kClipSlots];  // squared distance from a's centre to the plane point
   int32_t cid[kClipSlots];
   // listed pairs: (pair offset within a << 8) | slot of b; pairs to clip
@@ -158,11 +178,11 @@ __device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
   return p.dirty[p.periodic ? id % p.n : id] != 0;
 }
 
-template <int kClipGroup, int kClipWaves, int kWavesPerEu>
+template <int kClipGroup, int kClipWaves, int kWavesPerEu, bool kKeepNormals>
 __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu))) void k_clip(Pipeline p) {
   using ClipGroup = WaveGroupT<kClipGroup>;
   using ClipGroupLds = GroupLdsT<2 * kClipGroup>;
-  using ClipLds = ClipLdsT<kClipGroup>;
+  using ClipLds = ClipLdsT<kClipGroup, kKeepNormals>;
   __shared__ ClipLds lds[kClipWaves];
   const int wid = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
@@ -220,6 +240,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     w.pnx[lane] = P.pn.x;
     w.pny[lane] = P.pn.y;
     w.pnz[lane] = P.pn.z;
+    if (kKeepNormals) w.nn.put(lane, unit(P.pn));
     w.thr[lane] = norm2(P.pc - A.c);
     w.cid[lane] = c0;
   }
@@ -298,14 +319,13 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
   const int grp = lane / kClipGroup;
   ClipGroup G{grp * kClipGroup, lane & (kClipGroup - 1)};
   ClipGroupLds* gs = &w.grp[grp];
-  // the unit normal is recomputed per applied plane: keeping it in shared
-  // memory would cost the fourth wavefront per SIMD
+  // unit normals: from shared memory, or recomputed per applied plane
   struct {
     const ClipLds& w;
     __device__ double thr(int s) const { return w.thr[s]; }
     __device__ CutPlane plane(int s) const {
       const V3 pn{w.pnx[s], w.pny[s], w.pnz[s]};
-      return CutPlane{V3{w.pcx[s], w.pcy[s], w.pcz[s]}, pn, unit(pn), w.cid[s]};
+      return CutPlane{V3{w.pcx[s], w.pcy[s], w.pcz[s]}, pn, w.nn.get(s, pn), w.cid[s]};
     }
   } fetch{w};
   for (int r0 = 0; r0 < n_clip; r0 += kWave / kClipGroup) {
@@ -523,12 +543,14 @@ struct ClipVariant {
 // 3.94 for g8w2o4 and 4.31 for g8w4o4 at 100k). One wavefront per block lets
 // each ball's wavefront release its shared memory as soon as it ends.
 const ClipVariant kClipVariants[] = {
-    {"g8w1o4", 1, k_clip<8, 1, 4>},
-    {"g8w2o4", 2, k_clip<8, 2, 4>},
-    {"g8w4o4", 4, k_clip<8, 4, 4>},
-    {"g8w4o3", 4, k_clip<8, 4, 3>},
-    {"g16w2o4", 2, k_clip<16, 2, 4>},
-    {"g16w4o4", 4, k_clip<16, 4, 4>},
+    {"g8w1o4", 1, k_clip<8, 1, 4, false>},
+    {"g8w2o4", 2, k_clip<8, 2, 4, false>},
+    {"g8w4o4", 4, k_clip<8, 4, 4, false>},
+    {"g8w4o3", 4, k_clip<8, 4, 3, false>},
+    {"g16w2o4", 2, k_clip<16, 2, 4, false>},
+    {"g16w4o4", 4, k_clip<16, 4, 4, false>},
+    {"g8w1o3n", 1, k_clip<8, 1, 3, true>},
+    {"g8w1o4n", 1, k_clip<8, 1, 4, true>},
 };
 constexpr int kNumClipVariants = static_cast<int>(sizeof(kClipVariants) / sizeof(kClipVariants[0]));
 }  // namespace

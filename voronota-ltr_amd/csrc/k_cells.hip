@@ -13,22 +13,10 @@
 namespace vltr {
 namespace {
 
+// The sums a cell accumulates over its contacts (types.rs:589-610).
 struct Acc {
   double area = 0.0, arc = 0.0, sa_pos = 0.0, sa_neg = 0.0, pv_pos = 0.0, pv_neg = 0.0;
   int added = 0;
-  // types.rs:589-610
-  __device__ __forceinline__ void add(const PairRec& r, bool as_first) {
-    if (!(r.flags & kRecValid) || !(r.area > 0.0)) return;
-    added = 1;
-    area = area + r.area;
-    arc = arc + r.arc_length;
-    const double sa = as_first ? r.solid_a : r.solid_b;
-    const double pv = as_first ? r.pyr_a : r.pyr_b;
-    sa_pos = sa_pos + fmax(sa, 0.0);
-    sa_neg = sa_neg - fmin(sa, 0.0);
-    pv_pos = pv_pos + fmax(pv, 0.0);
-    pv_neg = pv_neg - fmin(pv, 0.0);
-  }
 };
 
 constexpr int kWave = 64;
@@ -44,7 +32,20 @@ __device__ __forceinline__ void wave_sync() {
 // One wavefront per ball b. Its contributions arrive in the reference's order:
 // first the pairs (a, b) with a < b by ascending pair index (= ascending a),
 // then b's own pairs. Lanes fetch records in parallel into a 64-entry LDS
-// window in that order; lane 0 sums the window sequentially.
+// window in that order; then lanes 0-5 each keep one of the six sums
+// (types.rs:589-610: area, arc length, positive and negative solid angles,
+// positive and negative pyramid volumes) and add the window's contributions
+// to it in order, skipping records without a contact. Each sum sees the
+// reference's sequence of additions, so it is bit-identical; the six run side
+// by side instead of one after another.
+__device__ __forceinline__ double cell_term(const PairRec& r, bool as_first, int f) {
+  if (f == 0) return r.area;
+  if (f == 1) return r.arc_length;
+  const double v = f < 4 ? (as_first ? r.solid_a : r.solid_b) : (as_first ? r.pyr_a : r.pyr_b);
+  // x - fmin(v, 0) is x + (-fmin(v, 0)) exactly
+  return (f & 1) ? -fmin(v, 0.0) : fmax(v, 0.0);
+}
+
 __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   __shared__ PairRec win[kCellsWaves][kWave];
   const int lane = threadIdx.x & (kWave - 1);
@@ -55,7 +56,9 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   const int nb = p.nb_cnt[b];
   const int nin = min(static_cast<int>(p.in_cnt[b]), cap);
   const int32_t* inc = p.incoming + b * static_cast<int64_t>(cap);
-  Acc acc;
+  const int f = lane < 6 ? lane : 0;  // the sum this lane keeps (lanes 0-5)
+  double sum = 0.0;
+  bool added = false;
   for (int c0 = 0; c0 < nin; c0 += kWave) {
     for (int e = lane; e < nin; e += kWave) {
       const int32_t pe = inc[e];
@@ -64,23 +67,48 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
       if (rank >= c0 && rank < c0 + kWave) win[w][rank - c0] = p.rec[pe];
     }
     wave_sync();
-    if (lane == 0) {
-      const int m = min(kWave, nin - c0);
-      for (int k = 0; k < m; ++k) acc.add(win[w][k], false);
+    const int m = min(kWave, nin - c0);
+    bool valid = false;
+    if (lane < m) {
+      const PairRec& r = win[w][lane];
+      valid = (r.flags & kRecValid) && r.area > 0.0;
+    }
+    unsigned long long vm = __ballot(valid);
+    added = added || vm != 0ull;
+    while (vm != 0ull) {
+      const int k = __builtin_ctzll(vm);
+      vm &= vm - 1ull;
+      sum = sum + cell_term(win[w][k], false, f);
     }
     wave_sync();
   }
   const int64_t own0 = p.pair_cnt[b];
   const int nown = static_cast<int>(static_cast<int64_t>(p.pair_cnt[b + 1]) - own0);
   for (int c0 = 0; c0 < nown; c0 += kWave) {
-    if (c0 + lane < nown) win[w][lane] = p.rec[own0 + c0 + lane];
+    bool valid = false;
+    if (c0 + lane < nown) {
+      const PairRec r = p.rec[own0 + c0 + lane];
+      win[w][lane] = r;
+      valid = (r.flags & kRecValid) && r.area > 0.0;
+    }
     wave_sync();
-    if (lane == 0) {
-      const int m = min(kWave, nown - c0);
-      for (int k = 0; k < m; ++k) acc.add(win[w][k], true);
+    unsigned long long vm = __ballot(valid);
+    added = added || vm != 0ull;
+    while (vm != 0ull) {
+      const int k = __builtin_ctzll(vm);
+      vm &= vm - 1ull;
+      sum = sum + cell_term(win[w][k], true, f);
     }
     wave_sync();
   }
+  Acc acc;
+  acc.area = __shfl(sum, 0, kWave);
+  acc.arc = __shfl(sum, 1, kWave);
+  acc.sa_pos = __shfl(sum, 2, kWave);
+  acc.sa_neg = __shfl(sum, 3, kWave);
+  acc.pv_pos = __shfl(sum, 4, kWave);
+  acc.pv_neg = __shfl(sum, 5, kWave);
+  acc.added = added ? 1 : 0;
   if (lane != 0) return;
 
   CellRec out;

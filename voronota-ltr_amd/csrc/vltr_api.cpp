@@ -174,11 +174,6 @@ int kbits_for(int64_t n) {
   return k;
 }
 
-struct Run {
-  Pipeline p{};
-  int64_t n_pairs = 0;
-};
-
 void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.balls_in = ctx->balls.as<double4>();
   p.sph = ctx->sph.as<double4>();
@@ -982,9 +977,6 @@ int host_tessellate(vltr_context* ctx, const vltr_ball* balls, size_t n, double 
   return host_entry(ctx, balls, n, probe, pbox, groups, n_groups, with_vertices, 0, out, nets);
 }
 }  // namespace vltr
-
-namespace {
-}  // namespace
 
 extern "C" {
 

@@ -118,32 +118,45 @@ __global__ void k_fill_u32(uint32_t* a, int64_t n, uint32_t v) {
 
 // types.rs:58-63 (R = r + probe) and spheres_searcher.rs:74-77 (box size).
 // The box size (and, with periodic images, the central bounding box) is
-// reduced over the wavefront first: one atomic per wavefront.
-__global__ void k_prepare(Pipeline p) {
+// reduced over the wavefront, then over the block in shared memory: one
+// atomic per block and quantity (atomics on one address serialise at about
+// 13 ns each, so one per wavefront cost 20 us at 100k balls).
+constexpr int kPrepBlock = 1024;
+__global__ __launch_bounds__(kPrepBlock) void k_prepare(Pipeline p) {
+  __shared__ unsigned long long red[7][kPrepBlock / kWave];
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (i - (threadIdx.x & (kWave - 1)) >= p.n) return;  // whole wavefront past the end
-  unsigned long long bs_bits = 0ull, lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
+  unsigned long long v[7] = {0ull, ~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};  // box size, min x y z, max x y z
   if (i < p.n) {
     const double4 b = p.balls_in[i];
     const double R = b.w + p.probe;
     p.sph[i] = make_double4(b.x, b.y, b.z, R);
     p.seg_of[i] = p.n_segs > 1 ? seg_of_ball(p.segs, p.n_segs, i) : 0;
     const double bs = fma(R, 2.0, 0.25);
-    if (bs > 0.0) bs_bits = static_cast<unsigned long long>(__double_as_longlong(bs));
-    lo[0] = hi[0] = ordered_key(b.x);
-    lo[1] = hi[1] = ordered_key(b.y);
-    lo[2] = hi[2] = ordered_key(b.z);
+    if (bs > 0.0) v[0] = static_cast<unsigned long long>(__double_as_longlong(bs));  // positive: bits order
+    v[1] = v[4] = ordered_key(b.x);
+    v[2] = v[5] = ordered_key(b.y);
+    v[3] = v[6] = ordered_key(b.z);
   }
-  // positive doubles order as their bits
-  bs_bits = wave_max_u64(bs_bits);
-  const bool lead = (threadIdx.x & (kWave - 1)) == 0;
-  if (lead && bs_bits != 0ull) atomicMax(p.box_bits, bs_bits);
-  if (p.periodic) {
-    for (int d = 0; d < 3; ++d) {
-      const unsigned long long l = wave_min_u64(lo[d]), h = wave_max_u64(hi[d]);
-      if (lead) {
-        atomicMin(p.bbox + d * kBboxStride, l);
-        atomicMax(p.bbox + (3 + d) * kBboxStride, h);
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const int nq = p.periodic ? 7 : 1;
+  for (int q = 0; q < nq; ++q) {
+    const unsigned long long r = (q >= 1 && q <= 3) ? wave_min_u64(v[q]) : wave_max_u64(v[q]);
+    if (lane == 0) red[q][wv] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < nq; ++q) {
+      unsigned long long r = red[q][0];
+      for (int k = 1; k < kPrepBlock / kWave; ++k) {
+        const unsigned long long x = red[q][k];
+        r = (q >= 1 && q <= 3) ? (x < r ? x : r) : (x > r ? x : r);
+      }
+      if (q == 0) {
+        if (r != 0ull) atomicMax(p.box_bits, r);
+      } else if (q <= 3) {
+        atomicMin(p.bbox + (q - 1) * kBboxStride, r);
+      } else {
+        atomicMax(p.bbox + (q - 1) * kBboxStride, r);
       }
     }
   }
@@ -583,7 +596,7 @@ hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream
 
 hipError_t launch_prepare(const Pipeline& p, hipStream_t s) {
   hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, s, p);
-  if (p.n > 0) hipLaunchKernelGGL(k_prepare, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p);
+  if (p.n > 0) hipLaunchKernelGGL(k_prepare, dim3(blocks_for(p.n, kPrepBlock)), dim3(kPrepBlock), 0, s, p);
   if (p.periodic && p.n > 0) hipLaunchKernelGGL(k_populate, dim3(blocks_for(p.n * 26, 256)), dim3(256), 0, s, p);
   return hipGetLastError();
 }

@@ -1,6 +1,6 @@
 """Comparison of a GPU result with the oracle (test helper).
 
-Bar (matching the north-star contract):
+Bar (pairs and indices bit-exact, values to 1e-9 relative):
   * contact pair sequence and central flags: exact, in the same order;
   * cell index sequence: exact;
   * areas, arc lengths, SAS, volumes: |gpu - ref| <= 1e-9 * |ref| + floor,

@@ -168,10 +168,11 @@ def test_graph_replays():
     first = batch(1)
     buf = _hip.DeviceBuffer(first.nbytes)
 
-    def run(balls, graphs):
+    def run(balls, graphs, contacts_only=0):
         buf.upload(balls)
         _native.set_graphs(graphs)
-        _native.check(lib.vltr_compute_tessellation_batched_device(ctx, buf.ptr, offp, len(sizes), 1.4, 0), "batched")
+        _native.check(lib.vltr_compute_tessellation_batched_device(ctx, buf.ptr, offp, len(sizes), 1.4,
+                                                                   contacts_only), "batched")
         return _device_outputs(lib, ctx, len(sizes))
 
     def same(a, b):
@@ -197,6 +198,12 @@ def test_graph_replays():
         assert _native.graph_replays() == r0
         same(run(dense, True), ref_dense)  # captured again, replayed
         assert _native.graph_replays() == r0 + 1
+        # contacts only: no cells, its own capture
+        ref_co = run(first, False, contacts_only=1)
+        assert len(ref_co[0]) > 0 and len(ref_co[1]) == 0
+        for _ in range(2):
+            same(run(first, True, contacts_only=1), ref_co)
+        assert _native.graph_replays() == r0 + 3
     finally:
         _native.set_graphs(False)
         buf.free()

@@ -295,8 +295,7 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
             hidden = true;
           } else if (!contains(A, Cs)) {
             emit = true;
-            e.key = circle_center_distance(A, Cs);
-            e.id = c;
+            e.id = c;  // its sort key follows once the list is complete
           }
         }
       }
@@ -318,27 +317,43 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
     if (lane == 0) p.nb_big[atomicAdd(p.nb_big_cnt, 1u)] = static_cast<int32_t>(a);
     return;
   }
+  if (any_hidden) {
+    if (lane == 0) {
+      p.nb_cnt[a] = 0;
+      p.excluded[a] = 1;
+      p.pair_cnt[a] = 0u;
+    }
+    return;
+  }
+  // sort keys of the listed neighbors only (about one candidate in nine
+  // collides), every lane busy
+  for (int e = lane; e < stored; e += kWave) {
+    const int32_t c = list[w][e].id;
+    const double4 C4 = p.sph[c];
+    list[w][e].key = circle_center_distance(A, Sph{V3{C4.x, C4.y, C4.z}, C4.w});
+  }
+  // the wavefront's list is private to it: a wavefront-level barrier
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int cap = p.nb_cap;
   int32_t* out = p.nb + a * static_cast<int64_t>(cap);
   uint32_t pairs = 0;
-  if (!any_hidden) {
-    for (int e = lane; e < stored; e += kWave) {
-      const NbEntry me = list[w][e];
-      int rank = 0;
-      for (int k = 0; k < stored; ++k) rank += nb_less(list[w][k], me) ? 1 : 0;
-      if (rank < cap) {
-        out[rank] = me.id;
-        if (pair_qualifies(p, a, me.id)) pairs += 1;
-      }
+  for (int e = lane; e < stored; e += kWave) {
+    const NbEntry me = list[w][e];
+    int rank = 0;
+    for (int k = 0; k < stored; ++k) rank += nb_less(list[w][k], me) ? 1 : 0;
+    if (rank < cap) {
+      out[rank] = me.id;
+      if (pair_qualifies(p, a, me.id)) pairs += 1;
     }
   }
   for (int off = 32; off > 0; off >>= 1) pairs += __shfl_xor(pairs, off, kWave);
   if (lane == 0) {
-    const int kept = any_hidden ? 0 : (stored < cap ? stored : cap);
-    p.nb_cnt[a] = kept;
-    p.excluded[a] = any_hidden ? 1 : 0;
-    p.pair_cnt[a] = any_hidden ? 0u : pairs;
-    if (!any_hidden && cnt > cap) atomicOr(p.status, kStatNbOverflow);
+    p.nb_cnt[a] = stored < cap ? stored : cap;
+    p.excluded[a] = 0;
+    p.pair_cnt[a] = pairs;
+    if (cnt > cap) atomicOr(p.status, kStatNbOverflow);
   }
 }
 

@@ -275,20 +275,31 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
   for (int k = 0; k < 27; ++k) pre_s[k] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(inc - rc), k));
   __syncthreads();
 
+  // candidate j's id and sphere; the loop fetches the next chunk's before it
+  // tests the current one, so the loads overlap the tests
+  auto fetch = [&](uint32_t j, int32_t* c, double4* C4) {
+    if (j >= total) return;
+    int t = 0;
+#pragma unroll
+    for (int k = 1; k < 27; ++k) t += pre_s[k] <= j ? 1 : 0;
+    const uint32_t pos = r_start[w][t] + (j - r_pre[w][t]);
+    *c = p.sorted_id[pos];
+    *C4 = p.sorted_sph[pos];
+  };
+  int32_t c_next = 0;
+  double4 C_next = make_double4(0.0, 0.0, 0.0, 0.0);
+  fetch(lane, &c_next, &C_next);
   int cnt = 0;
   bool hidden = false;
   for (uint32_t base = 0; base < total; base += kWave) {
     const uint32_t j = base + lane;
+    const int32_t c = c_next;
+    const double4 C4 = C_next;
+    fetch(j + kWave, &c_next, &C_next);
     bool emit = false;
     NbEntry e{0.0, 0};
     if (j < total) {
-      int t = 0;
-#pragma unroll
-      for (int k = 1; k < 27; ++k) t += pre_s[k] <= j ? 1 : 0;
-      const uint32_t pos = r_start[w][t] + (j - r_pre[w][t]);
-      const int32_t c = p.sorted_id[pos];
       if (c != a) {
-        const double4 C4 = p.sorted_sph[pos];
         const Sph Cs{V3{C4.x, C4.y, C4.z}, C4.w};
         if (intersects(A, Cs)) {
           if (contains(Cs, A) && (!equals(Cs, A) || a > c)) {

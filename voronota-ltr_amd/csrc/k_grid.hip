@@ -496,8 +496,13 @@ __global__ void k_pair_fill(Pipeline p) {
   const int cnt = p.nb_cnt[a];
   const int32_t* lst = p.nb + a * static_cast<int64_t>(p.nb_cap);
   int64_t j = 0;
+  // list entries read four at a time (the rows are 16-byte aligned: nb_cap is
+  // a multiple of 4), so each lane has several loads in flight
+  const int4* lst4 = reinterpret_cast<const int4*>(lst);
+  int4 q = cnt > 0 ? lst4[0] : make_int4(0, 0, 0, 0);
   for (int k = 0; k < cnt; ++k) {
-    const int32_t c = lst[k];
+    if ((k & 3) == 0 && k > 0) q = lst4[k >> 2];
+    const int32_t c = (k & 3) == 0 ? q.x : (k & 3) == 1 ? q.y : (k & 3) == 2 ? q.z : q.w;
     if (!pair_qualifies(p, a, c)) continue;
     if (base + j < p.pair_cap) {
       p.pair_a[base + j] = static_cast<int32_t>(a);

@@ -59,15 +59,8 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   const int f = lane < 6 ? lane : 0;  // the sum this lane keeps (lanes 0-5)
   double sum = 0.0;
   bool added = false;
-  for (int c0 = 0; c0 < nin; c0 += kWave) {
-    for (int e = lane; e < nin; e += kWave) {
-      const int32_t pe = inc[e];
-      int rank = 0;
-      for (int k = 0; k < nin; ++k) rank += inc[k] < pe ? 1 : 0;
-      if (rank >= c0 && rank < c0 + kWave) win[w][rank - c0] = p.rec[pe];
-    }
-    wave_sync();
-    const int m = min(kWave, nin - c0);
+  // adds the window's records with a contact to this lane's sum, in order
+  auto add_window = [&](int m, bool as_first) {
     bool valid = false;
     if (lane < m) {
       const PairRec& r = win[w][lane];
@@ -78,27 +71,41 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
     while (vm != 0ull) {
       const int k = __builtin_ctzll(vm);
       vm &= vm - 1ull;
-      sum = sum + cell_term(win[w][k], false, f);
+      sum = sum + cell_term(win[w][k], as_first, f);
     }
-    wave_sync();
-  }
+  };
   const int64_t own0 = p.pair_cnt[b];
   const int nown = static_cast<int>(static_cast<int64_t>(p.pair_cnt[b + 1]) - own0);
-  for (int c0 = 0; c0 < nown; c0 += kWave) {
-    bool valid = false;
-    if (c0 + lane < nown) {
-      const PairRec r = p.rec[own0 + c0 + lane];
-      win[w][lane] = r;
-      valid = (r.flags & kRecValid) && r.area > 0.0;
-    }
+  // the first 64 own records are requested first: their loads overlap the
+  // incoming part
+  PairRec own_first{};
+  if (lane < nown) own_first = p.rec[own0 + lane];
+  if (nin <= kWave) {
+    // one entry per lane, ranked by comparisons with the others' (uniform)
+    const int32_t pe = lane < nin ? inc[lane] : 0;
+    int rank = 0;
+    for (int k = 0; k < nin; ++k) rank += __builtin_amdgcn_readlane(pe, k) < pe ? 1 : 0;
+    if (lane < nin) win[w][rank] = p.rec[pe];
     wave_sync();
-    unsigned long long vm = __ballot(valid);
-    added = added || vm != 0ull;
-    while (vm != 0ull) {
-      const int k = __builtin_ctzll(vm);
-      vm &= vm - 1ull;
-      sum = sum + cell_term(win[w][k], true, f);
+    add_window(nin, false);
+    wave_sync();
+  } else {
+    for (int c0 = 0; c0 < nin; c0 += kWave) {
+      for (int e = lane; e < nin; e += kWave) {
+        const int32_t pe = inc[e];
+        int rank = 0;
+        for (int k = 0; k < nin; ++k) rank += inc[k] < pe ? 1 : 0;
+        if (rank >= c0 && rank < c0 + kWave) win[w][rank - c0] = p.rec[pe];
+      }
+      wave_sync();
+      add_window(min(kWave, nin - c0), false);
+      wave_sync();
     }
+  }
+  for (int c0 = 0; c0 < nown; c0 += kWave) {
+    if (c0 + lane < nown) win[w][lane] = c0 == 0 ? own_first : p.rec[own0 + c0 + lane];
+    wave_sync();
+    add_window(min(kWave, nown - c0), true);
     wave_sync();
   }
   Acc acc;

@@ -187,13 +187,13 @@ int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64
 /* HIP-graph replays of the device-resident batched run. No reference
  * counterpart. With graphs on, vltr_compute_tessellation_batched_device
  * captures its whole pipeline in a HIP graph (pair buffers sized 25% above
- * the pair count of a direct run) and replays it, one launch and one
- * synchronisation per call, while later calls keep the same d_balls pointer,
- * offsets, probe, mode and clip launch shape and no other run has used the
- * context in between (the balls' values may change); otherwise, or when a
- * replay finds more pairs or longer neighbor lists than the capture holds,
- * it runs directly and captures again. Results are identical either way;
- * vltr_context_last_timings then reports only the total. */
+ * the pair count of a direct run) once a call repeats the previous call's
+ * d_balls pointer, offsets, probe, mode and clip launch shape, and replays
+ * it, one launch and one synchronisation per call, while later calls keep
+ * them and no other run has used the context in between (the balls' values
+ * may change). Other calls, and a replay that finds more pairs or longer
+ * neighbor lists than the capture holds, run directly. Results are identical
+ * either way; vltr_context_last_timings then reports only the total. */
 int vltr_context_set_graphs(vltr_context* ctx, int enable);
 uint64_t vltr_context_graph_replays(const vltr_context* ctx);
 

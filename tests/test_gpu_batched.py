@@ -187,7 +187,9 @@ def test_graph_replays():
             r0 = _native.graph_replays()
             for k in range(3):
                 same(run(balls, True), ref)
-            assert _native.graph_replays() - r0 == 3
+            # a layout seen for the first time runs directly; it is captured
+            # when it comes again
+            assert _native.graph_replays() - r0 >= 2
         # captured on a sparse batch, replayed on a denser one at the same
         # address: about twice the pairs, past the capture's capacity
         sparse, dense = batch(4), batch(3, scale=0.8)
@@ -201,9 +203,9 @@ def test_graph_replays():
         # contacts only: no cells, its own capture
         ref_co = run(first, False, contacts_only=1)
         assert len(ref_co[0]) > 0 and len(ref_co[1]) == 0
-        for _ in range(2):
+        for _ in range(3):
             same(run(first, True, contacts_only=1), ref_co)
-        assert _native.graph_replays() == r0 + 3
+        assert _native.graph_replays() == r0 + 3  # new layout: direct, then captured and replayed twice
     finally:
         _native.set_graphs(False)
         buf.free()

@@ -893,23 +893,34 @@ int run_batched(vltr_context* ctx, const uint64_t* offsets, size_t n_structs, do
 
 // Batched run on device balls through the context's HIP graph: a call whose
 // layout matches the capture replays it (one launch, no host synchronisation
-// inside); otherwise a direct run sizes everything, the pipeline is captured
-// with a pair capacity 25% above the pairs seen, and the new graph runs.
+// inside); a call that repeats the previous one's layout but has no graph yet
+// runs directly, is captured with a pair capacity 25% above the pairs seen,
+// and the new graph runs; any other call runs directly.
 int run_batched_graph(vltr_context* ctx, const vltr_ball* d_balls, const uint64_t* offsets, size_t n_structs,
                       double probe, int mode, size_t* nc, size_t* ncell) {
   const size_t n = static_cast<size_t>(offsets[n_structs]);
   const size_t bytes = n * sizeof(vltr_ball);
   hipStream_t s = ctx->stream;
-  const bool match = ctx->graph != nullptr && ctx->graph_src == d_balls && ctx->graph_probe == probe &&
-                     ctx->graph_mode == mode && ctx->graph_variant == ctx->clip_variant &&
-                     ctx->graph_nb_cap == ctx->nb_cap && ctx->graph_offsets.size() == n_structs + 1 &&
-                     std::equal(offsets, offsets + n_structs + 1, ctx->graph_offsets.begin()) &&
+  const bool same_call = ctx->graph_src == d_balls && ctx->graph_probe == probe && ctx->graph_mode == mode &&
+                         ctx->graph_variant == ctx->clip_variant && ctx->graph_offsets.size() == n_structs + 1 &&
+                         std::equal(offsets, offsets + n_structs + 1, ctx->graph_offsets.begin());
+  const bool match = ctx->graph != nullptr && same_call && ctx->graph_nb_cap == ctx->nb_cap &&
                      ctx->graph_bufs == graph_buffers(ctx);
   if (!match) {
     VLTR_HIP(ctx->balls.ensure(bytes));
     VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, d_balls, bytes, hipMemcpyDeviceToDevice, s));
     int rc = run_batched(ctx, offsets, n_structs, probe, mode, nc, ncell);
     if (rc != VLTR_OK) return rc;
+    if (!same_call) {
+      // a call not seen just before: run directly and capture only when the
+      // next call repeats it
+      ctx->graph_offsets.assign(offsets, offsets + n_structs + 1);
+      ctx->graph_src = d_balls;
+      ctx->graph_probe = probe;
+      ctx->graph_mode = mode;
+      ctx->graph_variant = ctx->clip_variant;
+      return VLTR_OK;
+    }
     GraphCapture gc;
     const int64_t P = static_cast<int64_t>(ctx->counts[2]);
     gc.cap = P + P / 4 + 4096;
@@ -920,11 +931,6 @@ int run_batched_graph(vltr_context* ctx, const vltr_ball* d_balls, const uint64_
                       static_cast<int64_t>(n_structs), nullptr, &gc);
     if (rc != VLTR_OK) return rc;
     ctx->graph = gc.exec;
-    ctx->graph_offsets.assign(offsets, offsets + n_structs + 1);
-    ctx->graph_src = d_balls;
-    ctx->graph_probe = probe;
-    ctx->graph_mode = mode;
-    ctx->graph_variant = ctx->clip_variant;
     ctx->graph_nb_cap = ctx->nb_cap;
     ctx->graph_cap = gc.cap;
     ctx->graph_buckets = static_cast<int64_t>(ctx->counts[7]);

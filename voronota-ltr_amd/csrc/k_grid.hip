@@ -110,6 +110,10 @@ __global__ void k_init(Pipeline p) {
   }
 }
 
+__global__ void k_copy_word(uint32_t* dst, const uint32_t* src) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
 __global__ void k_fill_u32(uint32_t* a, int64_t n, uint32_t v) {
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -636,6 +640,11 @@ hipError_t launch_gather_rows(const Pipeline& p, const int32_t* ids, int64_t k, 
                               hipStream_t s) {
   if (k > 0)
     hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(k * p.nb_cap, 256)), dim3(256), 0, s, p, ids, k, cnt, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_word(uint32_t* dst, const uint32_t* src, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_word, dim3(1), dim3(64), 0, s, dst, src);
   return hipGetLastError();
 }
 

@@ -186,10 +186,11 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.sorted_sph = ctx->sorted_sph.as<double4>();
   unsigned char* small = ctx->small.as<unsigned char>();
   p.box_bits = reinterpret_cast<unsigned long long*>(small);
+  // the report block: status, then the counters and totals (kReport*)
   p.status = reinterpret_cast<uint32_t*>(small + 8);
-  p.overflow_cnt = reinterpret_cast<uint32_t*>(small + 12);
-  p.wide_cnt = reinterpret_cast<uint32_t*>(small + 16);
-  p.nb_big_cnt = reinterpret_cast<uint32_t*>(small + 20);
+  p.overflow_cnt = p.status + kReportOverflow;
+  p.wide_cnt = p.status + kReportWide;
+  p.nb_big_cnt = p.status + kReportNbBig;
   p.bbox = reinterpret_cast<unsigned long long*>(small + 256);  // 6 words, kBboxStride apart
   p.nb_big = ctx->nb_big.as<int32_t>();
   p.nb = ctx->nb.as<int32_t>();
@@ -290,9 +291,9 @@ int enqueue_front(vltr_context* ctx, const Pipeline& p, bool events, hipStream_t
   return VLTR_OK;
 }
 
-// Back half: pairs, clip, cells, compaction of contacts and cells; the
-// contact and cell totals, the status word and the slow-path counts go to
-// host_words[3..7].
+// Back half: pairs, clip, cells, compaction of contacts and cells; the run's
+// report block (status, slow-path counts, pair, contact and cell totals) goes
+// to host_words in one transfer.
 int enqueue_back(vltr_context* ctx, Pipeline& p, const RunExtras* extras, bool batched, bool events,
                  uint32_t* host_words, hipStream_t s) {
   const int64_t n = p.n;
@@ -316,7 +317,7 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunExtras* extras, bool b
   VLTR_HIP(launch_flag_contacts(p, flags, s));
   VLTR_HIP(launch_scan_u32(flags, p.n_pairs, ctx->scan_tmp.as<uint32_t>(), s));
   VLTR_HIP(launch_emit_contacts(p, flags, ctx->out_contacts.as<vltr_contact>(), s));
-  VLTR_HIP(hipMemcpyAsync(host_words + 3, flags + p.n_pairs, 4, hipMemcpyDeviceToHost, s));
+  VLTR_HIP(launch_copy_word(p.status + kReportContacts, flags + p.n_pairs, s));
   uint32_t* seg_out = ctx->seg_out.as<uint32_t>();
   if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 0, seg_out, s));
   if (p.net != nullptr) VLTR_HIP(launch_emit_net(p, flags, extras->net_out->as<ContactNet>(), s));
@@ -324,11 +325,10 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunExtras* extras, bool b
     VLTR_HIP(launch_flag_cells(p, flags, s));
     VLTR_HIP(launch_scan_u32(flags, n, ctx->scan_tmp.as<uint32_t>(), s));
     VLTR_HIP(launch_emit_cells(p, flags, ctx->out_cells.as<vltr_cell>(), s));
-    VLTR_HIP(hipMemcpyAsync(host_words + 4, flags + n, 4, hipMemcpyDeviceToHost, s));
+    VLTR_HIP(launch_copy_word(p.status + kReportCells, flags + n, s));
     if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 1, seg_out + p.n_segs + 1, s));
   }
-  VLTR_HIP(hipMemcpyAsync(host_words + 5, p.status, 4, hipMemcpyDeviceToHost, s));
-  VLTR_HIP(hipMemcpyAsync(host_words + 6, p.overflow_cnt, 8, hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipMemcpyAsync(host_words, p.status, kReportWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   if (events) VLTR_HIP(hipEventRecord(ctx->ev[6], s));
   return VLTR_OK;
 }
@@ -337,8 +337,8 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunExtras* extras, bool b
 void finish_run(vltr_context* ctx, const Pipeline& p, int64_t n_pairs, size_t n_structs, size_t* n_contacts,
                 size_t* n_cells) {
   const uint32_t* host_words = static_cast<const uint32_t*>(ctx->pinned.ptr);
-  *n_contacts = host_words[3];
-  *n_cells = p.mode == 0 ? host_words[4] : 0;
+  *n_contacts = host_words[kReportContacts];
+  *n_cells = p.mode == 0 ? host_words[kReportCells] : 0;
   ctx->last_n = static_cast<size_t>(p.n);
   ctx->last_structs = n_structs;
   ctx->last_nb_cap = p.nb_cap;
@@ -347,7 +347,7 @@ void finish_run(vltr_context* ctx, const Pipeline& p, int64_t n_pairs, size_t n_
   ctx->counts[2] = static_cast<uint64_t>(n_pairs);
   ctx->counts[3] = *n_contacts;
   ctx->counts[4] = *n_cells;
-  ctx->counts[5] = static_cast<uint64_t>(host_words[6]) + host_words[7];
+  ctx->counts[5] = static_cast<uint64_t>(host_words[kReportOverflow]) + host_words[kReportWide];
   ctx->counts[6] = static_cast<uint64_t>(p.nb_cap);
   ctx->counts[7] = static_cast<uint64_t>(p.n_buckets);
 }
@@ -434,7 +434,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
       VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, gc->src, gc->src_bytes, hipMemcpyDeviceToDevice, s));
       int r = enqueue_front(ctx, p, false, s);
       if (r != VLTR_OK) return r;
-      VLTR_HIP(hipMemcpyAsync(host_words, p.pair_cnt + n, 4, hipMemcpyDeviceToHost, s));
+      VLTR_HIP(launch_copy_word(p.status + kReportPairs, p.pair_cnt + n, s));
       return enqueue_back(ctx, p, extras, batched, false, host_words, s);
     }();
     hipGraph_t graph = nullptr;
@@ -460,26 +460,27 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     fill_pointers(ctx, p);
     const int rc = enqueue_front(ctx, p, true, s);
     if (rc != VLTR_OK) return rc;
-    VLTR_HIP(hipMemcpyAsync(host_words, p.pair_cnt + n, 4, hipMemcpyDeviceToHost, s));
-    VLTR_HIP(hipMemcpyAsync(host_words + 1, p.status, 4, hipMemcpyDeviceToHost, s));
+    VLTR_HIP(launch_copy_word(p.status + kReportPairs, p.pair_cnt + n, s));
+    VLTR_HIP(hipMemcpyAsync(host_words, p.status, kReportWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     VLTR_HIP(hipEventRecord(ctx->ev[2], s));
     VLTR_HIP(hipStreamSynchronize(s));
-    if (host_words[1] & kStatNbTooLong) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
-    if (host_words[1] & kStatNbOverflow) {
+    if (host_words[kReportStatus] & kStatNbTooLong)
+      return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
+    if (host_words[kReportStatus] & kStatNbOverflow) {
       if (ctx->nb_cap >= kNbMax) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
       ctx->nb_cap *= 2;
       continue;
     }
     break;
   }
-  const int64_t P = host_words[0];
+  const int64_t P = host_words[kReportPairs];
   int rc = ensure_back(ctx, p, P, n, extras);
   if (rc != VLTR_OK) return rc;
   p.n_pairs = P;
   rc = enqueue_back(ctx, p, extras, batched, true, host_words, s);
   if (rc != VLTR_OK) return rc;
   VLTR_HIP(hipStreamSynchronize(s));
-  if (host_words[5] & kStatNbOverflow) {
+  if (host_words[kReportStatus] & kStatNbOverflow) {
     // an incoming-pair list outgrew nb_cap (a hidden ball with many
     // neighbors): redo everything with longer lists
     if (ctx->nb_cap >= kNbMax) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
@@ -941,7 +942,8 @@ int run_batched_graph(vltr_context* ctx, const vltr_ball* d_balls, const uint64_
   VLTR_HIP(hipEventRecord(ctx->ev[kPhases - 1], s));
   VLTR_HIP(hipStreamSynchronize(s));
   const uint32_t* hw = static_cast<const uint32_t*>(ctx->pinned.ptr);
-  if (static_cast<int64_t>(hw[0]) > ctx->graph_cap || (hw[5] & (kStatNbOverflow | kStatNbTooLong | kStatPairOverflow))) {
+  if (static_cast<int64_t>(hw[kReportPairs]) > ctx->graph_cap ||
+      (hw[kReportStatus] & (kStatNbOverflow | kStatNbTooLong | kStatPairOverflow))) {
     // more pairs or longer neighbor lists than the capture holds: a direct
     // run (which drops the graph; the next call captures again)
     VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, d_balls, bytes, hipMemcpyDeviceToDevice, s));
@@ -952,7 +954,7 @@ int run_batched_graph(vltr_context* ctx, const vltr_ball* d_balls, const uint64_
   p.mode = mode;
   p.nb_cap = ctx->graph_nb_cap;
   p.n_buckets = ctx->graph_buckets;
-  finish_run(ctx, p, hw[0], n_structs, nc, ncell);
+  finish_run(ctx, p, hw[kReportPairs], n_structs, nc, ncell);
   float ms = 0.0f;
   for (int k = 0; k < kPhases - 1; ++k) ctx->timings[k] = 0.0;
   if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[kPhases - 1]) == hipSuccess) ctx->timings[kPhases - 1] = ms;

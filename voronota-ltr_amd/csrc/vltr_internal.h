@@ -41,6 +41,20 @@ constexpr uint32_t kStatNbTooLong = 8u;        // a ball has more than kNbMax ne
 constexpr int kNbMax = 4096;
 constexpr int kBboxStride = 32;  // 256 bytes
 
+// The run's report block: consecutive device words from p.status, copied to
+// the host in one transfer. The totals are copied in by k_copy_word after the
+// scans that produce them.
+enum ReportWord : int {
+  kReportStatus = 0,
+  kReportOverflow = 1,  // pairs listed for the one-lane kernel
+  kReportWide = 2,      // pairs listed for the 16-lane kernel
+  kReportNbBig = 3,     // balls listed for the block-per-ball neighbor kernel
+  kReportPairs = 4,     // candidate pairs (the pair count scan's total)
+  kReportContacts = 5,  // contacts (the contact flag scan's total)
+  kReportCells = 6,     // cells (the cell flag scan's total)
+  kReportWords = 7
+};
+
 // Launch parameters of the tessellation pipeline for one call (one or many
 // structures). All pointers are device pointers.
 struct Pipeline {
@@ -81,7 +95,7 @@ struct Pipeline {
   int32_t* pair_b;       // pair_cap
   PairRec* rec;          // pair_cap
   CellRec* cell;         // n
-  uint32_t* status;      // 1 word
+  uint32_t* status;      // 1 word; the first of the run's report block (kReport* words)
   int32_t* overflow_list;  // pair_cap (pairs needing the slow kernel)
   uint32_t* overflow_cnt;  // 1
   int32_t* wide_list;      // pair_cap (pairs of balls with more than 64 neighbors)
@@ -145,6 +159,8 @@ hipError_t launch_neighbors(const Pipeline& p, hipStream_t s);
 hipError_t launch_pairs(const Pipeline& p, hipStream_t s);
 // Exclusive scan of counts[0..n) in place into counts[0..n], counts[n] = total.
 hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream_t s);
+// *dst = *src on the device (one word, stream-ordered).
+hipError_t launch_copy_word(uint32_t* dst, const uint32_t* src, hipStream_t s);
 int64_t scan_tmp_words(int64_t n);
 
 // k_contact.hip

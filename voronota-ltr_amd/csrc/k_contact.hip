@@ -244,6 +244,23 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     w.thr[lane] = norm2(P.pc - A.c);
     w.cid[lane] = c0;
   }
+  // the pair geometries, staged in the clip groups' shared memory (unused
+  // until phase 2) for uniform-address reads in the loop below
+  static_assert(sizeof(double) * 10 * kWave <= sizeof(w.grp), "pair geometry fits the clip groups' buffers");
+  double* pg = reinterpret_cast<double*>(&w.grp[0]);
+  if (lane < cnt) {
+    pg[0 * kWave + lane] = gk.ic.c.x;
+    pg[1 * kWave + lane] = gk.ic.c.y;
+    pg[2 * kWave + lane] = gk.ic.c.z;
+    pg[3 * kWave + lane] = gk.ic.r;
+    pg[4 * kWave + lane] = gk.ic_dir.x;
+    pg[5 * kWave + lane] = gk.ic_dir.y;
+    pg[6 * kWave + lane] = gk.ic_dir.z;
+    pg[7 * kWave + lane] = C.c.x;
+    pg[8 * kWave + lane] = C.c.y;
+    pg[9 * kWave + lane] = C.c.z;
+  }
+  wave_sync();
   const uint64_t ok_mask = __ballot(gk_ok);
   int n_clip = 0, n_uncut = 0, jp = 0;
   for (int k = 0; k < cnt; ++k) {
@@ -260,10 +277,9 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     }
     // classify reads the circle, its direction from a, and b
     PairGeom g;
-    g.ic = Sph{V3{readlane_d(gk.ic.c.x, k), readlane_d(gk.ic.c.y, k), readlane_d(gk.ic.c.z, k)},
-               readlane_d(gk.ic.r, k)};
-    g.ic_dir = V3{readlane_d(gk.ic_dir.x, k), readlane_d(gk.ic_dir.y, k), readlane_d(gk.ic_dir.z, k)};
-    g.b = Sph{V3{readlane_d(C.c.x, k), readlane_d(C.c.y, k), readlane_d(C.c.z, k)}, readlane_d(C.r, k)};
+    g.ic = Sph{V3{pg[0 * kWave + k], pg[1 * kWave + k], pg[2 * kWave + k]}, pg[3 * kWave + k]};
+    g.ic_dir = V3{pg[4 * kWave + k], pg[5 * kWave + k], pg[6 * kWave + k]};
+    g.b = Sph{V3{pg[7 * kWave + k], pg[8 * kWave + k], pg[9 * kWave + k]}, readlane_d(C.r, k)};
     const int cl = (lane < cnt && lane != k) ? classify(g, C, P, c_has_a) : kSkip;
     const bool kill = __ballot(cl == kKill) != 0ull;
     const uint64_t M = __ballot(cl == kCut);

@@ -554,11 +554,15 @@ struct ClipVariant {
   int waves;  // wavefronts per block
   void (*kernel)(Pipeline);
 };
-// The first is the default: the fastest on the 100k-ball plain workload and
-// on the mixed batch of 200-20000-ball structures (pipeline 3.64 ms against
-// 3.94 for g8w2o4 and 4.31 for g8w4o4 at 100k). One wavefront per block lets
-// each ball's wavefront release its shared memory as soon as it ends.
+// The first is the default: the fastest in the autotuner's passes on the
+// 100k-ball plain workload and on the mixed batch of 200-20000-ball
+// structures (pipeline 3.37 ms against 3.45 for g8w1o4, 3.66 for g8w2o4 and
+// 3.97 for g8w4o4 at 100k). One wavefront per block lets each ball's
+// wavefront release its shared memory as soon as it ends; keeping the unit
+// normals costs 1.5 KB of it (3.5 wavefronts per SIMD) and saves their
+// recomputation at every applied plane.
 const ClipVariant kClipVariants[] = {
+    {"g8w1o4n", 1, k_clip<8, 1, 4, true>},
     {"g8w1o4", 1, k_clip<8, 1, 4, false>},
     {"g8w2o4", 2, k_clip<8, 2, 4, false>},
     {"g8w4o4", 4, k_clip<8, 4, 4, false>},
@@ -566,7 +570,6 @@ const ClipVariant kClipVariants[] = {
     {"g16w2o4", 2, k_clip<16, 2, 4, false>},
     {"g16w4o4", 4, k_clip<16, 4, 4, false>},
     {"g8w1o3n", 1, k_clip<8, 1, 3, true>},
-    {"g8w1o4n", 1, k_clip<8, 1, 4, true>},
 };
 constexpr int kNumClipVariants = static_cast<int>(sizeof(kClipVariants) / sizeof(kClipVariants[0]));
 }  // namespace

@@ -122,11 +122,22 @@ __global__ void k_adj_list(Pipeline p, const uint32_t* pos, int32_t* list) {
   list[pos[i]] = static_cast<int32_t>(i);
 }
 
-__global__ __launch_bounds__(kAdjBlock) void k_adjoint(Pipeline p, const int32_t* types, W16 W, double* grad,
+// Occupancy target: the vector-Jacobian products want about 256 registers per
+// lane; two wavefronts per SIMD (with some spills) measure slightly faster
+// than one (20k balls: forward + adjoint 1.39 vs 1.41 ms), three slower (1.62).
+#ifndef VLTR_ADJ_WAVES_PER_EU
+#define VLTR_ADJ_WAVES_PER_EU 2
+#endif
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(VLTR_ADJ_WAVES_PER_EU))) void k_adjoint(Pipeline p, const int32_t* types, W16 W, double* grad,
                                                        double* part, DedupTable T, const int32_t* list,
                                                        const uint32_t* count) {
   __shared__ AdjLds lds[kAdjGroups];
   __shared__ double red[kAdjBlock / 64];
+  // W indexed by the contact's types: from shared memory (indexing the
+  // by-value argument would copy it to scratch memory)
+  __shared__ double ws[16];
+  if (threadIdx.x < 16) ws[threadIdx.x] = W.w[threadIdx.x];
+  __syncthreads();
   const int gl = threadIdx.x / kGroup;
   const int ln = threadIdx.x & (kGroup - 1);
   WaveGroup G{(gl % (64 / kGroup)) * kGroup, ln};
@@ -142,7 +153,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(Pipeline p, const int32_t
     const int32_t a = p.pair_a[t], b = p.pair_b[t];
     const int64_t ca = a % p.n, cb = b % p.n;
     const bool counted = !p.periodic || b < p.n || dedup_first(p, T, t);
-    const double w = counted ? W.w[4 * types[ca] + types[cb]] : 0.0;
+    const double w = counted ? ws[4 * types[ca] + types[cb]] : 0.0;
     const bool b_cell = b < p.n;  // an image end has no cell share
     if (ln == 0) e += w * rec.area;
     group_contact_adjoint(G, &lds[gl], sph4, a, b, topo, w, p.seed[ca], b_cell ? p.seed[cb] : 0.0,

@@ -78,8 +78,14 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   const int nown = static_cast<int>(static_cast<int64_t>(p.pair_cnt[b + 1]) - own0);
   // the first 64 own records are requested first: their loads overlap the
   // incoming part
-  PairRec own_first{};
-  if (lane < nown) own_first = p.rec[own0 + lane];
+  // (as 8-byte words: a PairRec local would live in scratch memory)
+  constexpr int kRecWords = static_cast<int>(sizeof(PairRec) / 8);
+  static_assert(sizeof(PairRec) % 8 == 0, "records are moved as 8-byte words");
+  uint64_t own_first[kRecWords] = {};
+  if (lane < nown) {
+#pragma unroll
+    for (int k = 0; k < kRecWords; ++k) own_first[k] = reinterpret_cast<const uint64_t*>(p.rec + own0 + lane)[k];
+  }
   if (nin <= kWave) {
     // one entry per lane, ranked by comparisons with the others' (uniform)
     const int32_t pe = lane < nin ? inc[lane] : 0;
@@ -103,7 +109,14 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
     }
   }
   for (int c0 = 0; c0 < nown; c0 += kWave) {
-    if (c0 + lane < nown) win[w][lane] = c0 == 0 ? own_first : p.rec[own0 + c0 + lane];
+    if (c0 == 0) {
+      if (lane < nown) {
+#pragma unroll
+        for (int k = 0; k < kRecWords; ++k) reinterpret_cast<uint64_t*>(&win[w][lane])[k] = own_first[k];
+      }
+    } else if (c0 + lane < nown) {
+      win[w][lane] = p.rec[own0 + c0 + lane];
+    }
     wave_sync();
     add_window(min(kWave, nown - c0), true);
     wave_sync();

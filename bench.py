@@ -31,8 +31,8 @@ FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes one full-mode k_clip launch moves on the default workload (100,000
 # balls, seed 100000): FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB, factors from a
-# 256 MiB device copy, launch shape g8w1o4; profiles/k_clip_100k_hbm_bytes.txt.
-K_CLIP_TRAFFIC_BYTES_100K = 554.7e6
+# 256 MiB device copy, launch shape g8w1o4n; profiles/k_clip_100k_hbm_bytes.txt.
+K_CLIP_TRAFFIC_BYTES_100K = 476.9e6
 
 
 def main():

@@ -117,6 +117,7 @@ def main():
     ms_co, ph_co, counts_co = timed(1)
 
     value = args.n / (ms_full / 1e3)
+    shape = tune["chosen"] if tune else _native.clip_variants()[0]
     cut_ms = ph_full["clip"]
     flops = FLOP_PER_PAIR * counts["pairs"]
     achieved = flops / (cut_ms / 1e3) / 1e12 if cut_ms > 0 else 0.0
@@ -168,13 +169,15 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": K_CLIP_TRAFFIC_BYTES_100K if (args.n, args.seed, args.probe) == (100000, 100000, 1.4) else None,
+            # measured for this workload with the default launch shape only
+            "traffic": K_CLIP_TRAFFIC_BYTES_100K
+            if (args.n, args.seed, args.probe) == (100000, 100000, 1.4) and shape == "g8w1o4n" else None,
             "traffic_unit": "bytes per launch",
             "note": "FP64 vector work (no matrix op on this path); flops = "
                     f"{FLOP_PER_PAIR} per candidate pair x {counts['pairs']} pairs",
         },
         "cpu_baseline": cpu,
-        "clip_launch_shape": tune["chosen"] if tune else _native.clip_variants()[0],
+        "clip_launch_shape": shape,
         "autotune": tune,
         "phases_ms": ph_full,
         "counts": counts,

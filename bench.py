@@ -30,9 +30,9 @@ FLOP_PER_PAIR = 2914.39
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes one full-mode k_clip launch moves on the default workload (100,000
-# balls, seed 100000): FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB, factors from a
-# 256 MiB device copy, launch shape g8w1o4n; profiles/k_clip_100k_hbm_bytes.txt.
-K_CLIP_TRAFFIC_BYTES_100K = 476.9e6
+# balls, seed 100000), per launch shape: FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB,
+# factors from a 256 MiB device copy; profiles/k_clip_100k_hbm_bytes.txt.
+K_CLIP_TRAFFIC_BYTES_100K = {"g8w1o4n": 476.4e6, "g8w1o4": 518.9e6}
 
 
 def main():
@@ -54,6 +54,8 @@ def main():
                     help="balls of the forward + adjoint leg (0 skips it)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default clip launch shape instead of timing every shape first")
+    ap.add_argument("--clip-shape", default=None,
+                    help="use this clip launch shape (a name from the library's table) instead of the autotuner's")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (reference kernel for the "
                          "HBM byte counters)")
@@ -111,13 +113,15 @@ def main():
         return wall / args.steps * 1e3, dict(zip(phases, (acc / args.steps).round(4).tolist())), _native.last_counts()
 
     tune = None
-    if not args.no_autotune:
+    if args.clip_shape is not None:
+        _native.set_clip_variant(_native.clip_variants().index(args.clip_shape))
+    elif not args.no_autotune:
         tune = autotune(lib, ctx, dbuf.ptr, np.array([0, args.n], dtype=np.uint64), args.probe)
     ms_full, ph_full, counts = timed(0)
     ms_co, ph_co, counts_co = timed(1)
 
     value = args.n / (ms_full / 1e3)
-    shape = tune["chosen"] if tune else _native.clip_variants()[0]
+    shape = _native.clip_variants()[_native.clip_variant()]
     cut_ms = ph_full["clip"]
     flops = FLOP_PER_PAIR * counts["pairs"]
     achieved = flops / (cut_ms / 1e3) / 1e12 if cut_ms > 0 else 0.0
@@ -169,9 +173,9 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            # measured for this workload with the default launch shape only
-            "traffic": K_CLIP_TRAFFIC_BYTES_100K
-            if (args.n, args.seed, args.probe) == (100000, 100000, 1.4) and shape == "g8w1o4n" else None,
+            # measured for this workload, per launch shape
+            "traffic": K_CLIP_TRAFFIC_BYTES_100K.get(shape)
+            if (args.n, args.seed, args.probe) == (100000, 100000, 1.4) else None,
             "traffic_unit": "bytes per launch",
             "note": "FP64 vector work (no matrix op on this path); flops = "
                     f"{FLOP_PER_PAIR} per candidate pair x {counts['pairs']} pairs",
@@ -281,7 +285,7 @@ def batched_leg(args, lib, ctx):
         _native.check(lib.vltr_compute_tessellation_batched_device(ctx, dbuf.ptr, offp, len(sizes), args.probe, 0),
                       "vltr_compute_tessellation_batched_device")
 
-    tune = None if args.no_autotune else autotune(lib, ctx, dbuf.ptr, off, args.probe)
+    tune = None if (args.no_autotune or args.clip_shape is not None) else autotune(lib, ctx, dbuf.ptr, off, args.probe)
     batched_once()
     _hip.synchronize()
     reps = 2

@@ -199,6 +199,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
   const bool incremental = p.dirty != nullptr;
   int64_t ob = 0;
   int on = 0;
+  int32_t prev_b = -1;
   if (incremental) {
     ob = p.prev_pair_cnt[a];
     on = static_cast<int>(static_cast<int64_t>(p.prev_pair_cnt[a + 1]) - ob);
@@ -206,6 +207,8 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
       keep_ball(p, base, npairs, ob, on, lane);
       return;
     }
+    // the previous partners, one per lane, for the kept pairs' look-ups
+    if (lane < on) prev_b = p.prev_pair_b[ob + lane];
   }
   if (cnt > kClipSlots) {
     for (int j = lane; j < npairs; j += kWave) {
@@ -268,7 +271,14 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     if (!pair_qualifies(p, a, b)) continue;
     const int jj = jp++;
     if (incremental && !is_dirty(p, b)) {
-      keep_pair_wave(p, prev_find_wave(p, ob, on, b, lane), base + jj, lane);
+      int64_t from;
+      if (on <= kWave) {
+        const uint64_t hit = __ballot(prev_b == b);
+        from = hit != 0ull ? ob + __builtin_ctzll(hit) : -1;
+      } else {
+        from = prev_find_wave(p, ob, on, b, lane);
+      }
+      keep_pair_wave(p, from, base + jj, lane);
       continue;
     }
     if (!((ok_mask >> k) & 1ull)) {

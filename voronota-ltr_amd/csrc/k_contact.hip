@@ -51,9 +51,10 @@ constexpr int kClipSlots = 64;  // neighbor slots handled by k_clip
 // variants of kClipVariants: lanes per pair in phase 2 (8: eight pairs side
 // by side, contours beyond 8 points, about 1% of the clipped pairs, go to
 // k_clip_wide's 16-lane groups; 16: four pairs side by side), wavefronts per
-// block, and the occupancy target in wavefronts per SIMD (4 caps registers at
-// 128 per lane, which spills some phase-1 state; 3 allows 168). The shared
-// memory of a 4-wavefront block (38.9 KB) allows four blocks per CU.
+// block, the occupancy target in wavefronts per SIMD (4 caps registers at 128
+// per lane, with some spills; 3 allows 168) and whether the planes' unit
+// normals stay in shared memory (9.7 KB per wavefront without them, 11.3 KB
+// with them).
 
 // Unit normals of the planes: kept in shared memory (NN) or recomputed each
 // time a plane is applied (the shared memory then allows four wavefronts per

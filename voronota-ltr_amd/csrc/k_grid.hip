@@ -272,8 +272,8 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
     r_pre[w][lane] = inc - rc;
   }
   // the 27 cell prefixes as wavefront-uniform values: a candidate's cell is
-  // the number of later cells whose prefix it has reached (prefixes never
-  // decrease; empty cells repeat their successor's)
+  // the number of cells 1..26 whose prefix it has reached (prefixes never
+  // decrease; an empty cell has its successor's prefix)
   uint32_t pre_s[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) pre_s[k] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(inc - rc), k));
@@ -501,7 +501,7 @@ __global__ void k_pair_fill(Pipeline p) {
   const int32_t* lst = p.nb + a * static_cast<int64_t>(p.nb_cap);
   int64_t j = 0;
   // list entries read four at a time (the rows are 16-byte aligned: nb_cap is
-  // a multiple of 4), so each lane has several loads in flight
+  // a multiple of 4): a quarter of the dependent loads per lane
   const int4* lst4 = reinterpret_cast<const int4*>(lst);
   int4 q = cnt > 0 ? lst4[0] : make_int4(0, 0, 0, 0);
   for (int k = 0; k < cnt; ++k) {

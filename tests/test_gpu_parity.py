@@ -123,6 +123,46 @@ def test_periodic_balls_outside_the_box(vltr, oracle):
     _run(vltr, oracle, b, 1.4, pbox=corners_box((0, 0, 0), (0.9 * L, 0.95 * L, L)), name="rsa box, smaller box")
 
 
+def _lattice(n_side, spacing, r):
+    g = np.arange(n_side) * spacing
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    return np.column_stack([x.ravel(), y.ravel(), z.ravel(), np.full(x.size, r)])
+
+
+@pytest.mark.parametrize("case", ["lattice", "lattice_jitter", "duplicates", "nested", "chain", "probe0", "plane"])
+def test_degenerate_inputs(vltr, oracle, case):
+    """Ties and degeneracies: a cubic lattice (equal neighbor keys, radical
+    planes through shared points), the same with 1e-9 jitter, duplicate
+    spheres (hidden by index), spheres inside others, a straight chain, zero
+    probe, and a flat sheet."""
+    rng = np.random.default_rng(7)
+    probe = 1.4
+    if case == "lattice":
+        b = _lattice(6, 2.0, 1.5)
+    elif case == "lattice_jitter":
+        b = _lattice(6, 2.0, 1.5)
+        b[:, :3] += rng.normal(scale=1e-9, size=(len(b), 3))
+    elif case == "duplicates":
+        b = _lattice(4, 3.0, 1.6)
+        b = np.concatenate([b, b[::3], b[::5]])
+    elif case == "nested":
+        b = _lattice(4, 3.2, 1.7)
+        inner = b[::2].copy()
+        inner[:, 3] = 0.5
+        inner[:, :3] += 0.3
+        b = np.concatenate([b, inner])
+    elif case == "chain":
+        b = np.column_stack([np.arange(40) * 1.1, np.zeros(40), np.zeros(40), np.full(40, 1.2)])
+    elif case == "probe0":
+        b = _lattice(5, 2.5, 1.6)
+        b[:, :3] += rng.normal(scale=0.3, size=(len(b), 3))
+        probe = 0.0
+    else:
+        g = _lattice(8, 2.4, 1.5)
+        b = g[g[:, 2] == 0.0]
+    _run(vltr, oracle, b, probe, name=case)
+
+
 def test_rsa_2500(vltr, oracle):
     from voronota_ltr import workload
 

@@ -322,6 +322,8 @@ struct OracleUpd {
 
 oracle_updateable* oracle_updateable_new(int with_backup, int threads) { return new OracleUpd(with_backup != 0, threads); }
 void oracle_updateable_free(oracle_updateable* u) { delete u; }
+// Moved spheres leave their old grid boxes (off: the reference's behaviour).
+void oracle_updateable_set_remove_from_old_box(oracle_updateable* u, int on) { u->t.set_remove_from_old_box(on != 0); }
 int oracle_updateable_init(oracle_updateable* u, const double* balls, size_t n, double probe, const double* pbox9) {
   auto b = load_balls<double>(balls, n);
   PeriodicBox<double> pb;

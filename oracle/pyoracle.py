@@ -55,6 +55,7 @@ def _load():
     L.oracle_updateable_new.argtypes = [C.c_int, C.c_int]
     L.oracle_updateable_new.restype = _vp
     L.oracle_updateable_free.argtypes = [_vp]
+    L.oracle_updateable_set_remove_from_old_box.argtypes = [_vp, C.c_int]
     L.oracle_updateable_init.argtypes = [_vp, _f64p, sz, C.c_double, _f64p]
     L.oracle_updateable_update.argtypes = [_vp, _f64p, sz]
     L.oracle_updateable_update_with_changed.argtypes = [_vp, _f64p, sz, _u64p, sz]
@@ -279,11 +280,19 @@ def max_threads() -> int:
 
 
 class Updateable:
-    """UpdateableTessellation restated on the CPU (updateable.rs:154-742)."""
+    """UpdateableTessellation restated on the CPU (updateable.rs:154-742).
 
-    def __init__(self, with_backup=False, threads=0):
+    remove_from_old_box=True corrects spheres_searcher.rs:144-175, where a
+    moved sphere stays listed in its old grid box (the reference removes it
+    from the box of its new position), so that later searches report it twice
+    and the updateable state gains duplicate contacts. The default keeps the
+    reference's behaviour.
+    """
+
+    def __init__(self, with_backup=False, threads=0, remove_from_old_box=False):
         self._L = _load()
         self._h = self._L.oracle_updateable_new(int(with_backup), int(threads))
+        self._L.oracle_updateable_set_remove_from_old_box(self._h, int(bool(remove_from_old_box)))
         self._n = 0
 
     def __del__(self):

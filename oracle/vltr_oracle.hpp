@@ -411,8 +411,18 @@ class SpheresSearcher {
   }
   const std::vector<Sphere<R>>& spheres() const { return spheres_; }
 
-  // spheres_searcher.rs:144-175
-  void update(const std::vector<Sphere<R>>& spheres, const std::vector<size_t>& changed) {
+  // spheres_searcher.rs:144-175. The reference writes the new spheres
+  // before remove_sphere_from_grid looks up their boxes, so the removal
+  // searches the NEW box: a sphere that moved to another box stays listed in
+  // its old one as well, and a search that reaches both boxes reports it
+  // twice. remove_from_old_box = true removes the entries while the old
+  // spheres are still in place (the evident intent), for the checks of
+  // callers that list every colliding pair once.
+  void update(const std::vector<Sphere<R>>& spheres, const std::vector<size_t>& changed,
+              bool remove_from_old_box = false) {
+    if (remove_from_old_box)
+      for (size_t id : changed)
+        if (id < spheres_.size()) remove_sphere_from_grid(id);
     for (size_t id : changed)
       if (id < spheres_.size() && id < spheres.size()) spheres_[id] = spheres[id];
     const GridParameters np = make_grid_parameters(spheres_);
@@ -421,8 +431,9 @@ class SpheresSearcher {
       params_ = np;
       init_boxes();
     } else {
-      for (size_t id : changed)
-        if (id < spheres_.size()) remove_sphere_from_grid(id);
+      if (!remove_from_old_box)
+        for (size_t id : changed)
+          if (id < spheres_.size()) remove_sphere_from_grid(id);
       for (size_t id : changed)
         if (id < spheres_.size()) add_sphere_to_grid(id);
     }
@@ -1424,7 +1435,7 @@ class SpheresContainer {
       spheres_[sid] = ns[sid];
       update_sphere_periodic_instances(sid, &changed_populated);
     }
-    if (has_searcher_) searcher_.update(populated_, changed_populated);
+    if (has_searcher_) searcher_.update(populated_, changed_populated, remove_from_old_box_);
     update_collisions_for_spheres(affected, threads);
     std::vector<size_t> more;
     for (size_t sid : changed)
@@ -1568,6 +1579,10 @@ class SpheresContainer {
   PeriodicBox<R> pbox_{};
   std::vector<Sphere<R>> populated_;
   std::vector<bool> exclusion_;
+ public:
+  // SpheresSearcher::update's remove_from_old_box for this container's updates
+  bool remove_from_old_box_ = false;
+ private:
   std::vector<std::vector<ValuedId<R>>> colliding_;
   SpheresSearcher<R> searcher_;
   bool has_searcher_ = false;
@@ -1605,6 +1620,9 @@ class UpdateableTessellation {
     compute_full_tessellation();
     return !st_.result_empty();
   }
+  // Moved spheres leave their old grid boxes (SpheresSearcher::update); the
+  // reference's behaviour is the default, off.
+  void set_remove_from_old_box(bool on) { st_.container.remove_from_old_box_ = on; }
   // updateable.rs:208-219
   bool update(const std::vector<Ball<R>>& balls) { return update_internal(balls, nullptr); }
   bool update_with_changed(const std::vector<Ball<R>>& balls, const std::vector<size_t>& changed) {

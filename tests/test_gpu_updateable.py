@@ -44,11 +44,11 @@ def _same_summary(gpu, ref, balls, probe):
 class Pair:
     """Runs every operation on the device updateable and on the oracle's."""
 
-    def __init__(self, oracle, with_backup=False):
+    def __init__(self, oracle, with_backup=False, remove_from_old_box=False):
         from voronota_ltr import _native
 
         self.g = _native.Updateable(with_backup=with_backup)
-        self.o = oracle.Updateable(with_backup=with_backup)
+        self.o = oracle.Updateable(with_backup=with_backup, remove_from_old_box=remove_from_old_box)
         self.balls = None
         self.probe = None
 
@@ -240,3 +240,61 @@ def test_energy_gradient_with_updates(oracle):
     e, g = v.energy_gradient()
     e2, g2 = _native.energy_gradient(prev, 1.4, types, W, pbox=box)
     assert e == pytest.approx(e2, rel=1e-12) and np.allclose(g, g2, rtol=1e-9, atol=1e-9)
+
+
+def test_hiding_and_exclusion_moves(oracle):
+    """Degenerate transitions with the re-clipping of affected pairs only: a
+    ball moved onto another (a duplicate, hidden by index), inside another
+    (hidden), back out, and moves next to a ball the caller excluded; every
+    step against the oracle's updateable. The moves cross grid boxes of the
+    reference's searcher, which then lists the moved ball twice
+    (spheres_searcher.rs:144-175): the oracle runs with that corrected, as the
+    device path computes every pair once. The stale statuses the reference
+    keeps are matched: a hidden ball's exclusion stays until the ball is
+    affected again, and a ball the caller excluded is included again when a
+    neighbor moves."""
+    g = np.arange(5) * 3.0
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    balls = np.column_stack([x.ravel(), y.ravel(), z.ravel(), np.full(x.size, 1.6)])
+    balls[:, :3] += np.random.default_rng(11).normal(scale=0.2, size=(len(balls), 3))
+    p = Pair(oracle, with_backup=True, remove_from_old_box=True)
+    print("\n[gpu] updateable hiding moves", flush=True)
+    assert p.init(balls, 1.4)
+    cur = balls.copy()
+    home = cur[62].copy()
+    cur[62, :3] = cur[63, :3]  # onto another ball: equal spheres, the higher index hidden
+    cur[62, 3] = cur[63, 3]
+    p.update(cur, [62])
+    cur[62, :3] = cur[37, :3] + 0.1  # inside a bigger ball
+    cur[62, 3] = 0.4
+    p.update(cur, [62])
+    cur[62] = home  # back out
+    p.update(cur, [62])
+    p.op("set_exclusion", 31, True)
+    for k in (30, 32, 56):  # neighbors of the excluded ball move
+        cur[k, :3] += 0.35
+        p.update(cur, [k])
+    p.op("set_exclusion", 31, False)
+    cur[31, :3] += 0.2
+    p.update(cur, [31])
+    p.op("restore")
+
+
+def test_cross_box_moves(oracle):
+    """Three balls of a 6x6x6 lattice moved at a time, about 0.4 A per axis,
+    so that many cross a grid box: every step against the oracle's updateable
+    with moved spheres taken out of their old boxes (the reference keeps them
+    listed there as well, spheres_searcher.rs:144-175, and then reports
+    duplicate contacts)."""
+    g = np.arange(6) * 3.0
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    balls = np.column_stack([x.ravel(), y.ravel(), z.ravel(), np.full(x.size, 1.6)])
+    balls[:, :3] += np.random.default_rng(5).normal(scale=0.2, size=(len(balls), 3))
+    rng = np.random.default_rng(5)
+    p = Pair(oracle, remove_from_old_box=True)
+    print("\n[gpu] updateable cross-box moves", flush=True)
+    assert p.init(balls, 1.4)
+    for _ in range(25):
+        ids = rng.choice(len(balls), 3, replace=False)
+        balls[ids, :3] += rng.normal(scale=0.4, size=(3, 3))
+        assert p.update(balls, ids)

@@ -165,3 +165,54 @@ def test_incremental_equals_full_on_rsa(oracle):
         assert max(abs(fi[k] - ii[k]) for k in fi) < 1e-12
         assert np.allclose(inc.cell_sas, full.cell_sas, rtol=0, atol=1e-10)
         assert np.allclose(inc.cell_volume, full.cell_volume, rtol=0, atol=1e-10)
+
+
+def _lattice(n_side, seed):
+    g = np.arange(n_side) * 3.0
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    balls = np.column_stack([x.ravel(), y.ravel(), z.ravel(), np.full(x.size, 1.6)])
+    balls[:, :3] += np.random.default_rng(seed).normal(scale=0.2, size=(len(balls), 3))
+    return balls
+
+
+def _cross_box_moves(oracle, remove_from_old_box, steps=25):
+    """Three balls at a time moved about 0.4 A per axis (many cross a grid
+    box of the searcher) on a 6x6x6 lattice; yields (summary, full
+    tessellation) after each update."""
+    rng = np.random.default_rng(5)
+    balls = _lattice(6, 5)
+    t = oracle.Updateable(remove_from_old_box=remove_from_old_box)
+    assert t.init(balls, 1.4)
+    for _ in range(steps):
+        ids = rng.choice(len(balls), 3, replace=False)
+        balls[ids, :3] += rng.normal(scale=0.4, size=(3, 3))
+        assert t.update_with_changed(balls, list(ids))
+        yield t.summary(), oracle.compute_tessellation(balls, 1.4)
+
+
+def test_searcher_keeps_stale_box_entries(oracle):
+    """spheres_searcher.rs:144-175 writes a moved sphere before
+    remove_sphere_from_grid looks up its box, so a sphere moved to another
+    box stays listed in the old one: a search reaching both boxes reports it
+    twice, collect_pairs (updateable.rs:516-561) lists the pair twice and
+    update_using_affected stores the contact twice, so summary() reports
+    duplicate contacts and the cell sums count them twice. The restatement
+    keeps this by default; it appears only once a moved ball has changed box."""
+    dups = [len(s.contact_ids) - len({tuple(int(v) for v in k) for k in s.contact_ids})
+            for s, _ in _cross_box_moves(oracle, False)]
+    assert max(dups) > 0
+
+
+def test_corrected_searcher_equals_full(oracle):
+    """With moved spheres taken out of their old boxes, the incremental
+    state equals a full recompute after every move (contact set exactly,
+    values to 1e-12)."""
+    for inc, full in _cross_box_moves(oracle, True):
+        ki = [tuple(int(v) for v in k) for k in inc.contact_ids]
+        kf = [tuple(int(v) for v in k) for k in full.contact_ids]
+        assert sorted(ki) == sorted(kf)
+        a = dict(zip(ki, inc.contact_area))
+        assert max(abs(a[k] - v) for k, v in zip(kf, full.contact_area)) < 1e-12
+        assert list(inc.cell_index) == list(full.cell_index)
+        assert np.allclose(inc.cell_volume, full.cell_volume, rtol=0, atol=1e-10)
+        assert np.allclose(inc.cell_sas, full.cell_sas, rtol=0, atol=1e-10)

@@ -262,7 +262,11 @@ __global__ void k_seg_offsets(Pipeline p, const uint32_t* pos, int what, uint32_
 // incremental update an affected ball without contacts is detached whenever
 // it is not excluded, and unaffected balls keep their cells. refresh != 0
 // (init and updates, not set_exclusion) resets the exclusion status of every
-// recomputed ball to its hidden status, as the collision update does.
+// recomputed ball to its hidden status, as the collision update does. A ball
+// that joined the affected set during the run (a new neighbor of a moved
+// ball) still carried its old status when the pairs were counted; when that
+// status was excluded and is now cleared, the run's pairs miss this ball's
+// and the report word kReportExclFlip asks the host for a second pass.
 __global__ void k_updateable_cells(Pipeline p, CellRec* state, uint8_t* excl, const uint8_t* affected, int full,
                                    int refresh) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -271,7 +275,11 @@ __global__ void k_updateable_cells(Pipeline p, CellRec* state, uint8_t* excl, co
     p.cell[i] = state[i];
     return;
   }
-  if (refresh) excl[i] = p.excluded[i];
+  if (refresh) {
+    const uint8_t hidden = p.excluded[i];
+    if (excl[i] != 0 && hidden == 0) p.status[kReportExclFlip] = 1u;
+    excl[i] = hidden;
+  }
   const CellRec rec = p.cell[i];
   const bool from_contacts = rec.stage == 2 && p.nb_cnt[i] > 0;
   CellRec out = rec;

@@ -107,6 +107,7 @@ __global__ void k_init(Pipeline p) {
     *p.overflow_cnt = 0u;
     *p.wide_cnt = 0u;
     *p.nb_big_cnt = 0u;
+    p.status[kReportExclFlip] = 0u;
   }
 }
 

@@ -1489,6 +1489,12 @@ struct vltr_updateable {
   DevBuf prev_pair_cnt, prev_pair_b, prev_rec, prev_topo;
   bool prev_ok = false, prev_topo_ok = false;
   size_t prev_n = 0;
+  int64_t prev_pairs = 0;
+  // the same for the backup state: a restore keeps every pair's record
+  DevBuf bk_pair_cnt, bk_pair_b, bk_rec, bk_topo;
+  bool bk_ok = false, bk_topo_ok = false;
+  size_t bk_n = 0;
+  int64_t bk_pairs = 0;
 };
 
 namespace {
@@ -1671,9 +1677,20 @@ int upd_run(Upd* u, const RunOpts& o) {
     return launch_adjoint(p, d_types, w16, d_grad, d_part, keys, first, work, st2);
   };
   size_t nc = 0, ncell = 0;
-  const int rc = run_pipeline(ctx, static_cast<int64_t>(n), st.probe, st.periodic ? &st.pbox : nullptr, nullptr, 0, 0,
-                              &nc, &ncell, nullptr, 0, &ex);
+  int rc = run_pipeline(ctx, static_cast<int64_t>(n), st.probe, st.periodic ? &st.pbox : nullptr, nullptr, 0, 0, &nc,
+                        &ncell, nullptr, 0, &ex);
   if (rc != VLTR_OK) return rc;
+  if (static_cast<const uint32_t*>(ctx->pinned.ptr)[kReportExclFlip] != 0u) {
+    // a new neighbor of a moved ball lost a stale exclusion (it was hidden
+    // by a ball that has moved away, or excluded by the caller) after the
+    // pairs were counted without it: the same run again, with the affected
+    // set and exclusion statuses it left (spheres_container.rs:150-170
+    // refreshes these balls before the contacts are computed)
+    ex.pre_clip = nullptr;
+    rc = run_pipeline(ctx, static_cast<int64_t>(n), st.probe, st.periodic ? &st.pbox : nullptr, nullptr, 0, 0, &nc,
+                      &ncell, nullptr, 0, &ex);
+    if (rc != VLTR_OK) return rc;
+  }
   // this run's pair lists and records become the previous run's (the
   // context's buffers are scratch of the next run)
   std::swap(ctx->pair_cnt, u->prev_pair_cnt);
@@ -1683,6 +1700,7 @@ int upd_run(Upd* u, const RunOpts& o) {
   u->prev_ok = true;
   u->prev_topo_ok = adj;
   u->prev_n = n;
+  u->prev_pairs = static_cast<int64_t>(ctx->counts[2]);
   if (adj) {
     std::vector<double> parts(static_cast<size_t>(adjoint_blocks()));
     VLTR_HIP(hipMemcpy(parts.data(), d_part, parts.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1707,6 +1725,29 @@ int upd_prepare(Upd* u) {
     VLTR_HIP(u->excl_backup.ensure(n));
     VLTR_HIP(hipMemcpyAsync(u->cells_backup.ptr, u->cells_state.ptr, n * sizeof(CellRec), hipMemcpyDeviceToDevice, s));
     VLTR_HIP(hipMemcpyAsync(u->excl_backup.ptr, u->excl.ptr, n, hipMemcpyDeviceToDevice, s));
+    // the pair records the backup state was made of (the next run reads the
+    // current ones as its previous records, so they are copied, not moved)
+    u->bk_ok = u->prev_ok;
+    u->bk_topo_ok = u->prev_topo_ok;
+    u->bk_n = u->prev_n;
+    u->bk_pairs = u->prev_pairs;
+    if (u->prev_ok) {
+      const size_t np = static_cast<size_t>(u->prev_pairs);
+      VLTR_HIP(u->bk_pair_cnt.ensure((u->prev_n + 1) * sizeof(uint32_t)));
+      VLTR_HIP(hipMemcpyAsync(u->bk_pair_cnt.ptr, u->prev_pair_cnt.ptr, (u->prev_n + 1) * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, s));
+      if (np > 0) {
+        VLTR_HIP(u->bk_pair_b.ensure(np * sizeof(int32_t)));
+        VLTR_HIP(u->bk_rec.ensure(np * sizeof(PairRec)));
+        VLTR_HIP(hipMemcpyAsync(u->bk_pair_b.ptr, u->prev_pair_b.ptr, np * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        VLTR_HIP(hipMemcpyAsync(u->bk_rec.ptr, u->prev_rec.ptr, np * sizeof(PairRec), hipMemcpyDeviceToDevice, s));
+        if (u->prev_topo_ok) {
+          VLTR_HIP(u->bk_topo.ensure(np * sizeof(ContactTopo)));
+          VLTR_HIP(hipMemcpyAsync(u->bk_topo.ptr, u->prev_topo.ptr, np * sizeof(ContactTopo), hipMemcpyDeviceToDevice,
+                                  s));
+        }
+      }
+    }
     u->has_backup = true;
   }
   u->st.changed.clear();
@@ -1861,7 +1902,8 @@ void vltr_updateable_destroy(vltr_updateable* u) {
     (void)hipStreamSynchronize(u->ctx->stream);
     for (DevBuf* b : {&u->cells_state, &u->cells_backup, &u->excl, &u->excl_backup, &u->affected_mask, &u->ids, &u->rows,
                       &u->rows_cnt, &u->moved_vals, &u->types, &u->grad, &u->part, &u->topo, &u->seed, &u->dedup_keys,
-                      &u->dedup_first, &u->prev_pair_cnt, &u->prev_pair_b, &u->prev_rec, &u->prev_topo})
+                      &u->dedup_first, &u->prev_pair_cnt, &u->prev_pair_b, &u->prev_rec, &u->prev_topo,
+                      &u->bk_pair_cnt, &u->bk_pair_b, &u->bk_rec, &u->bk_topo})
       b->release();
     u->stage.release();
     vltr_context_destroy(u->ctx);
@@ -1974,9 +2016,27 @@ int vltr_updateable_restore(vltr_updateable* u, int* ok) {
   VLTR_HIP(hipMemcpyAsync(u->cells_state.ptr, u->cells_backup.ptr, n * sizeof(CellRec), hipMemcpyDeviceToDevice, s));
   VLTR_HIP(hipMemcpyAsync(u->excl.ptr, u->excl_backup.ptr, n, hipMemcpyDeviceToDevice, s));
   const auto kept = u->st;
-  // contacts are recomputed from the restored balls and exclusions; every
-  // cell keeps its restored record
-  const int rc = upd_run(u, RunOpts{});
+  // the backup state's pair records come back as the previous run's; a run
+  // with no ball affected then gives every pair of the restored balls the
+  // record it had (the reference restores the contact lists themselves,
+  // updateable.rs:296-309, which may hold contacts a recompute would not
+  // give, e.g. of a ball whose exclusion is stale) and every cell its
+  // restored record
+  u->prev_ok = false;
+  if (u->bk_ok) {
+    std::swap(u->prev_pair_cnt, u->bk_pair_cnt);
+    std::swap(u->prev_pair_b, u->bk_pair_b);
+    std::swap(u->prev_rec, u->bk_rec);
+    std::swap(u->prev_topo, u->bk_topo);
+    u->prev_ok = true;
+    u->prev_topo_ok = u->bk_topo_ok;
+    u->prev_n = u->bk_n;
+    u->prev_pairs = u->bk_pairs;
+    u->bk_ok = false;
+  }
+  RunOpts o;
+  o.incremental = true;
+  const int rc = upd_run(u, o);
   if (rc != VLTR_OK) return rc;
   u->st.changed = kept.changed;
   u->st.affected = kept.affected;

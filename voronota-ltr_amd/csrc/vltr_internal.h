@@ -52,7 +52,9 @@ enum ReportWord : int {
   kReportPairs = 4,     // candidate pairs (the pair count scan's total)
   kReportContacts = 5,  // contacts (the contact flag scan's total)
   kReportCells = 6,     // cells (the cell flag scan's total)
-  kReportWords = 7
+  kReportExclFlip = 7,  // nonzero: an updateable run cleared a stale exclusion status
+                        // after the pairs were counted (k_updateable_cells)
+  kReportWords = 8
 };
 
 // Launch parameters of the tessellation pipeline for one call (one or many

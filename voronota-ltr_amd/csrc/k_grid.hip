@@ -19,7 +19,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kNbBlock = 64;                  // one wavefront per block
 constexpr int kNbWaves = kNbBlock / kWave;
-constexpr int kListCap = 512;                 // per-wave LDS candidate list
+constexpr int kListCap = 256;                 // per-wave LDS list (4 KB: eight wavefronts per SIMD)
 
 __device__ __forceinline__ unsigned long long lanemask_lt() {
   const int lane = threadIdx.x & (kWave - 1);

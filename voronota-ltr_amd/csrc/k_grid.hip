@@ -493,7 +493,11 @@ __global__ __launch_bounds__(kBigBlock) void k_neighbors_big(Pipeline p) {
 
 // Pair list in reference order: by first ball, then neighbor order. Each pair
 // (a, c) whose second ball is canonical is also appended to c's incoming list
-// (tessellation.rs:624-631: an image end receives nothing).
+// (tessellation.rs:624-631: an image end receives nothing); the incoming
+// order is free, k_cells puts each list in pair order. A lane per ball: the
+// device-scope atomic appends (one per pair) set the pace, and spreading each
+// lane's appends over its list keeps fewer of them in flight at once than a
+// wavefront per ball does (0.16 vs 0.21 ms for the 2.45 M pairs of 100k balls).
 __global__ void k_pair_fill(Pipeline p) {
   const int64_t a = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (a >= p.n) return;
@@ -523,6 +527,40 @@ __global__ void k_pair_fill(Pipeline p) {
       atomicOr(p.status, kStatPairOverflow);
     }
     ++j;
+  }
+}
+
+// The same pair list without incoming lists (contacts only: no cells). One
+// wavefront per ball, a lane per list entry; the qualifying entries take
+// consecutive slots by a ballot, so the stores are coalesced.
+constexpr int kFillBlock = 256;
+__global__ __launch_bounds__(kFillBlock) void k_pair_fill_wave(Pipeline p) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t a = blockIdx.x * static_cast<int64_t>(kFillBlock / kWave) + threadIdx.x / kWave;
+  if (a >= p.n) return;  // the whole wavefront
+  const int64_t base = p.pair_cnt[a];
+  const int cnt = p.nb_cnt[a];
+  const int32_t* lst = p.nb + a * static_cast<int64_t>(p.nb_cap);
+  int64_t j = 0;
+  for (int k0 = 0; k0 < cnt; k0 += kWave) {
+    const int k = k0 + lane;
+    int32_t c = 0;
+    bool q = false;
+    if (k < cnt) {
+      c = lst[k];
+      q = pair_qualifies(p, a, c);
+    }
+    const unsigned long long m = __ballot(q);
+    if (q) {
+      const int64_t idx = base + j + __popcll(m & lanemask_lt());
+      if (idx < p.pair_cap) {
+        p.pair_a[idx] = static_cast<int32_t>(a);
+        p.pair_b[idx] = c;
+      } else {
+        atomicOr(p.status, kStatPairOverflow);
+      }
+    }
+    j += __popcll(m);
   }
 }
 
@@ -673,6 +711,10 @@ hipError_t launch_neighbors(const Pipeline& p, hipStream_t s) {
 
 hipError_t launch_pairs(const Pipeline& p, hipStream_t s) {
   if (p.n > 0) {
+    if (p.mode != 0) {
+      hipLaunchKernelGGL(k_pair_fill_wave, dim3(blocks_for(p.n, kFillBlock / kWave)), dim3(kFillBlock), 0, s, p);
+      return hipGetLastError();
+    }
     const int64_t fb = blocks_for(p.n, 256) < 4096 ? blocks_for(p.n, 256) : 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, s, p.in_cnt, p.n, 0u);
     hipLaunchKernelGGL(k_pair_fill, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p);

@@ -1513,6 +1513,7 @@ class SpheresContainer {
   const std::vector<ValuedId<R>>& colliding_ids(size_t id) const { return colliding_[id]; }
   const std::vector<bool>& exclusion_statuses() const { return exclusion_; }
   bool is_excluded(size_t id) const { return id < exclusion_.size() && exclusion_[id]; }
+  void set_remove_from_old_box(bool on) { remove_from_old_box_ = on; }
   bool periodic() const { return has_pbox_; }
 
  private:
@@ -1579,10 +1580,8 @@ class SpheresContainer {
   PeriodicBox<R> pbox_{};
   std::vector<Sphere<R>> populated_;
   std::vector<bool> exclusion_;
- public:
   // SpheresSearcher::update's remove_from_old_box for this container's updates
   bool remove_from_old_box_ = false;
- private:
   std::vector<std::vector<ValuedId<R>>> colliding_;
   SpheresSearcher<R> searcher_;
   bool has_searcher_ = false;
@@ -1622,7 +1621,7 @@ class UpdateableTessellation {
   }
   // Moved spheres leave their old grid boxes (SpheresSearcher::update); the
   // reference's behaviour is the default, off.
-  void set_remove_from_old_box(bool on) { st_.container.remove_from_old_box_ = on; }
+  void set_remove_from_old_box(bool on) { st_.container.set_remove_from_old_box(on); }
   // updateable.rs:208-219
   bool update(const std::vector<Ball<R>>& balls) { return update_internal(balls, nullptr); }
   bool update_with_changed(const std::vector<Ball<R>>& balls, const std::vector<size_t>& changed) {

@@ -366,14 +366,32 @@ VLTR_HD Plane plane_of(const Sph& a, const Sph& c) {
 // pair has no contact as soon as one neighbor kills it, and otherwise its
 // contour is cut by the kCut neighbors in sorted order.
 enum NbClass : int { kSkip = 0, kCut = 1, kKill = 2 };
+
+// l / sqrt(t) >= r, the quotient and root rounded in double precision as the
+// reference computes them (contact.rs:261-262). On the device a
+// single-precision estimate decides first when it is far from r: l and t
+// rounded to single (2^-24 each), a hardware reciprocal square root (1 ulp,
+// 2^-23) and one product (2^-24) put it within 5e-7 of the exact quotient
+// (relative), so outside a 1e-5 margin around r its answer is the exact one.
+// Near r, and for l or t outside [1e-30, 1e15], the exact quotient decides.
+VLTR_HD bool quotient_reaches(double l, double t, double r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (t >= 1e-30 && l >= 1e-30 && l <= 1e15) {
+    const double xa = static_cast<double>(static_cast<float>(l) * __builtin_amdgcn_rsqf(static_cast<float>(t)));
+    if (xa > r * (1.0 + 1e-5)) return true;
+    if (xa < r * (1.0 - 1e-5)) return false;
+  }
+#endif
+  return l / sqrt(t) >= r;
+}
+
 VLTR_HD int classify(const PairGeom& g, const Sph& c, const Plane& pl, bool c_contains_a) {
   if (!intersects(g.ic, c) || !intersects(g.b, c)) return kSkip;
   if (c_contains_a || contains(c, g.b)) return kKill;
   const double cosv = dot(g.ic_dir, pl.pdir);
   if (fabs(cosv) >= 1.0) return halfspace(pl.pc, pl.pn, g.ic.c) > 0 ? kKill : kSkip;
   const double l = fabs(dot(pl.pn, g.ic.c - pl.pc));
-  const double xl = l / sqrt(fma(cosv, -cosv, 1.0));
-  if (xl >= g.ic.r) return halfspace(pl.pc, pl.pn, g.ic.c) >= 0 ? kKill : kSkip;
+  if (quotient_reaches(l, fma(cosv, -cosv, 1.0), g.ic.r)) return halfspace(pl.pc, pl.pn, g.ic.c) >= 0 ? kKill : kSkip;
   return kCut;
 }
 

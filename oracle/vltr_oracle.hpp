@@ -415,14 +415,19 @@ class SpheresSearcher {
   // before remove_sphere_from_grid looks up their boxes, so the removal
   // searches the NEW box: a sphere that moved to another box stays listed in
   // its old one as well, and a search that reaches both boxes reports it
-  // twice. remove_from_old_box = true removes the entries while the old
-  // spheres are still in place (the evident intent), for the checks of
-  // callers that list every colliding pair once.
-  void update(const std::vector<Sphere<R>>& spheres, const std::vector<size_t>& changed,
+  // twice; a changed id given twice is also added twice. remove_from_old_box
+  // = true removes the entries while the old spheres are still in place and
+  // adds each sphere once (the evident intent), for the checks of callers
+  // that list every colliding pair once.
+  void update(const std::vector<Sphere<R>>& spheres, const std::vector<size_t>& changed_in,
               bool remove_from_old_box = false) {
-    if (remove_from_old_box)
+    std::vector<size_t> changed = changed_in;
+    if (remove_from_old_box) {
+      std::sort(changed.begin(), changed.end());
+      changed.erase(std::unique(changed.begin(), changed.end()), changed.end());
       for (size_t id : changed)
         if (id < spheres_.size()) remove_sphere_from_grid(id);
+    }
     for (size_t id : changed)
       if (id < spheres_.size() && id < spheres.size()) spheres_[id] = spheres[id];
     const GridParameters np = make_grid_parameters(spheres_);

@@ -147,6 +147,33 @@ def test_auto_detect_no_change_and_reinit(oracle):
     assert q.g.last_update_was_full_reinit()
     # a different ball count: full reinit
     assert q.update(line[:7])
+    # few changed balls with many old neighbors: changed + old neighbors past
+    # the threshold (24 balls that all collide: 2 + 22 > 12) is a full reinit,
+    # and a repeated changed id counts twice; below it the update is incremental
+    rng = np.random.default_rng(24)
+    blob = np.column_stack([rng.uniform(0, 3, size=(24, 3)), np.full(24, 1.0)])
+    r = Pair(oracle)
+    assert r.init(blob, 0.5)
+    bm = blob.copy()
+    bm[[3, 17], 0] += 0.1
+    assert r.update(bm, [3, 17])
+    assert r.g.last_update_was_full_reinit()
+    bm[5, 1] += 0.1
+    assert r.update(bm, [5, 5])
+    assert r.g.last_update_was_full_reinit()
+    # (a repeated id is added twice to its grid box by the reference's
+    # searcher, spheres_searcher.rs:164-173, and then listed twice: the
+    # oracle adds it once here, as the device path counts each pair once)
+    chain = np.array([[2.0 * i, 0.1 * (i % 3), 0, 1.0] for i in range(30)])
+    c = Pair(oracle, remove_from_old_box=True)
+    assert c.init(chain, 0.5)
+    cm = chain.copy()
+    cm[[4, 15, 26], 1] += 0.1
+    assert c.update(cm, [4, 15, 26])
+    assert not c.g.last_update_was_full_reinit()
+    cm[[7, 8], 2] += 0.1
+    assert c.update(cm, [7, 8, 8])
+    assert not c.g.last_update_was_full_reinit()
 
 
 def test_2zsk_incremental(oracle):

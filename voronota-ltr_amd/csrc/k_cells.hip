@@ -299,6 +299,30 @@ __global__ void k_updateable_cells(Pipeline p, CellRec* state, uint8_t* excl, co
   state[i] = out;
 }
 
+// Balls marked 1 (old neighbors of changed balls) and 3 (changed) for the
+// affected-set threshold of spheres_container.rs:105-124; one atomic per
+// wavefront and count.
+__global__ void k_count_marks(const uint8_t* mask, int64_t n, uint32_t* counts) {
+  uint32_t c1 = 0, c3 = 0;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const uint8_t m = mask[i];
+    c1 += m == 1 ? 1u : 0u;
+    c3 += m == 3 ? 1u : 0u;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    c1 += __shfl_xor(c1, off, 64);
+    c3 += __shfl_xor(c3, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (c1) atomicAdd(&counts[0], c1);
+    if (c3) atomicAdd(&counts[1], c3);
+  }
+}
+__global__ void k_zero_two(uint32_t* counts) {
+  if (threadIdx.x < 2) counts[threadIdx.x] = 0u;
+}
+
 // spheres_container.rs:105-124 and 150-170: the colliding neighbors
 // (canonical ids) of the changed balls join the affected set; a ball not yet
 // marked takes the mark `value` (every writer of one ball writes the same).
@@ -376,6 +400,12 @@ hipError_t launch_mark_neighbors(const Pipeline& p, const int32_t* changed, int6
 }
 hipError_t launch_mark_ids(uint8_t* mask, int64_t n, const int32_t* ids, int64_t k, uint8_t value, hipStream_t s) {
   if (k > 0) hipLaunchKernelGGL(k_mark_ids, dim3(blocks_for(k, 256)), dim3(256), 0, s, mask, n, ids, k, value);
+  return hipGetLastError();
+}
+hipError_t launch_count_marks(const uint8_t* mask, int64_t n, uint32_t* counts, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, s, counts);
+  const int64_t b = blocks_for(n, 256);
+  if (n > 0) hipLaunchKernelGGL(k_count_marks, dim3(b < 1024 ? b : 1024), dim3(256), 0, s, mask, n, counts);
   return hipGetLastError();
 }
 hipError_t launch_fill_u8(uint8_t* mask, int64_t n, uint8_t value, hipStream_t s) {

@@ -174,6 +174,9 @@ int kbits_for(int64_t n) {
   return k;
 }
 
+// The run's report block (kReport* words) in the context's small buffer.
+uint32_t* report_words(vltr_context* ctx) { return reinterpret_cast<uint32_t*>(ctx->small.as<unsigned char>() + 8); }
+
 void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.balls_in = ctx->balls.as<double4>();
   p.sph = ctx->sph.as<double4>();
@@ -187,7 +190,7 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   unsigned char* small = ctx->small.as<unsigned char>();
   p.box_bits = reinterpret_cast<unsigned long long*>(small);
   // the report block: status, then the counters and totals (kReport*)
-  p.status = reinterpret_cast<uint32_t*>(small + 8);
+  p.status = report_words(ctx);
   p.overflow_cnt = p.status + kReportOverflow;
   p.wide_cnt = p.status + kReportWide;
   p.nb_big_cnt = p.status + kReportNbBig;
@@ -233,7 +236,13 @@ struct RunExtras {
   const int32_t* prev_pair_b = nullptr;
   const PairRec* prev_rec = nullptr;
   const ContactTopo* prev_topo = nullptr;
+  // called with the report block once the neighbor lists are complete; false
+  // ends the run there (run_pipeline returns kRunAborted)
+  std::function<bool(const uint32_t*)> front_check;
 };
+
+// run_pipeline's return value when RunExtras::front_check ended the run.
+constexpr int kRunAborted = -1000;
 
 // Capture of a batched run into a HIP graph (see run_batched): the pair
 // capacity it is captured with and the caller's device balls, which the
@@ -473,6 +482,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     }
     break;
   }
+  if (extras && extras->front_check && !extras->front_check(host_words)) return kRunAborted;
   const int64_t P = host_words[kReportPairs];
   int rc = ensure_back(ctx, p, P, n, extras);
   if (rc != VLTR_OK) return rc;
@@ -1548,6 +1558,7 @@ struct RunOpts {
                              // (ids in u->ids, values in u->moved_vals, on the device)
   bool more = false;         // the moved balls' new neighbors join the affected set (mark 2)
   uint8_t* mask_out = nullptr;  // pinned host copy of the final affected mask (n bytes)
+  std::function<bool(const uint32_t*)> front_check;  // RunExtras::front_check
 };
 
 // The ids i < n with mask[i] != 0, in order. Branch-free: about 40% of the
@@ -1617,6 +1628,7 @@ int upd_run(Upd* u, const RunOpts& o) {
   }
   RunExtras ex;
   ex.user_excl = o.full ? nullptr : u->excl.as<uint8_t>();
+  ex.front_check = o.front_check;
   CellRec* state = u->cells_state.as<CellRec>();
   uint8_t* excl = u->excl.as<uint8_t>();
   const int32_t* moved_ids = u->ids.as<int32_t>();
@@ -1687,6 +1699,7 @@ int upd_run(Upd* u, const RunOpts& o) {
     // set and exclusion statuses it left (spheres_container.rs:150-170
     // refreshes these balls before the contacts are computed)
     ex.pre_clip = nullptr;
+    ex.front_check = nullptr;
     rc = run_pipeline(ctx, static_cast<int64_t>(n), st.probe, st.periodic ? &st.pbox : nullptr, nullptr, 0, 0, &nc,
                       &ncell, nullptr, 0, &ex);
     if (rc != VLTR_OK) return rc;
@@ -1830,35 +1843,16 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
   last.nb_cnt = ctx->nb_cnt.as<int32_t>();
   last.nb_cap = ctx->last_nb_cap;
   VLTR_HIP(launch_mark_neighbors(last, d_ids, static_cast<int64_t>(k), am, 1, s));
-  VLTR_HIP(hipMemcpyAsync(stage + mask_off, am, n, hipMemcpyDeviceToHost, s));
-  VLTR_HIP(hipStreamSynchronize(s));
+  // the marks are counted on the device and reach the host with the run's
+  // first report, where the threshold is checked before the pairs are made
+  VLTR_HIP(launch_count_marks(am, static_cast<int64_t>(n), report_words(ctx) + kReportMarksOld, s));
+  for (uint64_t id : changed) st.balls[id] = balls[id];
   // the reference inserts the old neighbors one by one into the sorted
   // changed list and gives up (full reinit) when one more would pass the
   // threshold, i.e. when |changed| + |new ids| > threshold
   // (spheres_container.rs:105-124); a changed id listed twice stays twice
   // until the final sort + unique
-  size_t n_old = 0, n_changed_unique = 0;
-  for (size_t i = 0; i < n; ++i) {
-    n_old += h_mask[i] == 1;
-    n_changed_unique += h_mask[i] == 3;
-  }
-  const bool repeats = n_changed_unique != k;
-  if (k + n_old > threshold) {
-    rc = upd_full(u, balls, nb);
-    if (rc == VLTR_OK) *ok = 1;
-    return rc;
-  }
-  // the sorted affected list: the marked ids in order
-  std::vector<uint64_t>& affected = u->aff_scratch;
-  if (repeats) {
-    affected = changed;
-    for (size_t i = 0; i < n; ++i)
-      if (h_mask[i] == 1) affected.push_back(i);
-    std::sort(affected.begin(), affected.end());
-  } else {
-    marked_ids(h_mask, n, &affected);
-  }
-  for (uint64_t id : changed) st.balls[id] = balls[id];
+  bool repeats = false;
   RunOpts o;
   o.refresh = true;
   o.incremental = true;
@@ -1866,13 +1860,31 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
   o.n_moved = static_cast<int64_t>(k);
   o.more = true;
   o.mask_out = stage + mask_off;
+  o.front_check = [&](const uint32_t* hw) {
+    repeats = hw[kReportMarksChanged] != k;
+    return k + hw[kReportMarksOld] <= threshold;
+  };
   rc = upd_run(u, o);
+  if (rc == kRunAborted) {
+    rc = upd_full(u, balls, nb);
+    if (rc == VLTR_OK) *ok = 1;
+    return rc;
+  }
   if (rc != VLTR_OK) return rc;
-  // spheres_container.rs:150-170: new neighbors of the changed balls join;
-  // the list is then sorted and without repeats
+  // the sorted affected list: the marked ids in order, with the new neighbors
+  // of the changed balls (spheres_container.rs:150-170); without new ones, a
+  // repeated changed id stays repeated
+  std::vector<uint64_t>& affected = u->aff_scratch;
   bool more = false;
   for (size_t i = 0; i < n && !more; ++i) more = h_mask[i] == 2;
-  if (more) marked_ids(h_mask, n, &affected);
+  if (repeats && !more) {
+    affected = changed;
+    for (size_t i = 0; i < n; ++i)
+      if (h_mask[i] == 1) affected.push_back(i);
+    std::sort(affected.begin(), affected.end());
+  } else {
+    marked_ids(h_mask, n, &affected);
+  }
   st.changed.swap(changed);
   st.affected.swap(affected);  // the old list's storage is the next step's scratch
   st.was_full_reinit = false;

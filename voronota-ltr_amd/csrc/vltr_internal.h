@@ -54,7 +54,9 @@ enum ReportWord : int {
   kReportCells = 6,     // cells (the cell flag scan's total)
   kReportExclFlip = 7,  // nonzero: an updateable run cleared a stale exclusion status
                         // after the pairs were counted (k_updateable_cells)
-  kReportWords = 8
+  kReportMarksOld = 8,      // updateable step: balls marked as old neighbors (k_count_marks,
+  kReportMarksChanged = 9,  // written before the run) and as changed
+  kReportWords = 10
 };
 
 // Launch parameters of the tessellation pipeline for one call (one or many
@@ -237,6 +239,8 @@ hipError_t launch_mark_neighbors(const Pipeline& p, const int32_t* changed, int6
                                  uint8_t value, hipStream_t s);
 // mask[ids[t]] = value, t < k (ids outside [0, n) skipped).
 hipError_t launch_mark_ids(uint8_t* mask, int64_t n, const int32_t* ids, int64_t k, uint8_t value, hipStream_t s);
+// counts[0] = balls marked 1, counts[1] = balls marked 3 (counts zeroed here first).
+hipError_t launch_count_marks(const uint8_t* mask, int64_t n, uint32_t* counts, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* mask, int64_t n, uint8_t value, hipStream_t s);
 // mask[i] = 0 where where[i] != 0.
 hipError_t launch_mask_clear(uint8_t* mask, const uint8_t* where, int64_t n, hipStream_t s);

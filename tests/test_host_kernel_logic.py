@@ -143,3 +143,33 @@ def test_adjoint_matches_dual_numbers(oracle, n, seed):
         for axis in range(3):
             _, de = oracle.energy_dual(balls, 1.4, types, W, ball, axis)
             assert abs(g[ball, axis] - de) <= 1e-7 * max(1.0, abs(de)), (ball, axis, g[ball, axis], de)
+
+
+def test_single_precision_cut_distance_bound():
+    """The device's cut-distance test (`quotient_reaches` in contour.h) lets a
+    single-precision estimate of l / sqrt(t) decide outside a 1e-5 relative
+    margin around the circle radius. Its error: l and t rounded to single, a
+    reciprocal square root within 2 ulp (the hardware's is within 1), one
+    single product. Over the domain where the estimate is used (t and l in
+    [1e-30, 1e15], t <= 1) it stays within 5e-7 of the exact double quotient,
+    so an estimate beyond the margin can never disagree with the exact test."""
+    rng = np.random.default_rng(7)
+    t = 10.0 ** rng.uniform(-30, 0, 200000)
+    l = 10.0 ** rng.uniform(-30, 15, 200000)
+    exact = l / np.sqrt(t)
+    tf = t.astype(np.float32)
+    rs = (1.0 / np.sqrt(tf.astype(np.float64))).astype(np.float32)
+    worst = 0.0
+    for ulps in (-2, 0, 2):
+        r = rs
+        for _ in range(abs(ulps)):
+            r = np.nextafter(r, np.float32(np.inf) if ulps > 0 else np.float32(0))
+        est = (l.astype(np.float32) * r).astype(np.float64)
+        ok = np.isfinite(est)
+        assert ok.all()
+        worst = max(worst, float(np.max(np.abs(est / exact - 1.0))))
+    assert worst < 5e-7
+    # and so the decision outside the margin is the exact one
+    radius = exact * (1.0 + rng.uniform(-3e-5, 3e-5, exact.size))
+    decided = (est > radius * (1.0 + 1e-5)) | (est < radius * (1.0 - 1e-5))
+    assert np.array_equal((est > radius * (1.0 + 1e-5))[decided], (exact >= radius)[decided])

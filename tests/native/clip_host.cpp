@@ -14,6 +14,7 @@
 #include "adjoint.h"
 #include "contour.h"
 #include "group_clip.h"
+#include "scan_total.h"
 
 namespace {
 void hexagon(double* s, double* c) {
@@ -518,5 +519,28 @@ int clip_host_records(const double* sph4, const int64_t* nb_off, const int32_t* 
     flags[t] = (r.flags & 3u) | extra;
   }
   return 0;
+}
+
+// The pair total as the device scan forms it (launch_scan_u32 in k_grid.hip):
+// 32-bit tile sums of 2048 counts, added 256 tiles per round into a
+// ScanTotal, then checked_pair_total on the (32-bit total, beyond-int32)
+// report words. Returns that result (-1: the run is refused) and the 32-bit
+// total the offsets would hold.
+int64_t scan_pair_guard(const uint32_t* counts, int64_t n, uint32_t* total32) {
+  const int64_t tile = 2048, round = 256;
+  std::vector<uint32_t> tiles;
+  for (int64_t b = 0; b < n; b += tile) {
+    uint32_t s = 0;
+    for (int64_t i = b; i < n && i < b + tile; ++i) s += counts[i];
+    tiles.push_back(s);
+  }
+  vltr::ScanTotal carry;
+  for (size_t b = 0; b < tiles.size(); b += round) {
+    uint32_t tot = 0;
+    for (size_t k = b; k < tiles.size() && k < b + round; ++k) tot += tiles[k];
+    carry.add(tot);
+  }
+  *total32 = carry.lo;
+  return vltr::checked_pair_total(carry.lo, carry.beyond_int32());
 }
 }

@@ -173,3 +173,33 @@ def test_single_precision_cut_distance_bound():
     radius = exact * (1.0 + rng.uniform(-3e-5, 3e-5, exact.size))
     decided = (est > radius * (1.0 + 1e-5)) | (est < radius * (1.0 - 1e-5))
     assert np.array_equal((est > radius * (1.0 + 1e-5))[decided], (exact >= radius)[decided])
+
+
+@pytest.mark.parametrize("n,per,extra,ok", [
+    (100000, 24, 0, True),            # the 100k workload's size
+    (524287, 4096, 4095, True),       # exactly INT32_MAX pairs
+    (524288, 4096, 0, False),         # 2^31 pairs
+    (1100000, 4096, 0, False),        # 4.5e9 pairs: the 32-bit total wraps to 2.1e8
+    (2097152, 2048, 0, False),        # exactly 2^32 pairs: the 32-bit total wraps to 0
+])
+def test_pair_total_guard(n, per, extra, ok):
+    """The candidate-pair total is refused beyond INT32_MAX (pair indices are
+    int32), also where the 32-bit scan total has wrapped to a small value.
+    Synthetic per-ball counts (at most 4096 each, as a ball has at most 4096
+    listed neighbors) through the scan's own total arithmetic."""
+    import ctypes as C
+
+    L = hostclip._load()
+    f = L.scan_pair_guard
+    f.restype = C.c_int64
+    f.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_uint32)]
+    counts = np.full(n, per, np.uint32)
+    counts[-1] += extra
+    t32 = C.c_uint32()
+    got = f(counts.ctypes.data, n, C.byref(t32))
+    exact = int(counts.astype(np.uint64).sum())
+    assert t32.value == exact % (1 << 32)
+    if ok:
+        assert got == exact
+    else:
+        assert got == -1, f"total {exact} (32-bit {t32.value}) was accepted"

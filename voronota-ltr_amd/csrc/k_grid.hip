@@ -616,18 +616,26 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* in, 
   if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
+// Tile sums -> tile offsets; sums[nb] = the total modulo 2^32 and
+// sums[nb + 1] = 1 when the exact total exceeds INT32_MAX. A tile sums at
+// most 2048 counts of at most 4096, so 256 tile sums add up to at most 2^31
+// and each round's block total is exact in 32 bits; the running total is
+// kept in 64 bits (ScanTotal).
 __global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, int64_t nb) {
   __shared__ uint32_t sh[kScanBlock / kWave];
-  uint32_t carry = 0;
+  ScanTotal carry;
   for (int64_t base = 0; base < nb; base += kScanBlock) {
     const int64_t i = base + threadIdx.x;
     const uint32_t v = i < nb ? sums[i] : 0u;
     uint32_t tot;
     const uint32_t ex = block_exclusive_scan(v, &tot, sh);
-    if (i < nb) sums[i] = carry + ex;
-    carry += tot;
+    if (i < nb) sums[i] = carry.lo + ex;
+    carry.add(tot);
   }
-  if (threadIdx.x == 0) sums[nb] = carry;
+  if (threadIdx.x == 0) {
+    sums[nb] = carry.lo;
+    sums[nb + 1] = carry.beyond_int32();
+  }
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t* data, int64_t n, const uint32_t* sums) {
@@ -656,9 +664,14 @@ inline int64_t blocks_for(int64_t n, int64_t per) { return (n + per - 1) / per; 
 
 int64_t scan_tmp_words(int64_t n) { return blocks_for(n, kScanTile) + 2; }
 
+const uint32_t* scan_beyond_word(const uint32_t* tmp, int64_t n) {
+  return tmp + (n > 0 ? blocks_for(n, kScanTile) : 0) + 1;
+}
+
 hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream_t s) {
   if (n <= 0) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counts, 1, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, tmp, 2, 0u);
     return hipGetLastError();
   }
   const int64_t nb = blocks_for(n, kScanTile);

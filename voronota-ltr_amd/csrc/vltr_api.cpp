@@ -297,6 +297,21 @@ int enqueue_front(vltr_context* ctx, const Pipeline& p, bool events, hipStream_t
   if (events) VLTR_HIP(hipEventRecord(ctx->ev[1], s));
   VLTR_HIP(launch_neighbors(p, s));
   VLTR_HIP(launch_scan_u32(p.pair_cnt, p.n, ctx->scan_tmp.as<uint32_t>(), s));
+  // the pair total and whether its exact value exceeds INT32_MAX, read
+  // before any pair buffer is sized from it
+  VLTR_HIP(launch_copy_word(p.status + kReportPairs, p.pair_cnt + p.n, s));
+  VLTR_HIP(launch_copy_word(p.status + kReportPairsBeyond, scan_beyond_word(ctx->scan_tmp.as<uint32_t>(), p.n), s));
+  return VLTR_OK;
+}
+
+// The pair total of a run whose front half has completed (report block on
+// the host), or an error when the pairs cannot be indexed by int32.
+int checked_pairs(const uint32_t* host_words, int64_t* P) {
+  *P = checked_pair_total(host_words[kReportPairs], host_words[kReportPairsBeyond]);
+  if (*P < 0)
+    return fail(VLTR_ERR_INVALID_ARGUMENT,
+                "more than 2147483647 candidate pairs in one call (pair indices are 32-bit): split the input "
+                "into smaller calls or batches");
   return VLTR_OK;
 }
 
@@ -438,12 +453,12 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     if (rc != VLTR_OK) return rc;
     p.n_pairs = p.pair_cap;
     p.n_pairs_dev = p.pair_cnt + n;
+    p.n_pairs_beyond_dev = p.status + kReportPairsBeyond;
     VLTR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     rc = [&]() -> int {
       VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, gc->src, gc->src_bytes, hipMemcpyDeviceToDevice, s));
       int r = enqueue_front(ctx, p, false, s);
       if (r != VLTR_OK) return r;
-      VLTR_HIP(launch_copy_word(p.status + kReportPairs, p.pair_cnt + n, s));
       return enqueue_back(ctx, p, extras, batched, false, host_words, s);
     }();
     hipGraph_t graph = nullptr;
@@ -469,7 +484,6 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     fill_pointers(ctx, p);
     const int rc = enqueue_front(ctx, p, true, s);
     if (rc != VLTR_OK) return rc;
-    VLTR_HIP(launch_copy_word(p.status + kReportPairs, p.pair_cnt + n, s));
     VLTR_HIP(hipMemcpyAsync(host_words, p.status, kReportWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     VLTR_HIP(hipEventRecord(ctx->ev[2], s));
     VLTR_HIP(hipStreamSynchronize(s));
@@ -483,8 +497,10 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     break;
   }
   if (extras && extras->front_check && !extras->front_check(host_words)) return kRunAborted;
-  const int64_t P = host_words[kReportPairs];
-  int rc = ensure_back(ctx, p, P, n, extras);
+  int64_t P = 0;
+  int rc = checked_pairs(host_words, &P);
+  if (rc != VLTR_OK) return rc;
+  rc = ensure_back(ctx, p, P, n, extras);
   if (rc != VLTR_OK) return rc;
   p.n_pairs = P;
   rc = enqueue_back(ctx, p, extras, batched, true, host_words, s);
@@ -497,6 +513,8 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     ctx->nb_cap *= 2;
     goto restart;
   }
+  if (host_words[kReportStatus] & kStatPairOverflow)
+    return fail(VLTR_ERR_INTERNAL, "pair lists outgrew the buffers sized from the pair total");
   finish_run(ctx, p, P, batched ? static_cast<size_t>(n_structs) : 0, n_contacts, n_cells);
   float ms = 0.0f;
   for (int k = 0; k < kPhases - 1; ++k) {
@@ -952,7 +970,7 @@ int run_batched_graph(vltr_context* ctx, const vltr_ball* d_balls, const uint64_
   VLTR_HIP(hipEventRecord(ctx->ev[kPhases - 1], s));
   VLTR_HIP(hipStreamSynchronize(s));
   const uint32_t* hw = static_cast<const uint32_t*>(ctx->pinned.ptr);
-  if (static_cast<int64_t>(hw[kReportPairs]) > ctx->graph_cap ||
+  if (static_cast<int64_t>(hw[kReportPairs]) > ctx->graph_cap || hw[kReportPairsBeyond] != 0u ||
       (hw[kReportStatus] & (kStatNbOverflow | kStatNbTooLong | kStatPairOverflow))) {
     // more pairs or longer neighbor lists than the capture holds: a direct
     // run (which drops the graph; the next call captures again)

@@ -9,6 +9,7 @@
 
 #include "contour.h"
 #include "group_clip.h"
+#include "scan_total.h"
 #include "vltr_geom.h"
 #include "voronota_ltr.h"
 
@@ -56,7 +57,8 @@ enum ReportWord : int {
                         // after the pairs were counted (k_updateable_cells)
   kReportMarksOld = 8,      // updateable step: balls marked as old neighbors (k_count_marks,
   kReportMarksChanged = 9,  // written before the run) and as changed
-  kReportWords = 10
+  kReportPairsBeyond = 10,  // 1: the exact pair total exceeds INT32_MAX (kReportPairs wrapped)
+  kReportWords = 11
 };
 
 // Launch parameters of the tessellation pipeline for one call (one or many
@@ -132,12 +134,18 @@ struct Pipeline {
   // earlier run), kernels over pairs run to n_pairs = pair_cap and stop at
   // the device-side total *n_pairs_dev. Null in direct runs (n_pairs exact).
   const uint32_t* n_pairs_dev;
+  // Graph replays: the pair scan's beyond-int32 word (nonzero: *n_pairs_dev
+  // wrapped and no pair offset can be trusted).
+  const uint32_t* n_pairs_beyond_dev;
 };
 
-// A graph replay whose pair total outgrew the capacity it was captured with:
-// the pair kernels do nothing and the host reruns without the graph.
+// A graph replay whose pair total outgrew the capacity it was captured with
+// (or INT32_MAX, where the 32-bit total wraps): the pair kernels do nothing
+// and the host reruns without the graph.
 __device__ __forceinline__ bool pairs_overflowed(const Pipeline& p) {
-  return p.n_pairs_dev != nullptr && static_cast<int64_t>(*p.n_pairs_dev) > p.pair_cap;
+  return p.n_pairs_dev != nullptr &&
+         (static_cast<int64_t>(*p.n_pairs_dev) > p.pair_cap ||
+          (p.n_pairs_beyond_dev != nullptr && *p.n_pairs_beyond_dev != 0u));
 }
 
 // tessellation.rs:323-328
@@ -163,6 +171,9 @@ hipError_t launch_neighbors(const Pipeline& p, hipStream_t s);
 hipError_t launch_pairs(const Pipeline& p, hipStream_t s);
 // Exclusive scan of counts[0..n) in place into counts[0..n], counts[n] = total.
 hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream_t s);
+// The word of tmp where launch_scan_u32 over n counts leaves 1 when the exact
+// total exceeds INT32_MAX (ScanTotal), else 0.
+const uint32_t* scan_beyond_word(const uint32_t* tmp, int64_t n);
 // *dst = *src on the device (one word, stream-ordered).
 hipError_t launch_copy_word(uint32_t* dst, const uint32_t* src, hipStream_t s);
 int64_t scan_tmp_words(int64_t n);

@@ -15,6 +15,7 @@
 // counts only if it is that first one, found through a hash table of
 // canonical pairs filled by k_dedup_insert (smallest pair index per key).
 #include "adjoint.h"
+#include "dedup.h"
 #include "vltr_internal.h"
 #include "wave_group.h"
 
@@ -40,30 +41,6 @@ struct DevAcc {
   }
 };
 
-// Open-addressing table of canonical pairs: key lo * n + hi + 1 (0 = empty),
-// first = smallest pair index with that key. Sized to at least twice the
-// candidate pairs, so every probe sequence reaches its key or an empty slot.
-struct DedupTable {
-  unsigned long long* keys;
-  uint32_t* first;
-  uint64_t mask;
-};
-
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdull;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ull;
-  x ^= x >> 33;
-  return x;
-}
-
-__device__ __forceinline__ unsigned long long pair_key(const Pipeline& p, int64_t t) {
-  const int64_t ca = p.pair_a[t] % p.n, cb = p.pair_b[t] % p.n;
-  const int64_t lo = ca < cb ? ca : cb, hi = ca < cb ? cb : ca;
-  return static_cast<unsigned long long>(lo) * static_cast<unsigned long long>(p.n) + hi + 1ull;
-}
-
 __global__ void k_dedup_clear(DedupTable T) {
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i <= T.mask;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -87,13 +64,6 @@ __global__ void k_dedup_insert(Pipeline p, DedupTable T) {
     }
     slot = (slot + 1) & T.mask;
   }
-}
-
-__device__ bool dedup_first(const Pipeline& p, const DedupTable& T, int64_t t) {
-  const unsigned long long key = pair_key(p, t);
-  uint64_t slot = mix64(key) & T.mask;
-  while (T.keys[slot] != key) slot = (slot + 1) & T.mask;
-  return T.first[slot] == static_cast<uint32_t>(t);
 }
 
 __global__ void k_zero_u32(uint32_t* a, int64_t n) {
@@ -152,13 +122,21 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(VLTR_
     const ContactTopo& topo = p.topo[t];
     const int32_t a = p.pair_a[t], b = p.pair_b[t];
     const int64_t ca = a % p.n, cb = b % p.n;
-    const bool counted = !p.periodic || b < p.n || dedup_first(p, T, t);
-    const double w = counted ? ws[4 * types[ca] + types[cb]] : 0.0;
-    const bool b_cell = b < p.n;  // an image end has no cell share
+    const bool image = p.periodic && b >= p.n;
+    const bool first = image && dedup_first(p, T, t);
+    const double w = !image || first ? ws[4 * types[ca] + types[cb]] : 0.0;
+    // cell shares: each ball's own records (an image end has no share), or,
+    // with dedup cells, the first record of a canonical pair through an
+    // image for both ends and the others for neither
+    bool a_cell = true, b_cell = b < p.n;
+    if (p.dd_keys != nullptr && image) {
+      a_cell = first;
+      b_cell = first && cb != ca;
+    }
     if (ln == 0) e += w * rec.area;
-    group_contact_adjoint(G, &lds[gl], sph4, a, b, topo, w, p.seed[ca], b_cell ? p.seed[cb] : 0.0,
-                          rec.solid_a < 0.0 ? -1.0 : 1.0, b_cell ? (rec.solid_b < 0.0 ? -1.0 : 1.0) : 0.0, acc,
-                          p.volume_weight);
+    group_contact_adjoint(G, &lds[gl], sph4, a, b, topo, w, a_cell ? p.seed[ca] : 0.0, b_cell ? p.seed[cb] : 0.0,
+                          a_cell ? (rec.solid_a < 0.0 ? -1.0 : 1.0) : 0.0,
+                          b_cell ? (rec.solid_b < 0.0 ? -1.0 : 1.0) : 0.0, acc, p.volume_weight);
   }
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < p.n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -179,11 +157,14 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(VLTR_
 
 int64_t adjoint_blocks() { return kAdjBlocks; }
 
-uint64_t dedup_slots(int64_t n_pairs) {
-  uint64_t h = 1024;
-  while (h < 2 * static_cast<uint64_t>(n_pairs > 0 ? n_pairs : 1)) h <<= 1;
-  return h;
+hipError_t launch_dedup_table(const Pipeline& p, unsigned long long* keys, uint32_t* first, hipStream_t s) {
+  DedupTable T{keys, first, dedup_slots(p.n_pairs) - 1};
+  hipLaunchKernelGGL(k_dedup_clear, dim3(1024), dim3(256), 0, s, T);
+  if (p.n_pairs > 0) hipLaunchKernelGGL(k_dedup_insert, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, T);
+  return hipGetLastError();
 }
+
+uint64_t dedup_slots(int64_t n_pairs) { return dedup_slots_mask(n_pairs) + 1; }
 
 hipError_t launch_adjoint(const Pipeline& p, const int32_t* types, const double* w16, double* grad, double* part,
                           unsigned long long* dedup_keys, uint32_t* dedup_first, const AdjointWork& work,
@@ -191,16 +172,16 @@ hipError_t launch_adjoint(const Pipeline& p, const int32_t* types, const double*
   W16 W;
   for (int k = 0; k < 16; ++k) W.w[k] = w16[k];
   DedupTable T{dedup_keys, dedup_first, dedup_slots(p.n_pairs) - 1};
+  if (p.dd_keys != nullptr) T = DedupTable{p.dd_keys, p.dd_first, dedup_slots(p.n_pairs) - 1};
   if (p.n_pairs > 0) {
     hipLaunchKernelGGL(k_adj_flag, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, work.flags);
     const hipError_t e = launch_scan_u32(work.flags, p.n_pairs, work.scan_tmp, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_adj_list, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, work.flags, work.list);
   }
-  if (p.periodic) {
-    hipLaunchKernelGGL(k_dedup_clear, dim3(1024), dim3(256), 0, s, T);
-    if (p.n_pairs > 0)
-      hipLaunchKernelGGL(k_dedup_insert, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, T);
+  if (p.periodic && p.dd_keys == nullptr) {
+    const hipError_t e = launch_dedup_table(p, dedup_keys, dedup_first, s);
+    if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, grad, 3 * p.n);
   // no pairs: the scanned total flags[0] is not written; the list is empty

@@ -7,6 +7,7 @@
 // ascending a, and then its own pairs in order. k_cells reproduces exactly
 // that order per ball, so sums are bit-identical to the sequential reference
 // (no atomics, no reduction-order drift).
+#include "dedup.h"
 #include "vltr_internal.h"
 #include "voronota_ltr.h"
 
@@ -59,12 +60,19 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   const int f = lane < 6 ? lane : 0;  // the sum this lane keeps (lanes 0-5)
   double sum = 0.0;
   bool added = false;
-  // adds the window's records with a contact to this lane's sum, in order
-  auto add_window = [&](int m, bool as_first) {
+  // dedup cells: an own record through an image counts only when it is the
+  // first of its canonical pair (the first records of other rows came in
+  // through the incoming list)
+  const DedupTable T{p.dd_keys, p.dd_first, p.dd_keys != nullptr ? dedup_slots_mask(p.n_pairs) : 0ull};
+  // adds the window's records with a contact to this lane's sum, in order;
+  // own_base >= 0: the window holds own records own_base, own_base + 1, ...
+  auto add_window = [&](int m, bool as_first, int64_t own_base) {
     bool valid = false;
     if (lane < m) {
       const PairRec& r = win[w][lane];
       valid = (r.flags & kRecValid) && r.area > 0.0;
+      if (valid && own_base >= 0 && p.dd_keys != nullptr && p.pair_b[own_base + lane] >= p.n)
+        valid = dedup_first(p, T, own_base + lane);
     }
     unsigned long long vm = __ballot(valid);
     added = added || vm != 0ull;
@@ -93,7 +101,7 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
     for (int k = 0; k < nin; ++k) rank += __builtin_amdgcn_readlane(pe, k) < pe ? 1 : 0;
     if (lane < nin) win[w][rank] = p.rec[pe];
     wave_sync();
-    add_window(nin, false);
+    add_window(nin, false, -1);
     wave_sync();
   } else {
     for (int c0 = 0; c0 < nin; c0 += kWave) {
@@ -104,7 +112,7 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
         if (rank >= c0 && rank < c0 + kWave) win[w][rank - c0] = p.rec[pe];
       }
       wave_sync();
-      add_window(min(kWave, nin - c0), false);
+      add_window(min(kWave, nin - c0), false, -1);
       wave_sync();
     }
   }
@@ -118,7 +126,7 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
       win[w][lane] = p.rec[own0 + c0 + lane];
     }
     wave_sync();
-    add_window(min(kWave, nown - c0), true);
+    add_window(min(kWave, nown - c0), true, own0 + c0);
     wave_sync();
   }
   Acc acc;
@@ -174,6 +182,25 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   }
   p.cell[b] = out;
   if (p.seed != nullptr) p.seed[b] = seed;
+}
+
+// Dedup cells: the first record of a canonical pair through an image also
+// counts for its second end's canonical ball (updateable.rs:565-630 stores it
+// in both balls' lists), which k_cells reads from its incoming list.
+__global__ void k_dedup_incoming(Pipeline p, uint64_t mask) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= p.n_pairs) return;
+  const int64_t b = p.pair_b[t];
+  if (b < p.n) return;
+  const PairRec& r = p.rec[t];
+  if (!(r.flags & kRecValid) || !(r.area > 0.0)) return;
+  const int64_t cb = b % p.n;
+  if (cb == p.pair_a[t] || !dedup_first(p, DedupTable{p.dd_keys, p.dd_first, mask}, t)) return;
+  const uint32_t slot = atomicAdd(&p.in_cnt[cb], 1u);
+  if (slot < static_cast<uint32_t>(p.nb_cap))
+    p.incoming[cb * p.nb_cap + slot] = static_cast<int32_t>(t);
+  else
+    atomicOr(p.status, kStatNbOverflow);
 }
 
 __global__ void k_flag_contacts(const PairRec* rec, int64_t n, const uint32_t* n_dev, uint32_t* flag) {
@@ -359,6 +386,13 @@ __global__ void k_mask_clear(uint8_t* mask, const uint8_t* where, int64_t n) {
 inline int64_t blocks_for(int64_t n, int64_t per) { return (n + per - 1) / per; }
 
 }  // namespace
+
+hipError_t launch_dedup_incoming(const Pipeline& p, hipStream_t s) {
+  if (p.n_pairs > 0)
+    hipLaunchKernelGGL(k_dedup_incoming, dim3(blocks_for(p.n_pairs, 256)), dim3(256), 0, s, p,
+                       dedup_slots_mask(p.n_pairs));
+  return hipGetLastError();
+}
 
 hipError_t launch_cells(const Pipeline& p, hipStream_t s) {
   if (p.n > 0) hipLaunchKernelGGL(k_cells, dim3(blocks_for(p.n, kCellsWaves)), dim3(kCellsBlock), 0, s, p);

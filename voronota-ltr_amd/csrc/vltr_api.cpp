@@ -239,6 +239,10 @@ struct RunExtras {
   // called with the report block once the neighbor lists are complete; false
   // ends the run there (run_pipeline returns kRunAborted)
   std::function<bool(const uint32_t*)> front_check;
+  // dedup cells (incremental steps of a periodic updateable): the
+  // canonical-pair table's buffers (Pipeline::dd_keys)
+  DevBuf* dd_keys = nullptr;
+  DevBuf* dd_first = nullptr;
 };
 
 // run_pipeline's return value when RunExtras::front_check ended the run.
@@ -278,6 +282,11 @@ int ensure_back(vltr_context* ctx, Pipeline& p, int64_t cap, int64_t n, const Ru
   if (extras && extras->topo_buf) {
     VLTR_HIP(extras->topo_buf->ensure(p.pair_cap * sizeof(ContactTopo)));
     p.topo = extras->topo_buf->as<ContactTopo>();
+  }
+  if (extras && extras->dd_keys && extras->dd_first) {
+    const uint64_t slots = dedup_slots(p.pair_cap);
+    VLTR_HIP(extras->dd_keys->ensure(slots * sizeof(unsigned long long)));
+    VLTR_HIP(extras->dd_first->ensure(slots * sizeof(uint32_t)));
   }
   if (extras && extras->net_buf && extras->net_out) {
     VLTR_HIP(extras->net_buf->ensure(p.pair_cap * sizeof(ContactNet)));
@@ -335,6 +344,12 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunExtras* extras, bool b
   if (events) VLTR_HIP(hipEventRecord(ctx->ev[4], s));
   VLTR_HIP(launch_fallbacks(p, s));
   if (events) VLTR_HIP(hipEventRecord(ctx->ev[5], s));
+  if (p.mode == 0 && p.periodic && extras && extras->dd_keys && extras->dd_first) {
+    p.dd_keys = extras->dd_keys->as<unsigned long long>();
+    p.dd_first = extras->dd_first->as<uint32_t>();
+    VLTR_HIP(launch_dedup_table(p, p.dd_keys, p.dd_first, s));
+    VLTR_HIP(launch_dedup_incoming(p, s));
+  }
   if (p.mode == 0) VLTR_HIP(launch_cells(p, s));
   if (p.mode == 0 && extras && extras->post_cells) VLTR_HIP(extras->post_cells(p, s));
   uint32_t* flags = ctx->flags.as<uint32_t>();
@@ -1683,6 +1698,14 @@ int upd_run(Upd* u, const RunOpts& o) {
     ex.prev_pair_b = u->prev_pair_b.as<int32_t>();
     ex.prev_rec = u->prev_rec.as<PairRec>();
     ex.prev_topo = adj ? u->prev_topo.as<ContactTopo>() : nullptr;
+  }
+  // updateable.rs:691-727: an incremental step recomputes the affected cells
+  // from the deduplicated per-ball contact lists (in a periodic box, of each
+  // canonical pair through an image only its first record in pair order);
+  // init and full reinits sum every ball's own records (updateable.rs:633-689)
+  if (!full && st.periodic) {
+    ex.dd_keys = &u->dedup_keys;
+    ex.dd_first = &u->dedup_first;
   }
   ex.post_cells = [=](const Pipeline& p, hipStream_t st2) -> hipError_t {
     hipError_t e = launch_updateable_cells(p, state, excl, full ? nullptr : am, full ? 1 : 0, refresh ? 1 : 0, st2);

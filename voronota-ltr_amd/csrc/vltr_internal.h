@@ -137,6 +137,17 @@ struct Pipeline {
   // Graph replays: the pair scan's beyond-int32 word (nonzero: *n_pairs_dev
   // wrapped and no pair offset can be trusted).
   const uint32_t* n_pairs_beyond_dev;
+
+  // Dedup cells (incremental steps of a periodic updateable,
+  // updateable.rs:476-512 with 691-727): a cell sums the contacts of the
+  // deduplicated per-ball lists, i.e. of each canonical pair through an image
+  // only the first record in pair order, for both of its ends. Non-null: the
+  // canonical-pair table (launch_dedup_table) of this run's records; such
+  // first records were appended to the second end's incoming list
+  // (launch_dedup_incoming). Null: every ball sums its own records
+  // (tessellation.rs:607-658).
+  unsigned long long* dd_keys;
+  uint32_t* dd_first;
 };
 
 // A graph replay whose pair total outgrew the capacity it was captured with
@@ -197,6 +208,10 @@ hipError_t upload_hexagon_constants(hipStream_t s);
 // per block into part[0 .. adjoint_blocks()).
 // Slots of the canonical-pair table (periodic runs): a power of two >= 2 n_pairs.
 uint64_t dedup_slots(int64_t n_pairs);
+// Fills the canonical-pair table (dedup_slots(p.n_pairs) slots) from this
+// run's records: per canonical pair through an image, the smallest pair index
+// with a contact.
+hipError_t launch_dedup_table(const Pipeline& p, unsigned long long* keys, uint32_t* first, hipStream_t s);
 // Scratch of the adjoint: flags (n_pairs + 1 words) and scan_tmp as for
 // launch_scan_u32 over n_pairs, list (n_pairs entries): the contacts the
 // adjoint visits.
@@ -223,6 +238,9 @@ hipError_t launch_solvent_atoms(const double4* balls, int64_t n, double probe, d
 
 // k_cells.hip
 hipError_t launch_cells(const Pipeline& p, hipStream_t s);
+// Dedup cells: appends every first record through an image to the incoming
+// list of its second end's canonical ball (p.dd_keys set, table filled).
+hipError_t launch_dedup_incoming(const Pipeline& p, hipStream_t s);
 hipError_t launch_flag_contacts(const Pipeline& p, uint32_t* flag, hipStream_t s);
 hipError_t launch_emit_contacts(const Pipeline& p, const uint32_t* pos, vltr_contact* out, hipStream_t s);
 hipError_t launch_flag_cells(const Pipeline& p, uint32_t* flag, hipStream_t s);

@@ -13,6 +13,7 @@ plausible restatement error (dropped term, wrong sign, swapped operand) fails:
   * forward-mode dual derivative against central differences.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -260,3 +261,28 @@ def test_counted_flops_are_deterministic(oracle):
     f2 = oracle.count_flops(balls, 1.4, mode=0)
     s = oracle.count_flops(balls, 1.4, mode=2)
     assert f1 == f2 and f1[0] > s[0] > 0
+
+
+def test_work_counts_fixture(oracle):
+    """tests/golden/data/work_counts.json (the algorithmic work bench.py
+    prices the clip kernel with) is what the counting scalar gives now: the
+    20k workload recounted in full, the 100k one in its full-mode total."""
+    import json
+
+    from voronota_ltr import workload
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "data", "work_counts.json")) as f:
+        wl = json.load(f)["workloads"]
+    w = wl["rsa_sphere_20000"]
+    balls, _ = workload.rsa_sphere(w["balls"], seed=w["seed"])
+    sfl, stops = oracle.count_flops(balls, w["probe"], mode=2)
+    assert (sfl, stops) == (w["mode0"]["flops_search"], w["mode0"]["tops_search"])
+    for mode in (0, 1):
+        fl, tops = oracle.count_flops(balls, w["probe"], mode=mode)
+        assert (fl, tops) == (w[f"mode{mode}"]["flops_total"], w[f"mode{mode}"]["tops_total"])
+        assert w[f"mode{mode}"]["flops_contacts"] == fl - sfl
+    off, idx, _, _, pairs = oracle.collisions(balls, w["probe"])
+    assert len(pairs) == w["candidate_pairs"]
+    w = wl["rsa_sphere_100000"]
+    balls, _ = workload.rsa_sphere(w["balls"], seed=w["seed"])
+    assert oracle.count_flops(balls, w["probe"], mode=0) == (w["mode0"]["flops_total"], w["mode0"]["tops_total"])

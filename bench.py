@@ -23,16 +23,59 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "voronota-ltr_amd"))
 
 METRIC = "balls/s full tessellation f64 @100k: plain vs batched; fwd+adjoint/fwd ratio"
-# Algorithmic FP64 work of one candidate pair (clip + solid angles + pyramid
-# volumes), counted by the oracle's counting scalar on this workload
-# (add/sub/mul/div/sqrt = 1, fma = 2; acos/sin/cos excluded).
-FLOP_PER_PAIR = 2914.39
+# Algorithmic work per workload (FLOPs: add/sub/mul/div/sqrt = 1, fma = 2;
+# acos/sin/cos counted separately as T-ops), counted by the oracle's counting
+# scalar and committed as data by tests/golden/make_work_counts.py.
+WORK_COUNTS = os.path.join(ROOT, "tests", "golden", "data", "work_counts.json")
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-# HBM bytes one full-mode k_clip launch moves on the default workload (100,000
-# balls, seed 100000), per launch shape: FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB,
-# factors from a 256 MiB device copy; profiles/k_clip_100k_hbm_bytes.txt.
+# HBM bytes one full-mode k_clip launch moves on the 100,000-ball workload
+# (seed 100000, probe 1.4), per launch shape: FETCH_SIZE x 2 + WRITE_SIZE x 1
+# KiB per dispatch, factors from a 256 MiB device copy read with the same
+# counters (profiles/k_clip_100k_hbm_bytes.txt). Other workloads: not measured.
 K_CLIP_TRAFFIC_BYTES_100K = {"g8w1o4n": 476.4e6, "g8w1o4": 518.9e6}
+
+
+def work_counts(n, seed, probe):
+    """The counted work of this workload, or None when it is not one of the
+    counted workloads."""
+    with open(WORK_COUNTS) as f:
+        data = json.load(f)
+    for w in data["workloads"].values():
+        if (w["balls"], w["seed"], w["probe"]) == (n, seed, probe):
+            return w
+    return None
+
+
+def host_cpu():
+    """What the CPU leg runs on: logical CPUs of the host, the CPUs this
+    process may use, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "cpus_allowed": allowed, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "cpu_model": model}
+
+
+def cpu_threads():
+    """Threads of the CPU leg: every CPU this process may use, capped by
+    OMP_NUM_THREADS when it is set (the share of the host a one-GPU job gets)."""
+    info = host_cpu()
+    t = info["cpus_allowed"]
+    omp = info["omp_num_threads"]
+    if omp and omp.isdigit() and int(omp) > 0:
+        t = min(t, int(omp))
+    return max(t, 1)
 
 
 def main():
@@ -100,16 +143,21 @@ def main():
     phases = ["grid", "neighbors", "pairs", "clip", "slow", "cells", "total"]
 
     def timed(contacts_only):
+        """Wall time of K steps (nothing else in the loop), then a separate
+        pass of K steps that reads the per-phase HIP-event times after each."""
         for _ in range(args.warmup):
             step(contacts_only)
         _hip.synchronize()
-        acc = np.zeros(len(phases))
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step(contacts_only)
-            acc += np.array([_native.last_timings()[k] for k in phases])
         _hip.synchronize()
         wall = time.perf_counter() - t0
+        acc = np.zeros(len(phases))
+        for _ in range(args.steps):
+            step(contacts_only)
+            t = _native.last_timings()
+            acc += np.array([t[k] for k in phases])
         return wall / args.steps * 1e3, dict(zip(phases, (acc / args.steps).round(4).tolist())), _native.last_counts()
 
     tune = None
@@ -119,12 +167,11 @@ def main():
         tune = autotune(lib, ctx, dbuf.ptr, np.array([0, args.n], dtype=np.uint64), args.probe)
     ms_full, ph_full, counts = timed(0)
     ms_co, ph_co, counts_co = timed(1)
+    host_api = host_api_leg(args, lib, ctx, balls)
 
     value = args.n / (ms_full / 1e3)
     shape = _native.clip_variants()[_native.clip_variant()]
-    cut_ms = ph_full["clip"]
-    flops = FLOP_PER_PAIR * counts["pairs"]
-    achieved = flops / (cut_ms / 1e3) / 1e12 if cut_ms > 0 else 0.0
+    roofline = clip_roofline(args, ph_full["clip"], counts, shape)
 
     batched = batched_leg(args, lib, ctx) if args.batch_structs > 0 else None
     dynamics = dynamics_leg(args) if args.dyn_balls > 0 else None
@@ -135,14 +182,16 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
 
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         t1 = pyoracle.time_tessellation(balls, args.probe, threads=threads, reps=1)
-        reps = max(1, int(args.cpu_seconds / max(t1, 1e-3)))
+        reps = max(5, int(args.cpu_seconds / max(t1, 1e-3)))
         tcpu = pyoracle.time_tessellation(balls, args.probe, threads=threads, reps=reps)
         t_single = pyoracle.time_tessellation(balls, args.probe, threads=1, reps=1)
         cpu = {"value": args.n / tcpu, "unit": "balls/s", "cores": threads, "kind": "port",
-               "sample": f"full compute_tessellation of the same {args.n}-ball input, {reps} run(s) "
-                         f"after one warm-up, C++ restatement (-O3, OpenMP over balls and pairs)",
+               "sample": f"full compute_tessellation of the same {args.n}-ball input, mean of {reps} runs after one "
+                         f"warm-up, C++ restatement of the reference (-O3, OpenMP over balls and pairs as Rayon "
+                         f"does), {threads} threads: every CPU this process may use, capped by OMP_NUM_THREADS",
+               "host": host_cpu(),
                "single_thread_balls_per_s": round(args.n / t_single, 1),
                "calibration": cpu_calibration(pyoracle, threads)}
 
@@ -166,21 +215,9 @@ def main():
             "balls": args.n,
             "probe": args.probe,
         },
-        "roofline": {
-            "bound": "mfma",
-            "kernel": "k_clip",
-            "achieved": round(achieved, 3),
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            # measured for this workload, per launch shape
-            "traffic": K_CLIP_TRAFFIC_BYTES_100K.get(shape)
-            if (args.n, args.seed, args.probe) == (100000, 100000, 1.4) else None,
-            "traffic_unit": "bytes per launch",
-            "note": "FP64 vector work (no matrix op on this path); flops = "
-                    f"{FLOP_PER_PAIR} per candidate pair x {counts['pairs']} pairs",
-        },
+        "roofline": roofline,
         "cpu_baseline": cpu,
+        "host_api": host_api,
         "clip_launch_shape": shape,
         "autotune": tune,
         "phases_ms": ph_full,
@@ -199,6 +236,65 @@ def main():
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
     print(json.dumps(out), flush=True)
+
+
+def clip_roofline(args, clip_ms, counts, shape):
+    """FP64 roofline of k_clip: the counted FLOPs of contact construction per
+    candidate pair (tests/golden/data/work_counts.json) times the pairs the
+    launch clips itself (candidate pairs minus those it hands to the
+    fallback kernels), over its HIP-event time on the library stream."""
+    w = work_counts(args.n, args.seed, args.probe)
+    clipped = counts["pairs"] - counts["slow_pairs"]
+    if w is None or clip_ms <= 0:
+        return {"bound": "fp64-valu", "kernel": "k_clip", "achieved": None, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": None, "traffic": None,
+                "note": "work not counted for this workload (tests/golden/make_work_counts.py)"}
+    m = w["mode0"]
+    flop_per_pair = m["flops_contacts"] / w["candidate_pairs"]
+    tops_per_pair = m["tops_contacts"] / w["candidate_pairs"]
+    flops = flop_per_pair * clipped
+    achieved = flops / (clip_ms / 1e3) / 1e12
+    traffic = K_CLIP_TRAFFIC_BYTES_100K.get(shape) if (args.n, args.seed, args.probe) == (100000, 100000, 1.4) else None
+    return {"bound": "fp64-valu", "kernel": "k_clip", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
+            "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
+            "flops_per_launch": round(flops), "flop_per_pair": round(flop_per_pair, 2),
+            "tops_per_launch": round(tops_per_pair * clipped), "tops_per_s": round(tops_per_pair * clipped /
+                                                                                 (clip_ms / 1e3), 1),
+            "clip_ms": clip_ms, "pairs_clipped": clipped,
+            "note": "FP64 vector work, no matrix op on this path. FLOPs counted by the oracle's counting scalar "
+                    "(add/sub/mul/div/sqrt = 1, fma = 2); acos/sin/cos are T-ops, reported apart"}
+
+
+def host_api_leg(args, lib, ctx, balls):
+    """End to end through the host entry point (vltr_compute_tessellation:
+    host balls in, host contacts and cells out, PCIe copies included)."""
+    import ctypes as C
+
+    import numpy as np
+
+    from voronota_ltr import _native
+
+    b = np.ascontiguousarray(balls)
+    bp = b.ctypes.data_as(C.c_void_p)
+
+    def once():
+        h = C.c_void_p()
+        _native.check(lib.vltr_compute_tessellation(ctx, bp, len(b), args.probe, None, None, 0, 0, C.byref(h)),
+                      "vltr_compute_tessellation")
+        nc = lib.vltr_result_num_contacts(h)
+        lib.vltr_result_free(h)
+        return nc
+
+    for _ in range(max(1, args.warmup)):
+        once()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        once()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    return {"balls_per_s": round(args.n / (ms / 1e3), 1), "ms_per_call": round(ms, 4),
+            "note": "vltr_compute_tessellation: host balls in, host contacts and cells out (result object built "
+                    "and freed), full mode"}
 
 
 # The reference's published times (README.md:375-381, Apple M4, 10 cores,
@@ -451,14 +547,16 @@ def dynamics_leg(args):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
 
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         o = pyoracle.Updateable(threads=threads)
         o.init(balls, args.probe, box)
-        k = 2
+        o.update_with_changed(*moves[0])  # warm-up
+        k = min(5, len(moves) - 1)
         t0 = time.perf_counter()
-        for b, ch in moves[:k]:
+        for b, ch in moves[1:1 + k]:
             o.update_with_changed(b, ch)
-        out["cpu_oracle_ms_per_step"] = round((time.perf_counter() - t0) / k * 1e3, 2)
+        out["cpu_oracle_ms_per_step"] = round((time.perf_counter() - t0) / max(k, 1) * 1e3, 2)
+        out["cpu_oracle_steps_timed"] = k
         out["cpu_cores"] = threads
     return out
 

@@ -149,6 +149,26 @@ int vltr_compute_tessellation_device(vltr_context* ctx, const vltr_ball* d_balls
 int vltr_context_device_outputs(const vltr_context* ctx, const vltr_contact** d_contacts, size_t* n_contacts,
                                 const vltr_cell** d_cells, size_t* n_cells);
 
+/* Allocation-free variant of vltr_compute_tessellation_device. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Every intermediate buffer comes from a
+ * fixed 64 MiB device workspace per GPU, a file-scope device array whose
+ * address is taken when the first context on that GPU is created; the call
+ * itself allocates nothing. Outputs go to the caller's device buffers: at
+ * most contacts_cap contacts (the count is *n_contacts; beyond the cap the
+ * call fails after writing the first contacts_cap) and, in full mode, the
+ * cells (cells_cap must be at least n). Runs that do not fit the workspace
+ * (about 20,000 balls at protein density in full mode) fail with
+ * VLTR_ERR_INVALID_ARGUMENT and a message naming the limit. Calls on one GPU
+ * share its workspace: the library serializes them (a lock held until the
+ * call's results are complete), and the workspace keeps the last call's
+ * contents. Non-periodic; ctx supplies the stream. */
+int vltr_compute_tessellation_device_noalloc(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
+                                             const int32_t* d_groups, size_t n_groups, int contacts_only,
+                                             vltr_contact* d_contacts, size_t contacts_cap, vltr_cell* d_cells,
+                                             size_t cells_cap, size_t* n_contacts, size_t* n_cells);
+
 /* Neighbor lists of the last pipeline run on ctx (diagnostics and parity
  * tests): for each of the num_balls balls, its colliding spheres sorted by
  * (distance from the ball's centre to the intersection-circle centre, index),

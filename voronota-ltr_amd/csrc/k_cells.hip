@@ -218,7 +218,7 @@ __global__ void k_emit_contacts(Pipeline p, const uint32_t* pos, vltr_contact* o
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= p.n_pairs) return;
   const uint32_t k = pos[i];
-  if (pos[i + 1] == k) return;
+  if (pos[i + 1] == k || (p.contacts_cap > 0 && static_cast<int64_t>(k) >= p.contacts_cap)) return;
   const PairRec r = p.rec[i];
   // ids local to the ball's structure (batched runs)
   const int32_t a = p.pair_a[i];

@@ -20,6 +20,7 @@
 #include <string>
 #include <unordered_map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "vltr_host.h"
@@ -96,6 +97,12 @@ struct HostPinned {
 };
 
 constexpr int kPhases = 7;
+// Neighbor slots per ball a run starts with (doubled when a list outgrows them).
+constexpr int kNbCapDefault = 96;
+// Pinned host block of a context: the run's report words at offset 0, the
+// allocation-free entry point's cell-hash segment staged at kPinnedSegOff.
+constexpr size_t kPinnedBytes = 256;
+constexpr size_t kPinnedSegOff = 128;
 
 }  // namespace
 
@@ -104,7 +111,7 @@ struct vltr_context {
   hipStream_t stream = nullptr;
   hipEvent_t ev[kPhases + 1] = {};
   double timings[kPhases] = {};
-  int nb_cap = 96;
+  int nb_cap = kNbCapDefault;
   // device buffers
   DevBuf balls, groups, sph, seg_of, segs, bkey, brank, bcnt, sorted_id, sorted_sph, small, nb, nb_cnt, excluded,
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
@@ -296,20 +303,35 @@ int ensure_back(vltr_context* ctx, Pipeline& p, int64_t cap, int64_t n, const Ru
   return VLTR_OK;
 }
 
+// Scan scratch, contact flags and the compacted outputs of one run: the
+// context's buffers, or regions of the fixed workspace and the caller's
+// output buffers (the allocation-free entry point).
+struct RunBufs {
+  uint32_t* scan_tmp;
+  uint32_t* flags;
+  vltr_contact* contacts;
+  vltr_cell* cells;
+  uint32_t* seg_out;  // batched runs only
+};
+RunBufs ctx_bufs(vltr_context* ctx) {
+  return RunBufs{ctx->scan_tmp.as<uint32_t>(), ctx->flags.as<uint32_t>(), ctx->out_contacts.as<vltr_contact>(),
+                 ctx->out_cells.as<vltr_cell>(), ctx->seg_out.as<uint32_t>()};
+}
+
 // Front half: cell-list build and neighbor lists with their scanned pair
 // counts (pair_cnt[n] = the pair total).
-int enqueue_front(vltr_context* ctx, const Pipeline& p, bool events, hipStream_t s) {
+int enqueue_front(vltr_context* ctx, const Pipeline& p, const RunBufs& b, bool events, hipStream_t s) {
   VLTR_HIP(launch_prepare(p, s));
   VLTR_HIP(launch_grid(p, s));
-  VLTR_HIP(launch_scan_u32(p.bucket_cnt, p.n_buckets, ctx->scan_tmp.as<uint32_t>(), s));
+  VLTR_HIP(launch_scan_u32(p.bucket_cnt, p.n_buckets, b.scan_tmp, s));
   VLTR_HIP(launch_scatter(p, s));
   if (events) VLTR_HIP(hipEventRecord(ctx->ev[1], s));
   VLTR_HIP(launch_neighbors(p, s));
-  VLTR_HIP(launch_scan_u32(p.pair_cnt, p.n, ctx->scan_tmp.as<uint32_t>(), s));
+  VLTR_HIP(launch_scan_u32(p.pair_cnt, p.n, b.scan_tmp, s));
   // the pair total and whether its exact value exceeds INT32_MAX, read
   // before any pair buffer is sized from it
   VLTR_HIP(launch_copy_word(p.status + kReportPairs, p.pair_cnt + p.n, s));
-  VLTR_HIP(launch_copy_word(p.status + kReportPairsBeyond, scan_beyond_word(ctx->scan_tmp.as<uint32_t>(), p.n), s));
+  VLTR_HIP(launch_copy_word(p.status + kReportPairsBeyond, scan_beyond_word(b.scan_tmp, p.n), s));
   return VLTR_OK;
 }
 
@@ -327,7 +349,7 @@ int checked_pairs(const uint32_t* host_words, int64_t* P) {
 // Back half: pairs, clip, cells, compaction of contacts and cells; the run's
 // report block (status, slow-path counts, pair, contact and cell totals) goes
 // to host_words in one transfer.
-int enqueue_back(vltr_context* ctx, Pipeline& p, const RunExtras* extras, bool batched, bool events,
+int enqueue_back(vltr_context* ctx, Pipeline& p, const RunBufs& b, const RunExtras* extras, bool batched, bool events,
                  uint32_t* host_words, hipStream_t s) {
   const int64_t n = p.n;
   VLTR_HIP(launch_pairs(p, s));
@@ -352,18 +374,18 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunExtras* extras, bool b
   }
   if (p.mode == 0) VLTR_HIP(launch_cells(p, s));
   if (p.mode == 0 && extras && extras->post_cells) VLTR_HIP(extras->post_cells(p, s));
-  uint32_t* flags = ctx->flags.as<uint32_t>();
+  uint32_t* flags = b.flags;
   VLTR_HIP(launch_flag_contacts(p, flags, s));
-  VLTR_HIP(launch_scan_u32(flags, p.n_pairs, ctx->scan_tmp.as<uint32_t>(), s));
-  VLTR_HIP(launch_emit_contacts(p, flags, ctx->out_contacts.as<vltr_contact>(), s));
+  VLTR_HIP(launch_scan_u32(flags, p.n_pairs, b.scan_tmp, s));
+  VLTR_HIP(launch_emit_contacts(p, flags, b.contacts, s));
   VLTR_HIP(launch_copy_word(p.status + kReportContacts, flags + p.n_pairs, s));
-  uint32_t* seg_out = ctx->seg_out.as<uint32_t>();
+  uint32_t* seg_out = b.seg_out;
   if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 0, seg_out, s));
   if (p.net != nullptr) VLTR_HIP(launch_emit_net(p, flags, extras->net_out->as<ContactNet>(), s));
   if (p.mode == 0) {
     VLTR_HIP(launch_flag_cells(p, flags, s));
-    VLTR_HIP(launch_scan_u32(flags, n, ctx->scan_tmp.as<uint32_t>(), s));
-    VLTR_HIP(launch_emit_cells(p, flags, ctx->out_cells.as<vltr_cell>(), s));
+    VLTR_HIP(launch_scan_u32(flags, n, b.scan_tmp, s));
+    VLTR_HIP(launch_emit_cells(p, flags, b.cells, s));
     VLTR_HIP(launch_copy_word(p.status + kReportCells, flags + n, s));
     if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 1, seg_out + p.n_segs + 1, s));
   }
@@ -453,7 +475,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->cell.ensure(n * sizeof(CellRec)));
   VLTR_HIP(ctx->in_cnt.ensure(n * sizeof(uint32_t)));
   VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(std::max<int64_t>(p.n_buckets, 1) + n + 64) * sizeof(uint32_t) * 2));
-  VLTR_HIP(ctx->pinned.ensure(64));
+  VLTR_HIP(ctx->pinned.ensure(kPinnedBytes));
   uint32_t* host_words = static_cast<uint32_t*>(ctx->pinned.ptr);
 
   if (capture) {
@@ -472,9 +494,9 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     VLTR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     rc = [&]() -> int {
       VLTR_HIP(hipMemcpyAsync(ctx->balls.ptr, gc->src, gc->src_bytes, hipMemcpyDeviceToDevice, s));
-      int r = enqueue_front(ctx, p, false, s);
+      int r = enqueue_front(ctx, p, ctx_bufs(ctx), false, s);
       if (r != VLTR_OK) return r;
-      return enqueue_back(ctx, p, extras, batched, false, host_words, s);
+      return enqueue_back(ctx, p, ctx_bufs(ctx), extras, batched, false, host_words, s);
     }();
     hipGraph_t graph = nullptr;
     const hipError_t e_end = hipStreamEndCapture(s, &graph);
@@ -497,7 +519,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
     VLTR_HIP(ctx->nb.ensure(static_cast<size_t>(n) * p.nb_cap * sizeof(int32_t)));
     VLTR_HIP(ctx->incoming.ensure(static_cast<size_t>(n) * p.nb_cap * sizeof(int32_t)));
     fill_pointers(ctx, p);
-    const int rc = enqueue_front(ctx, p, true, s);
+    const int rc = enqueue_front(ctx, p, ctx_bufs(ctx), true, s);
     if (rc != VLTR_OK) return rc;
     VLTR_HIP(hipMemcpyAsync(host_words, p.status, kReportWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     VLTR_HIP(hipEventRecord(ctx->ev[2], s));
@@ -518,7 +540,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   rc = ensure_back(ctx, p, P, n, extras);
   if (rc != VLTR_OK) return rc;
   p.n_pairs = P;
-  rc = enqueue_back(ctx, p, extras, batched, true, host_words, s);
+  rc = enqueue_back(ctx, p, ctx_bufs(ctx), extras, batched, true, host_words, s);
   if (rc != VLTR_OK) return rc;
   VLTR_HIP(hipStreamSynchronize(s));
   if (host_words[kReportStatus] & kStatNbOverflow) {
@@ -1333,6 +1355,17 @@ int vltr_context_create(int device, vltr_context** out) {
     vltr_context_destroy(ctx);
     return fail(VLTR_ERR_DEVICE, "constant upload failed");
   }
+  // the run's report block on the host, and the allocation-free entry
+  // point's workspace on this device (its address taken once per device)
+  if (ctx->pinned.ensure(kPinnedBytes) != hipSuccess) {
+    vltr_context_destroy(ctx);
+    return fail(VLTR_ERR_OUT_OF_MEMORY, "pinned report block");
+  }
+  const hipError_t es = noalloc_scratch_setup(device);
+  if (es != hipSuccess) {
+    vltr_context_destroy(ctx);
+    return fail(VLTR_ERR_DEVICE, std::string("workspace address: ") + hipGetErrorString(es));
+  }
   *out = ctx;
   return VLTR_OK;
 }
@@ -1409,6 +1442,182 @@ int vltr_compute_tessellation_device(vltr_context* ctx, const vltr_ball* d_balls
   ctx->last_contacts = nc;
   ctx->last_cells = ncell;
   return VLTR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+static_assert(kReportWords * sizeof(uint32_t) <= kPinnedSegOff && kPinnedSegOff + sizeof(GridSeg) <= kPinnedBytes,
+              "the pinned block holds the report words and one segment");
+
+// Regions of the fixed workspace, 256-byte aligned, taken in order.
+struct Arena {
+  unsigned char* base;
+  size_t cap;
+  size_t used = 0;
+  template <class T>
+  T* take(size_t count) {
+    const size_t off = (used + 255) & ~size_t(255);
+    used = off + count * sizeof(T);
+    return used <= cap ? reinterpret_cast<T*>(base + off) : nullptr;
+  }
+  bool fits() const { return used <= cap; }
+};
+
+// Lanes of the one-lane fallback kernel in the workspace (fewer than the
+// context's: the kernel walks its work list with a stride of its lane count).
+constexpr int64_t kNoAllocSlowLanes = 256;
+
+// One calls at a time per device: calls share the device's workspace.
+std::mutex g_noalloc_lock[64];
+
+// vltr_compute_tessellation_device on the fixed workspace: the same kernels
+// and launch order, buffers carved from the workspace, outputs written to the
+// caller's buffers.
+int run_noalloc(vltr_context* ctx, const vltr_ball* d_balls, int64_t n, double probe, const int32_t* d_groups,
+                int64_t n_groups, int mode, vltr_contact* d_contacts, size_t contacts_cap, vltr_cell* d_cells,
+                size_t* n_contacts, size_t* n_cells) {
+  unsigned char* ws = static_cast<unsigned char*>(noalloc_scratch(ctx->device));
+  if (ws == nullptr) return fail(VLTR_ERR_DEVICE, "no workspace address for this device (context not set up)");
+  hipStream_t s = ctx->stream;
+  uint32_t* host_words = static_cast<uint32_t*>(ctx->pinned.ptr);
+  Pipeline p{};
+  p.n = n;
+  p.n_pop = n;
+  p.n_segs = 1;
+  p.mode = mode;
+  p.probe = probe;
+  p.groups = d_groups;
+  p.n_groups = n_groups;
+  p.clip_variant = ctx->clip_variant;
+  p.balls_in = reinterpret_cast<const double4*>(d_balls);
+  p.contacts_cap = static_cast<int64_t>(contacts_cap);
+  GridSeg* seg_host = reinterpret_cast<GridSeg*>(static_cast<unsigned char*>(ctx->pinned.ptr) + kPinnedSegOff);
+  seg_host->ball_off = 0;
+  seg_host->tab_off = 0;
+  seg_host->kbits = kbits_for(n);
+  seg_host->n = static_cast<int32_t>(n);
+  p.n_buckets = int64_t(1) << (3 * seg_host->kbits);
+  int nb_cap = kNbCapDefault;
+  for (;;) {
+    Arena A{ws, kNoAllocScratchBytes};
+    p.nb_cap = nb_cap;
+    unsigned char* small = A.take<unsigned char>(256 + 6 * kBboxStride * sizeof(unsigned long long));
+    GridSeg* segs = A.take<GridSeg>(1);
+    p.sph = A.take<double4>(n);
+    p.seg_of = A.take<int32_t>(n);
+    p.bucket_key = A.take<uint32_t>(n);
+    p.bucket_rank = A.take<uint32_t>(n);
+    p.bucket_cnt = A.take<uint32_t>(p.n_buckets + 1);
+    p.sorted_id = A.take<int32_t>(n);
+    p.sorted_sph = A.take<double4>(n);
+    p.nb = A.take<int32_t>(static_cast<size_t>(n) * nb_cap);
+    p.nb_cnt = A.take<int32_t>(n);
+    p.nb_big = A.take<int32_t>(n);
+    p.excluded = A.take<uint8_t>(n);
+    p.pair_cnt = A.take<uint32_t>(n + 1);
+    p.cell = A.take<CellRec>(n);
+    p.in_cnt = A.take<uint32_t>(n);
+    p.incoming = mode == 0 ? A.take<int32_t>(static_cast<size_t>(n) * nb_cap) : nullptr;
+    uint32_t* scan_front = A.take<uint32_t>(scan_tmp_words(std::max<int64_t>(p.n_buckets, n) + 64) + 64);
+    if (!A.fits())
+      return fail(VLTR_ERR_INVALID_ARGUMENT, "the run needs more than the " + std::to_string(kNoAllocScratchBytes >> 20) +
+                                                 " MiB workspace of the allocation-free entry point (" +
+                                                 std::to_string(n) + " balls): use vltr_compute_tessellation_device");
+    p.segs = segs;
+    p.box_bits = reinterpret_cast<unsigned long long*>(small);
+    p.status = reinterpret_cast<uint32_t*>(small + 8);
+    p.overflow_cnt = p.status + kReportOverflow;
+    p.wide_cnt = p.status + kReportWide;
+    p.nb_big_cnt = p.status + kReportNbBig;
+    p.bbox = reinterpret_cast<unsigned long long*>(small + 256);
+    VLTR_HIP(hipMemcpyAsync(segs, seg_host, sizeof(GridSeg), hipMemcpyHostToDevice, s));
+    const RunBufs front{scan_front, nullptr, nullptr, nullptr, nullptr};
+    int rc = enqueue_front(ctx, p, front, false, s);
+    if (rc != VLTR_OK) return rc;
+    VLTR_HIP(hipMemcpyAsync(host_words, p.status, kReportWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    VLTR_HIP(hipStreamSynchronize(s));
+    if (host_words[kReportStatus] & kStatNbTooLong)
+      return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
+    if (host_words[kReportStatus] & kStatNbOverflow) {
+      if (nb_cap >= kNbMax) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
+      nb_cap *= 2;
+      continue;
+    }
+    int64_t P = 0;
+    rc = checked_pairs(host_words, &P);
+    if (rc != VLTR_OK) return rc;
+    p.pair_cap = std::max<int64_t>(P, 1);
+    p.n_pairs = P;
+    p.pair_a = A.take<int32_t>(p.pair_cap);
+    p.pair_b = A.take<int32_t>(p.pair_cap);
+    p.rec = A.take<PairRec>(p.pair_cap);
+    p.overflow_list = A.take<int32_t>(p.pair_cap);
+    p.wide_list = A.take<int32_t>(p.pair_cap);
+    p.slow_ws_pairs = kNoAllocSlowLanes;
+    const int64_t cc = 6 + nb_cap, cr = 2 * cc;
+    p.slow_ws = A.take<double>(static_cast<size_t>(kNoAllocSlowLanes) * (5 * cc + 6 * cr));
+    uint32_t* flags = A.take<uint32_t>(std::max<int64_t>(P, n) + 2);
+    uint32_t* scan_back = A.take<uint32_t>(scan_tmp_words(std::max<int64_t>(P, n) + 64) + 64);
+    if (!A.fits())
+      return fail(VLTR_ERR_INVALID_ARGUMENT, "the run needs more than the " + std::to_string(kNoAllocScratchBytes >> 20) +
+                                                 " MiB workspace of the allocation-free entry point (" +
+                                                 std::to_string(n) + " balls, " + std::to_string(P) +
+                                                 " candidate pairs): use vltr_compute_tessellation_device");
+    const RunBufs back{scan_back, flags, d_contacts, d_cells, nullptr};
+    rc = enqueue_back(ctx, p, back, nullptr, false, false, host_words, s);
+    if (rc != VLTR_OK) return rc;
+    VLTR_HIP(hipStreamSynchronize(s));
+    if (host_words[kReportStatus] & kStatNbOverflow) {
+      // an incoming-pair list outgrew nb_cap: the whole run with longer lists
+      if (nb_cap >= kNbMax) return fail(VLTR_ERR_INTERNAL, "neighbor list longer than 4096 entries");
+      nb_cap *= 2;
+      continue;
+    }
+    if (host_words[kReportStatus] & kStatPairOverflow)
+      return fail(VLTR_ERR_INTERNAL, "pair lists outgrew the buffers sized from the pair total");
+    *n_contacts = host_words[kReportContacts];
+    *n_cells = mode == 0 ? host_words[kReportCells] : 0;
+    ctx->counts[0] = ctx->counts[1] = static_cast<uint64_t>(n);
+    ctx->counts[2] = static_cast<uint64_t>(P);
+    ctx->counts[3] = *n_contacts;
+    ctx->counts[4] = *n_cells;
+    ctx->counts[5] = static_cast<uint64_t>(host_words[kReportOverflow]) + host_words[kReportWide];
+    ctx->counts[6] = static_cast<uint64_t>(nb_cap);
+    ctx->counts[7] = static_cast<uint64_t>(p.n_buckets);
+    if (*n_contacts > contacts_cap)
+      return fail(VLTR_ERR_INVALID_ARGUMENT, "the run has " + std::to_string(*n_contacts) +
+                                                 " contacts, more than contacts_cap (" + std::to_string(contacts_cap) +
+                                                 "); only the first contacts_cap were written");
+    return VLTR_OK;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vltr_compute_tessellation_device_noalloc(vltr_context* ctx, const vltr_ball* d_balls, size_t n, double probe,
+                                             const int32_t* d_groups, size_t n_groups, int contacts_only,
+                                             vltr_contact* d_contacts, size_t contacts_cap, vltr_cell* d_cells,
+                                             size_t cells_cap, size_t* n_contacts, size_t* n_cells) {
+  if (!ctx || !n_contacts || !n_cells || (n > 0 && !d_balls) || (n > 0 && !d_contacts) ||
+      (n > 0 && !contacts_only && !d_cells))
+    return fail(VLTR_ERR_INVALID_ARGUMENT, "null argument");
+  *n_contacts = *n_cells = 0;
+  if (n == 0) return VLTR_OK;
+  if (contacts_cap == 0) return fail(VLTR_ERR_INVALID_ARGUMENT, "contacts_cap must be positive");
+  if (n > kNoAllocScratchBytes / 32)
+    return fail(VLTR_ERR_INVALID_ARGUMENT, "too many balls for the " + std::to_string(kNoAllocScratchBytes >> 20) +
+                                               " MiB workspace of the allocation-free entry point");
+  if (!contacts_only && cells_cap < n)
+    return fail(VLTR_ERR_INVALID_ARGUMENT, "cells_cap must be at least the ball count (one cell per ball at most)");
+  if (ctx->device < 0 || ctx->device >= 64) return fail(VLTR_ERR_INVALID_ARGUMENT, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_lock[ctx->device]);
+  VLTR_HIP(hipSetDevice(ctx->device));
+  return run_noalloc(ctx, d_balls, static_cast<int64_t>(n), probe, d_groups, d_groups ? static_cast<int64_t>(n_groups) : 0,
+                     contacts_only ? 1 : 0, d_contacts, contacts_cap, d_cells, n_contacts, n_cells);
 }
 
 int vltr_context_device_outputs(const vltr_context* ctx, const vltr_contact** d_contacts, size_t* n_contacts,

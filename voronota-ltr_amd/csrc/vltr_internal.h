@@ -148,6 +148,10 @@ struct Pipeline {
   // (tessellation.rs:607-658).
   unsigned long long* dd_keys;
   uint32_t* dd_first;
+
+  // Contacts the output buffer holds (0: one per candidate pair, the
+  // context's own buffer); contacts beyond it are counted, not written.
+  int64_t contacts_cap;
 };
 
 // A graph replay whose pair total outgrew the capacity it was captured with
@@ -188,6 +192,14 @@ const uint32_t* scan_beyond_word(const uint32_t* tmp, int64_t n);
 // *dst = *src on the device (one word, stream-ordered).
 hipError_t launch_copy_word(uint32_t* dst, const uint32_t* src, hipStream_t s);
 int64_t scan_tmp_words(int64_t n);
+
+// vltr_noalloc.hip: the fixed device workspace of the allocation-free entry
+// point, kNoAllocScratchBytes per GPU. noalloc_scratch_setup takes its
+// address on the current device (once per device; the context creation
+// calls it); noalloc_scratch returns that address (null before the setup).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+hipError_t noalloc_scratch_setup(int device);
+void* noalloc_scratch(int device);
 
 // k_contact.hip
 hipError_t launch_clip(const Pipeline& p, hipStream_t s);

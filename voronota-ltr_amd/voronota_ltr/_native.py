@@ -59,6 +59,8 @@ SIGNATURES = {
                                              C.POINTER(_vp)]),
     "vltr_compute_tessellation_device": (C.c_int, [_vp, _vp, _sz, C.c_double, _vp, _sz, C.c_int]),
     "vltr_context_device_outputs": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_sz), C.POINTER(_vp), C.POINTER(_sz)]),
+    "vltr_compute_tessellation_device_noalloc": (C.c_int, [_vp, _vp, _sz, C.c_double, _vp, _sz, C.c_int, _vp, _sz,
+                                                           _vp, _sz, C.POINTER(_sz), C.POINTER(_sz)]),
     "vltr_context_neighbor_lists": (C.c_int64, [_vp, _sz, _i32p, _i32p, _sz]),
     "vltr_result_num_balls": (_sz, [_vp]),
     "vltr_result_num_contacts": (_sz, [_vp]),

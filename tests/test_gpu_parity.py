@@ -183,3 +183,22 @@ def test_empty_and_single(vltr):
     r = vltr.tessellate(np.array([[0.0, 0.0, 0.0, 1.0]]), 0.5)
     assert len(r.contacts) == 0 and len(r.cells) == 1
     assert abs(r.cells["sas_area"][0] - 4 * np.pi * 1.5 ** 2) < 1e-12
+
+
+def test_neighbor_slots_after_a_dense_call(vltr):
+    """A dense call (2zsk at probe 7, up to 1,122 neighbors per ball) grows
+    the context's neighbor slots per ball; a later large call halves them
+    until its neighbor and incoming lists fit the list budget (2 GiB), instead
+    of sizing 200,000 balls' lists for the dense call's 1,536 or 3,072
+    slots.""" 
+    from voronota_ltr import _native, workload
+
+    print("\n[gpu] 2zsk probe 7, then 200k balls", flush=True)
+    vltr.tessellate(load_xyzr("balls_2zsk.xyzr"), 7.0)
+    grown = _native.last_counts()["nb_cap"]
+    assert grown >= 1536
+    b, _ = workload.rsa_sphere(200000, seed=200000)
+    r = vltr.tessellate(b, 1.4)
+    cap = _native.last_counts()["nb_cap"]
+    assert cap < grown and 200000 * cap * 8 <= 2 * 2 ** 30
+    assert len(r.cells) == 200000 and len(r.contacts) > 1400000

@@ -179,17 +179,19 @@ __device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
   return p.dirty[p.periodic ? id % p.n : id] != 0;
 }
 
-template <int kClipGroup, int kClipWaves, int kWavesPerEu, bool kKeepNormals>
-__global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu))) void k_clip(Pipeline p) {
+// One clip task of a wavefront: ball q (in cell order), share wsub of its
+// qualifying pairs when kWavesPerBall waves split the ball (pair jj of the
+// ball goes to wave jj % kWavesPerBall; each of them builds the plane table,
+// classifies and clips only its own pairs). Updateable steps (p.dirty set)
+// run whole balls in share 0.
+template <int kClipGroup, bool kKeepNormals, int kWavesPerBall>
+__device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup, kKeepNormals>& w, const int lane,
+                                          const int64_t q, const int wsub) {
   using ClipGroup = WaveGroupT<kClipGroup>;
   using ClipGroupLds = GroupLdsT<2 * kClipGroup>;
   using ClipLds = ClipLdsT<kClipGroup, kKeepNormals>;
-  __shared__ ClipLds lds[kClipWaves];
-  const int wid = threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  ClipLds& w = lds[wid];
-  const int64_t q = xcd_remap(blockIdx.x, gridDim.x) * kClipWaves + wid;
-  if (q >= p.n || pairs_overflowed(p)) return;
+  const int wpb = p.dirty != nullptr ? 1 : kWavesPerBall;
+  if (wsub >= wpb) return;
   const int64_t a = p.periodic ? q : static_cast<int64_t>(p.sorted_id[q]);
   const int64_t base = p.pair_cnt[a];
   const int npairs = static_cast<int>(static_cast<int64_t>(p.pair_cnt[a + 1]) - base);
@@ -212,6 +214,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     if (lane < on) prev_b = p.prev_pair_b[ob + lane];
   }
   if (cnt > kClipSlots) {
+    if (wsub != 0) return;
     for (int j = lane; j < npairs; j += kWave) {
       const int32_t b = p.pair_b[base + j];
       if (incremental && !is_dirty(p, b))
@@ -271,6 +274,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
     const int32_t b = __builtin_amdgcn_readlane(c0, k);
     if (!pair_qualifies(p, a, b)) continue;
     const int jj = jp++;
+    if (wpb > 1 && jj % wpb != wsub) continue;
     if (incremental && !is_dirty(p, b)) {
       int64_t from;
       if (on <= kWave) {
@@ -374,6 +378,53 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
         } else {
           p.rec[pidx] = rec;
         }
+      }
+    }
+  }
+}
+
+// One wavefront per clip task, tasks dealt by block index (remapped so
+// that each XCD walks a contiguous range of balls).
+template <int kClipGroup, int kClipWaves, int kWavesPerEu, bool kKeepNormals, int kWavesPerBall>
+__global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu))) void k_clip(Pipeline p) {
+  __shared__ ClipLdsT<kClipGroup, kKeepNormals> lds[kClipWaves];
+  const int wid = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t t = xcd_remap(blockIdx.x, gridDim.x) * kClipWaves + wid;
+  if (t >= p.n * kWavesPerBall || pairs_overflowed(p)) return;
+  clip_ball<kClipGroup, kKeepNormals, kWavesPerBall>(p, lds[wid], lane, t / kWavesPerBall,
+                                                     static_cast<int>(t % kWavesPerBall));
+}
+
+// Persistent form: as many single-wavefront blocks as the GPU holds at once,
+// each pulling clip tasks from a device work queue until it is empty. The
+// tasks (all balls of all structures of the call, times kWavesPerBall) are
+// split into kClipQueues contiguous ranges, one per XCD, each with its own
+// counter on its own cache line (block b runs on XCD b % 8): a wavefront
+// takes kQueueChunk tasks at a time from its XCD's range, then from the
+// other ranges in turn. Every wavefront leaves once all ranges are taken.
+constexpr int kQueueChunk = 2;
+template <int kClipGroup, int kWavesPerEu, bool kKeepNormals, int kWavesPerBall>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu))) void k_clip_queue(Pipeline p) {
+  __shared__ ClipLdsT<kClipGroup, kKeepNormals> lds;
+  const int lane = threadIdx.x;
+  if (pairs_overflowed(p)) return;
+  const int64_t tasks = p.n * kWavesPerBall;
+  const int x0 = static_cast<int>(blockIdx.x % kClipQueues);
+  for (int v = 0; v < kClipQueues; ++v) {
+    const int x = (x0 + v) % kClipQueues;
+    const int64_t lo = tasks * x / kClipQueues, hi = tasks * (x + 1) / kClipQueues;
+    uint32_t* ctr = p.clip_queue + x * kClipQueueStride;
+    for (;;) {
+      uint32_t got = 0;
+      if (lane == 0) got = atomicAdd(ctr, static_cast<uint32_t>(kQueueChunk));
+      const int64_t t0 = lo + static_cast<int64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(got)));
+      if (t0 >= hi) break;
+      const int64_t t1 = t0 + kQueueChunk < hi ? t0 + kQueueChunk : hi;
+      for (int64_t t = t0; t < t1; ++t) {
+        clip_ball<kClipGroup, kKeepNormals, kWavesPerBall>(p, lds, lane, t / kWavesPerBall,
+                                                           static_cast<int>(t % kWavesPerBall));
+        wave_sync();  // the table of this task is read before the next one's is written
       }
     }
   }
@@ -562,27 +613,55 @@ hipError_t upload_hexagon_constants(hipStream_t s) {
 namespace {
 struct ClipVariant {
   const char* name;
-  int waves;  // wavefronts per block
+  int waves;   // wavefronts per block
+  bool queue;  // persistent blocks on the device work queue
+  int wpb;     // wavefronts per ball
   void (*kernel)(Pipeline);
 };
-// The first is the default: the fastest in the autotuner's passes on the
-// 100k-ball plain workload and on the mixed batch of 200-20000-ball
-// structures (pipeline 3.37 ms against 3.45 for g8w1o4, 3.66 for g8w2o4 and
-// 3.97 for g8w4o4 at 100k). One wavefront per block lets each ball's
-// wavefront release its shared memory as soon as it ends; keeping the unit
-// normals costs 1.5 KB of it (3.5 wavefronts per SIMD) and saves their
-// recomputation at every applied plane.
+// Launch shapes: lanes per pair in phase 2 (g8/g16), wavefronts per block
+// (w), the occupancy target in wavefronts per SIMD (o), unit normals kept in
+// shared memory (n), wavefronts per ball (b2), and the persistent form on the
+// device work queue (q). The first is the default: the fastest in the
+// autotuner's passes on the 100k-ball plain workload (pipeline 3.18 ms
+// against 3.29 for g8w1o4n, the fastest one-shot launch) and on the mixed
+// batch of 4,096 200-20000-ball structures (478 ms against 501). One
+// wavefront per block lets each ball's wavefront release its shared memory as
+// soon as it ends; keeping the unit normals costs 1.5 KB of it (3.5
+// wavefronts per SIMD) and saves their recomputation at every applied plane;
+// two wavefronts per ball build the plane table twice and leave the clip
+// rounds half full (3.88 ms).
 const ClipVariant kClipVariants[] = {
-    {"g8w1o4n", 1, k_clip<8, 1, 4, true>},
-    {"g8w1o4", 1, k_clip<8, 1, 4, false>},
-    {"g8w2o4", 2, k_clip<8, 2, 4, false>},
-    {"g8w4o4", 4, k_clip<8, 4, 4, false>},
-    {"g8w4o3", 4, k_clip<8, 4, 3, false>},
-    {"g16w2o4", 2, k_clip<16, 2, 4, false>},
-    {"g16w4o4", 4, k_clip<16, 4, 4, false>},
-    {"g8w1o3n", 1, k_clip<8, 1, 3, true>},
+    {"g8w1o4q", 1, true, 1, k_clip_queue<8, 4, false, 1>},
+    {"g8w1o4n", 1, false, 1, k_clip<8, 1, 4, true, 1>},
+    {"g8w1o4", 1, false, 1, k_clip<8, 1, 4, false, 1>},
+    {"g8w2o4", 2, false, 1, k_clip<8, 2, 4, false, 1>},
+    {"g8w4o4", 4, false, 1, k_clip<8, 4, 4, false, 1>},
+    {"g8w4o3", 4, false, 1, k_clip<8, 4, 3, false, 1>},
+    {"g16w2o4", 2, false, 1, k_clip<16, 2, 4, false, 1>},
+    {"g16w4o4", 4, false, 1, k_clip<16, 4, 4, false, 1>},
+    {"g8w1o3n", 1, false, 1, k_clip<8, 1, 3, true, 1>},
+    {"g8w1o4b2", 1, false, 2, k_clip<8, 1, 4, false, 2>},
+    {"g8w1o4nq", 1, true, 1, k_clip_queue<8, 4, true, 1>},
+    {"g8w1o4qb2", 1, true, 2, k_clip_queue<8, 4, false, 2>},
 };
 constexpr int kNumClipVariants = static_cast<int>(sizeof(kClipVariants) / sizeof(kClipVariants[0]));
+
+// Blocks of a persistent variant: what the device holds at once (occupancy
+// of the kernel times the compute units), computed once per variant.
+int64_t resident_blocks(int v) {
+  static int64_t cache[kNumClipVariants] = {};
+  if (cache[v] > 0) return cache[v];
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kClipVariants[v].kernel),
+                                                    kClipVariants[v].waves * kWave, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 16;
+  cache[v] = static_cast<int64_t>(cus) * per_cu;
+  return cache[v];
+}
 }  // namespace
 
 int clip_variants() { return kNumClipVariants; }
@@ -590,8 +669,11 @@ const char* clip_variant_name(int v) { return v >= 0 && v < kNumClipVariants ? k
 
 hipError_t launch_clip(const Pipeline& p, hipStream_t s) {
   if (p.n <= 0 || p.n_pairs <= 0) return hipSuccess;
-  const ClipVariant& v = kClipVariants[p.clip_variant >= 0 && p.clip_variant < kNumClipVariants ? p.clip_variant : 0];
-  const int64_t blocks = (p.n + v.waves - 1) / v.waves;
+  const int vi = p.clip_variant >= 0 && p.clip_variant < kNumClipVariants ? p.clip_variant : 0;
+  const ClipVariant& v = kClipVariants[vi];
+  const int64_t tasks = p.n * v.wpb;
+  int64_t blocks = (tasks + v.waves - 1) / v.waves;
+  if (v.queue) blocks = blocks < resident_blocks(vi) ? blocks : resident_blocks(vi);
   hipLaunchKernelGGL(v.kernel, dim3(blocks), dim3(v.waves * kWave), 0, s, p);
   return hipGetLastError();
 }

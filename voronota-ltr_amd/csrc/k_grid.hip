@@ -103,6 +103,7 @@ __global__ void k_init(Pipeline p) {
       p.bbox[d * kBboxStride] = ~0ull;
       p.bbox[(3 + d) * kBboxStride] = 0ull;
     }
+    for (int x = 0; x < kClipQueues; ++x) p.clip_queue[x * kClipQueueStride] = 0u;
     *p.status = 0u;
     *p.overflow_cnt = 0u;
     *p.wide_cnt = 0u;

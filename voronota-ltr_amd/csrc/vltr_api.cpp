@@ -99,6 +99,9 @@ struct HostPinned {
 constexpr int kPhases = 7;
 // Neighbor slots per ball a run starts with (doubled when a list outgrows them).
 constexpr int kNbCapDefault = 96;
+// Bytes of neighbor and incoming lists up to which a call keeps the slot
+// count an earlier call grew to.
+constexpr double kNbListBudget = 2.0 * (1u << 30);
 // Pinned host block of a context: the run's report words at offset 0, the
 // allocation-free entry point's cell-hash segment staged at kPinnedSegOff.
 constexpr size_t kPinnedBytes = 256;
@@ -202,6 +205,7 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.wide_cnt = p.status + kReportWide;
   p.nb_big_cnt = p.status + kReportNbBig;
   p.bbox = reinterpret_cast<unsigned long long*>(small + 256);  // 6 words, kBboxStride apart
+  p.clip_queue = reinterpret_cast<uint32_t*>(small + kSmallQueueOff);
   p.nb_big = ctx->nb_big.as<int32_t>();
   p.nb = ctx->nb.as<int32_t>();
   p.nb_cnt = ctx->nb_cnt.as<int32_t>();
@@ -419,7 +423,15 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   hipStream_t s = ctx->stream;
   const bool capture = gc != nullptr;
   // a direct run may move or resize the buffers a captured graph uses
-  if (!capture) drop_graph(ctx);
+  if (!capture) {
+    drop_graph(ctx);
+    // The neighbor slots per ball grow when a list overflows and are kept
+    // for later calls, but only while the lists (and the incoming lists of
+    // the same size) of this call stay within kNbListBudget: a large call
+    // after a dense small one starts again from the default.
+    while (ctx->nb_cap > kNbCapDefault && static_cast<double>(n) * ctx->nb_cap * 8.0 > kNbListBudget)
+      ctx->nb_cap /= 2;
+  }
   const bool periodic = pbox != nullptr;
   const int64_t n_pop = periodic ? 27 * n : n;
   Pipeline p{};
@@ -467,7 +479,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   VLTR_HIP(ctx->bcnt.ensure((p.n_buckets + 1) * sizeof(uint32_t)));
   VLTR_HIP(ctx->sorted_id.ensure(n_pop * sizeof(int32_t)));
   VLTR_HIP(ctx->sorted_sph.ensure(n_pop * sizeof(double4)));
-  VLTR_HIP(ctx->small.ensure(256 + 6 * kBboxStride * sizeof(unsigned long long)));
+  VLTR_HIP(ctx->small.ensure(kSmallBytes));
   VLTR_HIP(ctx->nb_cnt.ensure(n * sizeof(int32_t)));
   VLTR_HIP(ctx->nb_big.ensure(n * sizeof(int32_t)));
   VLTR_HIP(ctx->excluded.ensure(n));
@@ -1503,7 +1515,7 @@ int run_noalloc(vltr_context* ctx, const vltr_ball* d_balls, int64_t n, double p
   for (;;) {
     Arena A{ws, kNoAllocScratchBytes};
     p.nb_cap = nb_cap;
-    unsigned char* small = A.take<unsigned char>(256 + 6 * kBboxStride * sizeof(unsigned long long));
+    unsigned char* small = A.take<unsigned char>(kSmallBytes);
     GridSeg* segs = A.take<GridSeg>(1);
     p.sph = A.take<double4>(n);
     p.seg_of = A.take<int32_t>(n);
@@ -1532,6 +1544,7 @@ int run_noalloc(vltr_context* ctx, const vltr_ball* d_balls, int64_t n, double p
     p.wide_cnt = p.status + kReportWide;
     p.nb_big_cnt = p.status + kReportNbBig;
     p.bbox = reinterpret_cast<unsigned long long*>(small + 256);
+    p.clip_queue = reinterpret_cast<uint32_t*>(small + kSmallQueueOff);
     VLTR_HIP(hipMemcpyAsync(segs, seg_host, sizeof(GridSeg), hipMemcpyHostToDevice, s));
     const RunBufs front{scan_front, nullptr, nullptr, nullptr, nullptr};
     int rc = enqueue_front(ctx, p, front, false, s);

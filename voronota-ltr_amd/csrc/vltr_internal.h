@@ -41,6 +41,15 @@ constexpr uint32_t kStatContourOverflow = 4u;
 constexpr uint32_t kStatNbTooLong = 8u;        // a ball has more than kNbMax neighbors
 constexpr int kNbMax = 4096;
 constexpr int kBboxStride = 32;  // 256 bytes
+// Work-queue counters of the persistent clip kernel: one per XCD, each on its
+// own 128-byte line.
+constexpr int kClipQueues = 8;
+constexpr int kClipQueueStride = 32;  // 32 words
+// The run's small device block: the box size word, the report words (from
+// byte 8), the bounding box (from byte 256, kBboxStride words apart) and the
+// clip queue counters (from kSmallQueueOff).
+constexpr size_t kSmallQueueOff = 256 + 6 * kBboxStride * sizeof(unsigned long long);
+constexpr size_t kSmallBytes = kSmallQueueOff + kClipQueues * kClipQueueStride * sizeof(uint32_t);
 
 // The run's report block: consecutive device words from p.status, copied to
 // the host in one transfer. The totals are copied in by k_copy_word after the
@@ -92,6 +101,7 @@ struct Pipeline {
   unsigned long long* box_bits;  // max over 2R + 0.25 as double bits
   unsigned long long* bbox;      // min x, y, z then max x, y, z of the central centres (order-preserving
                                  // keys), one every kBboxStride words so their atomics hit different lines
+  uint32_t* clip_queue;          // kClipQueues counters, kClipQueueStride words apart (cleared by k_init)
 
   int32_t* nb;           // n * nb_cap neighbor ids (sorted by key, id)
   int32_t* nb_cnt;       // n

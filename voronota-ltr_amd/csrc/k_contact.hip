@@ -175,6 +175,50 @@ __device__ void keep_ball(const Pipeline& p, int64_t base, int npairs, int64_t o
     keep_pair_wave(p, prev_find_wave(p, ob, on, p.pair_b[base + j], lane), base + j, lane);
 }
 
+__device__ __forceinline__ unsigned long long lanes_below(int lane) {
+  return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+}
+
+// The pair list of ball a (tessellation.rs:298-319: its qualifying neighbors
+// in list order, from index base) and, in full mode, each pair (a, c) with a
+// canonical c appended to c's incoming list (tessellation.rs:624-631: an
+// image end receives nothing; the incoming order is free, k_cells puts each
+// list in pair order). The qualifying entries of 64 list slots take
+// consecutive indices by a ballot: one coalesced store and one atomic
+// instruction per 64 slots.
+__device__ __forceinline__ void write_pairs(const Pipeline& p, int64_t a, int64_t base, int cnt, const int32_t* nbl,
+                                            int lane) {
+  int64_t j = 0;
+  const bool in_lists = p.mode == 0;
+  for (int k0 = 0; k0 < cnt; k0 += kWave) {
+    const int k = k0 + lane;
+    int32_t c = 0;
+    bool q = false;
+    if (k < cnt) {
+      c = nbl[k];
+      q = pair_qualifies(p, a, c);
+    }
+    const unsigned long long m = __ballot(q);
+    if (q) {
+      const int64_t idx = base + j + __popcll(m & lanes_below(lane));
+      if (idx < p.pair_cap) {
+        p.pair_a[idx] = static_cast<int32_t>(a);
+        p.pair_b[idx] = c;
+        if (in_lists && c < p.n) {
+          const uint32_t slot = atomicAdd(&p.in_cnt[c], 1u);
+          if (slot < static_cast<uint32_t>(p.nb_cap))
+            p.incoming[static_cast<int64_t>(c) * p.nb_cap + slot] = static_cast<int32_t>(idx);
+          else
+            atomicOr(p.status, kStatNbOverflow);
+        }
+      } else {
+        atomicOr(p.status, kStatPairOverflow);
+      }
+    }
+    j += __popcll(m);
+  }
+}
+
 __device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
   return p.dirty[p.periodic ? id % p.n : id] != 0;
 }
@@ -200,6 +244,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
   const int32_t* nbl = p.nb + a * static_cast<int64_t>(p.nb_cap);
   const double* sph4 = reinterpret_cast<const double*>(p.sph);
   const bool incremental = p.dirty != nullptr;
+  if (p.pairs_in_clip && wsub == 0) write_pairs(p, a, base, cnt, nbl, lane);
   int64_t ob = 0;
   int on = 0;
   int32_t prev_b = -1;

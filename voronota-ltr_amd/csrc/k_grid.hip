@@ -724,6 +724,14 @@ hipError_t launch_neighbors(const Pipeline& p, hipStream_t s) {
 }
 
 hipError_t launch_pairs(const Pipeline& p, hipStream_t s) {
+  if (p.n > 0 && p.pairs_in_clip) {
+    // the clip kernel writes the lists; only the incoming counts start at 0
+    if (p.mode == 0) {
+      const int64_t fb = blocks_for(p.n, 256) < 4096 ? blocks_for(p.n, 256) : 4096;
+      hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, s, p.in_cnt, p.n, 0u);
+    }
+    return hipGetLastError();
+  }
   if (p.n > 0) {
     if (p.mode != 0) {
       hipLaunchKernelGGL(k_pair_fill_wave, dim3(blocks_for(p.n, kFillBlock / kWave)), dim3(kFillBlock), 0, s, p);

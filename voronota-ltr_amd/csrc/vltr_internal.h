@@ -162,6 +162,11 @@ struct Pipeline {
   // Contacts the output buffer holds (0: one per candidate pair, the
   // context's own buffer); contacts beyond it are counted, not written.
   int64_t contacts_cap;
+
+  // 1: the clip kernel writes the pair lists (and incoming lists) itself, as
+  // it walks each ball's neighbors; 0: k_pair_fill writes them before the clip
+  // (updateable steps, whose clip reads the pair lists of the step first).
+  int32_t pairs_in_clip;
 };
 
 // A graph replay whose pair total outgrew the capacity it was captured with

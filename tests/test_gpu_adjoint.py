@@ -9,11 +9,11 @@ with respect to the ball centres, on the GPU.
   large total does not cancel), on the 20,000-ball differentiation workload,
   64 random balls x 3 axes, skipping perturbations that change the contact
   set; a subset also against differences of the CPU path (the oracle) and
-  against the oracle's exact derivative. The forward terms near a ball carry
-  round-off of about 1e-11 in total (sums of solid angles, small arc angles
-  from acos), which a difference over 2h = 2e-6 A turns into up to ~5e-6: a
-  coordinate whose h = 1e-6 difference misses the bar is checked again at
-  h = 1e-4 (truncation error ~1e-8), and at most 5% may need that;
+  against the oracle's exact derivative. 1e-6 relative to the difference;
+  where the difference quotient is too noisy for that (round-off of the
+  terms over 2h, truncation near nearly degenerate vertices), a floor of
+  twice its change between h = 1e-6 and 2e-6, with those coordinates
+  counted (at most 10%) and checked against the oracle's dual number;
 * total-SAS-only and total-volume-only weights;
 * the torch.autograd.Function wrapper.
 """
@@ -160,7 +160,15 @@ def test_sas_and_volume_gradients(alpha, beta):
 
 
 def test_differentiation_workload_fd(oracle):
-    """20,000 balls (the differentiation workload), 64 random balls x 3 axes."""
+    """20,000 balls (the differentiation workload), 64 random balls x 3 axes:
+    |adjoint - fd| <= 1e-6 |fd| + floor, fd the central difference at
+    h = 1e-6 (term by term). The floor is 0 unless the relative bar alone
+    fails; it is then twice the difference between the estimates at h = 1e-6
+    and h = 2e-6, the difference quotient's own noise (round-off of the terms
+    over 2h = 2e-6, and truncation near a nearly degenerate contour vertex),
+    the coordinates that needed it are counted and reported, and each of them
+    is also checked against the oracle's forward-mode dual number (the exact
+    derivative of the reference computation) to 1e-7 relative."""
     from voronota_ltr import _native
 
     balls, types, W = _cluster(20000, 20000)
@@ -168,8 +176,8 @@ def test_differentiation_workload_fd(oracle):
     e, g = _native.energy_gradient(balls, 1.4, types, W)
     rng = np.random.default_rng(64)
     picks = rng.choice(len(balls), 64, replace=False)
-    checked = skipped = rechecked = 0
-    worst = 0.0
+    checked = skipped = floored = 0
+    worst = worst_floor = 0.0
     for k, ball in enumerate(picks):
         for axis in range(3):
             fd = _fd(balls, types, W, int(ball), axis, 1e-6)
@@ -177,20 +185,26 @@ def test_differentiation_workload_fd(oracle):
                 skipped += 1
                 continue
             checked += 1
-            err = abs(g[ball, axis] - fd) / max(abs(fd), 1.0)
-            if err > 1e-6:
-                rechecked += 1
-                fd4 = _fd(balls, types, W, int(ball), axis, 1e-4)
-                assert fd4 is not None and abs(g[ball, axis] - fd4) <= 1e-6 * max(abs(fd4), 1.0), \
-                    (int(ball), axis, g[ball, axis], fd, fd4)
-                err = abs(g[ball, axis] - fd4) / max(abs(fd4), 1.0)
-            worst = max(worst, err)
+            dev = abs(g[ball, axis] - fd)
+            if dev > 1e-6 * abs(fd):
+                fd2 = _fd(balls, types, W, int(ball), axis, 2e-6)
+                assert fd2 is not None, (int(ball), axis)
+                floor = 2.0 * abs(fd - fd2)
+                floored += 1
+                worst_floor = max(worst_floor, floor)
+                assert dev <= 1e-6 * abs(fd) + floor, (int(ball), axis, g[ball, axis], fd, fd2)
+                # where the difference quotient is this noisy, the exact
+                # derivative of the reference computation decides
+                _, de = oracle.energy_dual(balls, 1.4, types, W, int(ball), axis)
+                assert abs(g[ball, axis] - de) <= 1e-7 * abs(de) + 1e-9, (int(ball), axis, g[ball, axis], de)
+            worst = max(worst, dev / max(abs(fd), 1e-300))
         if k % 16 == 15:
-            print(f"[gpu] 20k adjoint: {k + 1} balls, worst {worst:.2e}", flush=True)
-    print(f"[gpu] 20k adjoint: checked {checked}, skipped {skipped}, rechecked at h=1e-4 {rechecked}, "
-          f"worst {worst:.2e}", flush=True)
+            print(f"[gpu] 20k adjoint: {k + 1} balls, worst relative deviation {worst:.2e}", flush=True)
+    print(f"[gpu] 20k adjoint: checked {checked}, skipped (contact set changes within h) {skipped}, within 1e-6 "
+          f"relative {checked - floored}, floor-limited {floored} (largest floor {worst_floor:.2e}), worst relative "
+          f"deviation {worst:.2e}", flush=True)
     assert checked >= 150
-    assert rechecked <= 0.05 * checked
+    assert floored <= 0.1 * checked
     # the oracle's exact derivative (forward-mode dual numbers) at full size
     for ball in picks[:4]:
         _, de = oracle.energy_dual(balls, 1.4, types, W, int(ball), 1)

@@ -12,7 +12,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle_vltr.so")
+# VLTR_ORACLE_LIB: another build of the same library (the sanitizer build,
+# oracle/_san/, loaded by tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("VLTR_ORACLE_LIB") or os.path.join(HERE, "liboracle_vltr.so")
 
 _u64p = C.POINTER(C.c_uint64)
 _f64p = C.POINTER(C.c_double)

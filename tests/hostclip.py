@@ -6,7 +6,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "native", "libclip_host.so")
+# VLTR_CLIP_HOST_LIB: the sanitizer build (tests/native/_san/), loaded by
+# tests/test_sanitizers.py
+LIB = os.environ.get("VLTR_CLIP_HOST_LIB") or os.path.join(HERE, "native", "libclip_host.so")
 _lib = None
 
 

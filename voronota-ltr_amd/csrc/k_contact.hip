@@ -446,32 +446,30 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
 // tasks (all balls of all structures of the call, times kWavesPerBall) are
 // split into kClipQueues contiguous ranges, one per XCD, each with its own
 // counter on its own cache line (block b runs on XCD b % 8): a wavefront
-// takes kQueueChunk tasks at a time from its XCD's range, then from the
-// other ranges in turn. Every wavefront leaves once all ranges are taken.
-constexpr int kQueueChunk = 2;
+// takes one task at a time from its XCD's range, then from the other ranges
+// in turn. Every wavefront leaves once all ranges are taken.
 template <int kClipGroup, int kWavesPerEu, bool kKeepNormals, int kWavesPerBall>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu))) void k_clip_queue(Pipeline p) {
   __shared__ ClipLdsT<kClipGroup, kKeepNormals> lds;
   const int lane = threadIdx.x;
   if (pairs_overflowed(p)) return;
   const int64_t tasks = p.n * kWavesPerBall;
-  const int x0 = static_cast<int>(blockIdx.x % kClipQueues);
-  for (int v = 0; v < kClipQueues; ++v) {
-    const int x = (x0 + v) % kClipQueues;
+  // one task per claim; the only state carried across a task is the queue
+  // being drained (v steps through the XCD's own range, then the others)
+  int v = 0;
+  while (v < kClipQueues) {
+    const int x = (static_cast<int>(blockIdx.x % kClipQueues) + v) % kClipQueues;
     const int64_t lo = tasks * x / kClipQueues, hi = tasks * (x + 1) / kClipQueues;
-    uint32_t* ctr = p.clip_queue + x * kClipQueueStride;
-    for (;;) {
-      uint32_t got = 0;
-      if (lane == 0) got = atomicAdd(ctr, static_cast<uint32_t>(kQueueChunk));
-      const int64_t t0 = lo + static_cast<int64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(got)));
-      if (t0 >= hi) break;
-      const int64_t t1 = t0 + kQueueChunk < hi ? t0 + kQueueChunk : hi;
-      for (int64_t t = t0; t < t1; ++t) {
-        clip_ball<kClipGroup, kKeepNormals, kWavesPerBall>(p, lds, lane, t / kWavesPerBall,
-                                                           static_cast<int>(t % kWavesPerBall));
-        wave_sync();  // the table of this task is read before the next one's is written
-      }
+    uint32_t got = 0;
+    if (lane == 0) got = atomicAdd(p.clip_queue + x * kClipQueueStride, 1u);
+    const int64_t t = lo + static_cast<int64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(got)));
+    if (t >= hi) {
+      ++v;
+      continue;
     }
+    clip_ball<kClipGroup, kKeepNormals, kWavesPerBall>(p, lds, lane, t / kWavesPerBall,
+                                                       static_cast<int>(t % kWavesPerBall));
+    wave_sync();  // the table of this task is read before the next one's is written
   }
 }
 
@@ -688,6 +686,8 @@ const ClipVariant kClipVariants[] = {
     {"g8w1o4b2", 1, false, 2, k_clip<8, 1, 4, false, 2>},
     {"g8w1o4nq", 1, true, 1, k_clip_queue<8, 4, true, 1>},
     {"g8w1o4qb2", 1, true, 2, k_clip_queue<8, 4, false, 2>},
+    {"g8w1o3q", 1, true, 1, k_clip_queue<8, 3, false, 1>},
+    {"g8w1o3nq", 1, true, 1, k_clip_queue<8, 3, true, 1>},
 };
 constexpr int kNumClipVariants = static_cast<int>(sizeof(kClipVariants) / sizeof(kClipVariants[0]));
 

@@ -32,8 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one full-mode k_clip launch moves on the 100,000-ball workload
 # (seed 100000, probe 1.4), per launch shape: FETCH_SIZE x 2 + WRITE_SIZE x 1
 # KiB per dispatch, factors from a 256 MiB device copy read with the same
-# counters (profiles/k_clip_100k_hbm_bytes.txt). Other workloads: not measured.
-K_CLIP_TRAFFIC_BYTES_100K = {"g8w1o4n": 476.4e6, "g8w1o4": 518.9e6}
+# counters (profiles/k_clip_100k_counters.txt). Other shapes and workloads:
+# not measured.
+K_CLIP_TRAFFIC_BYTES_100K = {"g8w1o4q": 1321.9e6, "g8w1o4nq": 1124.7e6, "g8w1o4": 692.9e6, "g8w1o3q": 395.1e6}
 
 
 def work_counts(n, seed, probe):

@@ -51,3 +51,14 @@ def test_1ctf_and_rsa(oracle):
 
     _check(oracle, load_xyzr("1ctf_pdb.xyzr"), 1.4, name="1ctf")
     _check(oracle, workload.rsa_sphere(2500, seed=2500)[0], 1.4, name="rsa 2500")
+
+
+def test_periodic_rsa_box(oracle):
+    """A periodic box of 1,500 balls: the network comes from every contact,
+    images included, before the contact dedup (tessellation.rs:474-478)."""
+    from voronota_ltr import workload
+
+    n = 1500
+    L = (n / 0.056) ** (1.0 / 3.0)
+    b, _ = workload.rsa_box(n, L, seed=1500)
+    _check(oracle, b, 1.4, corners_box((0, 0, 0), (L, L, L)), name="rsa box periodic")

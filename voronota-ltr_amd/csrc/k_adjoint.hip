@@ -49,11 +49,14 @@ __global__ void k_dedup_clear(DedupTable T) {
   }
 }
 
-__global__ void k_dedup_insert(Pipeline p, DedupTable T) {
+// all == 0: contacts through an image only; all != 0: every contact.
+__global__ void k_dedup_insert(Pipeline p, DedupTable T, int all) {
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (t >= p.n_pairs) return;
+  // graph replays: pairs past the device-side total hold no record of this run
+  if (p.n_pairs_dev != nullptr && t >= static_cast<int64_t>(*p.n_pairs_dev)) return;
   const PairRec rec = p.rec[t];
-  if (!(rec.flags & kRecValid) || !(rec.area > 0.0) || p.pair_b[t] < p.n) return;
+  if (!(rec.flags & kRecValid) || !(rec.area > 0.0) || (all == 0 && p.pair_b[t] < p.n)) return;
   const unsigned long long key = pair_key(p, t);
   uint64_t slot = mix64(key) & T.mask;
   for (;;) {
@@ -157,10 +160,10 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(VLTR_
 
 int64_t adjoint_blocks() { return kAdjBlocks; }
 
-hipError_t launch_dedup_table(const Pipeline& p, unsigned long long* keys, uint32_t* first, hipStream_t s) {
+hipError_t launch_dedup_table(const Pipeline& p, unsigned long long* keys, uint32_t* first, int all, hipStream_t s) {
   DedupTable T{keys, first, dedup_slots(p.n_pairs) - 1};
   hipLaunchKernelGGL(k_dedup_clear, dim3(1024), dim3(256), 0, s, T);
-  if (p.n_pairs > 0) hipLaunchKernelGGL(k_dedup_insert, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, T);
+  if (p.n_pairs > 0) hipLaunchKernelGGL(k_dedup_insert, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, T, all);
   return hipGetLastError();
 }
 
@@ -180,7 +183,7 @@ hipError_t launch_adjoint(const Pipeline& p, const int32_t* types, const double*
     hipLaunchKernelGGL(k_adj_list, dim3((p.n_pairs + 255) / 256), dim3(256), 0, s, p, work.flags, work.list);
   }
   if (p.periodic && p.dd_keys == nullptr) {
-    const hipError_t e = launch_dedup_table(p, dedup_keys, dedup_first, s);
+    const hipError_t e = launch_dedup_table(p, dedup_keys, dedup_first, 0, s);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, grad, 3 * p.n);

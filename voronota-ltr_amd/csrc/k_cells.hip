@@ -203,15 +203,23 @@ __global__ void k_dedup_incoming(Pipeline p, uint64_t mask) {
     atomicOr(p.status, kStatNbOverflow);
 }
 
-__global__ void k_flag_contacts(const PairRec* rec, int64_t n, const uint32_t* n_dev, uint32_t* flag) {
+// Records with a contact (area > 0). With a canonical-pair table (T.keys
+// non-null, periodic runs) only the first record in pair order of each
+// canonical pair the table holds is kept: full runs keep every contact
+// between two canonical balls and the first of each pair through an image
+// (tessellation.rs:527-601), contacts-only runs the first of each canonical
+// pair over all contacts (tessellation.rs:172-188).
+__global__ void k_flag_contacts(Pipeline p, DedupTable T, uint32_t* flag) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
+  if (i >= p.n_pairs) return;
   // graph replays: pairs past the device-side total hold no record of this run
-  if (n_dev != nullptr && i >= static_cast<int64_t>(*n_dev)) {
+  if (p.n_pairs_dev != nullptr && i >= static_cast<int64_t>(*p.n_pairs_dev)) {
     flag[i] = 0u;
     return;
   }
-  flag[i] = (rec[i].flags & kRecValid) && rec[i].area > 0.0 ? 1u : 0u;
+  bool keep = (p.rec[i].flags & kRecValid) && p.rec[i].area > 0.0;
+  if (keep && T.keys != nullptr && (p.mode != 0 || p.pair_b[i] >= p.n)) keep = dedup_first(p, T, i);
+  flag[i] = keep ? 1u : 0u;
 }
 
 __global__ void k_emit_contacts(Pipeline p, const uint32_t* pos, vltr_contact* out) {
@@ -224,8 +232,20 @@ __global__ void k_emit_contacts(Pipeline p, const uint32_t* pos, vltr_contact* o
   const int32_t a = p.pair_a[i];
   const int64_t off = p.n_segs > 1 ? p.segs[p.seg_of[a]].ball_off : 0;
   vltr_contact c;
-  c.id_a = static_cast<uint64_t>(a - off);
-  c.id_b = static_cast<uint64_t>(p.pair_b[i] - off);
+  if (p.periodic) {
+    // canonical ids; contacts-only runs order them (tessellation.rs:176-186)
+    int64_t ca = a % p.n, cb = p.pair_b[i] % p.n;
+    if (p.mode != 0 && ca > cb) {
+      const int64_t t = ca;
+      ca = cb;
+      cb = t;
+    }
+    c.id_a = static_cast<uint64_t>(ca);
+    c.id_b = static_cast<uint64_t>(cb);
+  } else {
+    c.id_a = static_cast<uint64_t>(a - off);
+    c.id_b = static_cast<uint64_t>(p.pair_b[i] - off);
+  }
   c.area = r.area;
   c.arc_length = r.arc_length;
   c.central = (r.flags & kRecCentral) ? 1 : 0;
@@ -399,10 +419,11 @@ hipError_t launch_cells(const Pipeline& p, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_flag_contacts(const Pipeline& p, uint32_t* flag, hipStream_t s) {
+hipError_t launch_flag_contacts(const Pipeline& p, unsigned long long* dd_keys, uint32_t* dd_first, uint32_t* flag,
+                                hipStream_t s) {
+  const DedupTable T{dd_keys, dd_first, dd_keys != nullptr ? dedup_slots_mask(p.n_pairs) : 0ull};
   if (p.n_pairs > 0)
-    hipLaunchKernelGGL(k_flag_contacts, dim3(blocks_for(p.n_pairs, 256)), dim3(256), 0, s, p.rec, p.n_pairs,
-                       p.n_pairs_dev, flag);
+    hipLaunchKernelGGL(k_flag_contacts, dim3(blocks_for(p.n_pairs, 256)), dim3(256), 0, s, p, T, flag);
   return hipGetLastError();
 }
 hipError_t launch_emit_contacts(const Pipeline& p, const uint32_t* pos, vltr_contact* out, hipStream_t s) {

@@ -97,6 +97,21 @@ struct HostPinned {
 };
 
 constexpr int kPhases = 7;
+// Regions of one device buffer, 256-byte aligned, taken in order (the
+// allocation-free entry point's workspace, the network build's scratch).
+struct Arena {
+  unsigned char* base;  // null: only the bytes taken are counted
+  size_t cap;
+  size_t used = 0;
+  template <class T>
+  T* take(size_t count) {
+    const size_t off = (used + 255) & ~size_t(255);
+    used = off + count * sizeof(T);
+    return (base != nullptr && used <= cap) ? reinterpret_cast<T*>(base + off) : nullptr;
+  }
+  bool fits() const { return used <= cap; }
+};
+
 // Neighbor slots per ball a run starts with (doubled when a list outgrows them).
 constexpr int kNbCapDefault = 96;
 // Bytes of neighbor and incoming lists up to which a call keeps the slot
@@ -120,7 +135,7 @@ struct vltr_context {
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
       in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad, net, net_out, solv_cnt, solv_comb, solv_parent,
       solv_sih, solv_ioff, solv_cur, solv_img, nb_big,
-      dedup_keys, dedup_first, adj_list;
+      dedup_keys, dedup_first, adj_list, net_eoff, net_voff, net_work;
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
@@ -299,10 +314,18 @@ int ensure_back(vltr_context* ctx, Pipeline& p, int64_t cap, int64_t n, const Ru
     VLTR_HIP(extras->dd_keys->ensure(slots * sizeof(unsigned long long)));
     VLTR_HIP(extras->dd_first->ensure(slots * sizeof(uint32_t)));
   }
-  if (extras && extras->net_buf && extras->net_out) {
+  if (p.periodic) {
+    // the canonical-pair table of the periodic contact output
+    const uint64_t slots = dedup_slots(p.pair_cap);
+    VLTR_HIP(ctx->dedup_keys.ensure(slots * sizeof(unsigned long long)));
+    VLTR_HIP(ctx->dedup_first.ensure(slots * sizeof(uint32_t)));
+  }
+  if (extras && extras->net_buf) {
     VLTR_HIP(extras->net_buf->ensure(p.pair_cap * sizeof(ContactNet)));
-    VLTR_HIP(extras->net_out->ensure(p.pair_cap * sizeof(ContactNet)));
     p.net = extras->net_buf->as<ContactNet>();
+    VLTR_HIP(ctx->net_eoff.ensure((p.pair_cap + 1) * sizeof(uint32_t)));
+    VLTR_HIP(ctx->net_voff.ensure((p.pair_cap + 1) * sizeof(uint32_t)));
+    if (extras->net_out) VLTR_HIP(extras->net_out->ensure(p.pair_cap * sizeof(ContactNet)));
   }
   return VLTR_OK;
 }
@@ -316,10 +339,14 @@ struct RunBufs {
   vltr_contact* contacts;
   vltr_cell* cells;
   uint32_t* seg_out;  // batched runs only
+  // periodic runs: the canonical-pair table of the contact output
+  unsigned long long* dd_keys;
+  uint32_t* dd_first;
 };
 RunBufs ctx_bufs(vltr_context* ctx) {
   return RunBufs{ctx->scan_tmp.as<uint32_t>(), ctx->flags.as<uint32_t>(), ctx->out_contacts.as<vltr_contact>(),
-                 ctx->out_cells.as<vltr_cell>(), ctx->seg_out.as<uint32_t>()};
+                 ctx->out_cells.as<vltr_cell>(), ctx->seg_out.as<uint32_t>(),
+                 ctx->dedup_keys.as<unsigned long long>(), ctx->dedup_first.as<uint32_t>()};
 }
 
 // Front half: cell-list build and neighbor lists with their scanned pair
@@ -374,19 +401,49 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunBufs& b, const RunExtr
   if (p.mode == 0 && p.periodic && extras && extras->dd_keys && extras->dd_first) {
     p.dd_keys = extras->dd_keys->as<unsigned long long>();
     p.dd_first = extras->dd_first->as<uint32_t>();
-    VLTR_HIP(launch_dedup_table(p, p.dd_keys, p.dd_first, s));
+    VLTR_HIP(launch_dedup_table(p, p.dd_keys, p.dd_first, 0, s));
     VLTR_HIP(launch_dedup_incoming(p, s));
   }
   if (p.mode == 0) VLTR_HIP(launch_cells(p, s));
   if (p.mode == 0 && extras && extras->post_cells) VLTR_HIP(extras->post_cells(p, s));
+  if (p.net != nullptr) {
+    // network records per contact, counted over every contact: the
+    // reference assembles the network before the periodic dedup
+    // (tessellation.rs:474-478); the network is built from them afterwards
+    uint32_t* ecnt = ctx->net_eoff.as<uint32_t>();
+    uint32_t* vcnt = ctx->net_voff.as<uint32_t>();
+    VLTR_HIP(launch_net_count(p, ecnt, vcnt, s));
+    VLTR_HIP(launch_scan_u32(ecnt, p.n_pairs, b.scan_tmp, s));
+    VLTR_HIP(launch_copy_word(p.status + kReportEdges, ecnt + p.n_pairs, s));
+    VLTR_HIP(launch_copy_word(p.status + kReportEdgesBeyond, scan_beyond_word(b.scan_tmp, p.n_pairs), s));
+    VLTR_HIP(launch_scan_u32(vcnt, p.n_pairs, b.scan_tmp, s));
+    VLTR_HIP(launch_copy_word(p.status + kReportVerts, vcnt + p.n_pairs, s));
+    VLTR_HIP(launch_copy_word(p.status + kReportVertsBeyond, scan_beyond_word(b.scan_tmp, p.n_pairs), s));
+  }
+  // periodic runs: of each canonical pair only its first record in pair order
+  // (through an image in full runs, any in contacts-only runs) is reported
+  unsigned long long* out_keys = nullptr;
+  uint32_t* out_first = nullptr;
+  if (p.periodic) {
+    if (p.mode == 0 && p.dd_keys != nullptr) {
+      out_keys = p.dd_keys;  // built before the cells, from the same records
+      out_first = p.dd_first;
+    } else {
+      if (b.dd_keys == nullptr || b.dd_first == nullptr) return fail(VLTR_ERR_INTERNAL, "no canonical-pair table");
+      out_keys = b.dd_keys;
+      out_first = b.dd_first;
+      VLTR_HIP(launch_dedup_table(p, out_keys, out_first, p.mode != 0 ? 1 : 0, s));
+    }
+  }
   uint32_t* flags = b.flags;
-  VLTR_HIP(launch_flag_contacts(p, flags, s));
+  VLTR_HIP(launch_flag_contacts(p, out_keys, out_first, flags, s));
   VLTR_HIP(launch_scan_u32(flags, p.n_pairs, b.scan_tmp, s));
   VLTR_HIP(launch_emit_contacts(p, flags, b.contacts, s));
   VLTR_HIP(launch_copy_word(p.status + kReportContacts, flags + p.n_pairs, s));
   uint32_t* seg_out = b.seg_out;
   if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 0, seg_out, s));
-  if (p.net != nullptr) VLTR_HIP(launch_emit_net(p, flags, extras->net_out->as<ContactNet>(), s));
+  if (p.net != nullptr && extras->net_out != nullptr)
+    VLTR_HIP(launch_emit_net(p, flags, extras->net_out->as<ContactNet>(), s));
   if (p.mode == 0) {
     VLTR_HIP(launch_flag_cells(p, flags, s));
     VLTR_HIP(launch_scan_u32(flags, n, b.scan_tmp, s));
@@ -574,117 +631,86 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   return VLTR_OK;
 }
 
-// tessellation.rs:527-601: among summaries touching a periodic image, keep the
-// first one of each unordered canonical pair; ids are reported modulo n.
-void periodic_dedup_full(std::vector<vltr_contact>& v, uint64_t n) {
-  std::unordered_map<uint64_t, size_t> first;
-  std::vector<vltr_contact> out;
-  out.reserve(v.size());
-  for (size_t i = 0; i < v.size(); ++i) {
-    vltr_contact c = v[i];
-    const bool boundary = c.id_a >= n || c.id_b >= n;
-    const uint64_t ca = c.id_a % n, cb = c.id_b % n;
-    if (boundary) {
-      const uint64_t key = std::min(ca, cb) * n + std::max(ca, cb);
-      if (first.emplace(key, i).second == false) continue;
-    }
-    c.id_a = ca;
-    c.id_b = cb;
-    out.push_back(c);
-  }
-  v.swap(out);
-}
-
-// tessellation.rs:172-188: first occurrence per canonical (min, max) pair,
-// ids canonical and ordered.
-void periodic_dedup_contacts_only(std::vector<vltr_contact>& v, uint64_t n) {
-  std::unordered_map<uint64_t, size_t> first;
-  std::vector<vltr_contact> out;
-  out.reserve(v.size());
-  for (const auto& c0 : v) {
-    vltr_contact c = c0;
-    uint64_t ca = c.id_a % n, cb = c.id_b % n;
-    if (ca > cb) std::swap(ca, cb);
-    if (!first.emplace(ca * n + cb, out.size()).second) continue;
-    c.id_a = ca;
-    c.id_b = cb;
-    out.push_back(c);
-  }
-  v.swap(out);
-}
-
-// Tessellation network (contact.rs:120-177, tessellation.rs:334-401) from
-// the restricted contours of the contacts in pair order: per contact point an
-// edge (a, b, right id or none for an arc) and a vertex (a, b, left, right);
-// an uncut contact is one arc edge. Ids become canonical (periodic), sorted
-// within each element; elements are stably sorted by ids and the first of
-// each run is kept (the reference's sort_unstable leaves that choice open).
-int build_network(vltr_result* r, const std::vector<ContactNet>& nets, uint64_t periodic_n) {
-  struct E {
-    uint64_t ids[3];
-    double len;
-  };
-  struct Vx {
-    uint64_t ids[4];
-    double x, y, z;
-  };
-  auto canon = [&](uint64_t id) { return (periodic_n && id != VLTR_NULL_ID) ? id % periodic_n : id; };
-  std::vector<E> edges;
-  std::vector<Vx> verts;
-  for (size_t k = 0; k < nets.size(); ++k) {
-    const ContactNet& nt = nets[k];
-    const vltr_contact& c = r->contacts[k];
-    const uint64_t a = c.id_a, b = c.id_b;
-    if (nt.m == kTopoTooLong) return fail(VLTR_ERR_INTERNAL, "network output: a contour has more than 32 points");
-    if (nt.m == 0) {
-      edges.push_back(E{{a, b, VLTR_NULL_ID}, c.arc_length});
-      continue;
-    }
-    for (int t = 0; t < nt.m; ++t) {
-      const int u = t + 1 < nt.m ? t + 1 : 0;
-      const bool arc = static_cast<uint64_t>(nt.r[t]) == a;
-      double len;
-      if (arc) {
-        len = nt.ang[t] * nt.r_ic;
-      } else {
-        const double dx = nt.x[t] - nt.x[u], dy = nt.y[t] - nt.y[u], dz = nt.z[t] - nt.z[u];
-        len = std::sqrt((dx * dx + dy * dy) + dz * dz);
-      }
-      edges.push_back(E{{a, b, arc ? VLTR_NULL_ID : static_cast<uint64_t>(nt.r[t])}, len});
-      const uint64_t v3 = static_cast<uint64_t>(nt.l[t]) == a ? VLTR_NULL_ID : static_cast<uint64_t>(nt.l[t]);
-      const uint64_t v4 = arc ? VLTR_NULL_ID : static_cast<uint64_t>(nt.r[t]);
-      verts.push_back(Vx{{a, b, v3, v4}, nt.x[t], nt.y[t], nt.z[t]});
-    }
-  }
-  for (auto& e : edges) {
-    for (auto& id : e.ids) id = canon(id);
-    std::sort(e.ids, e.ids + 3);
-  }
-  for (auto& v : verts) {
-    for (auto& id : v.ids) id = canon(id);
-    std::sort(v.ids, v.ids + 4);
-  }
-  std::stable_sort(edges.begin(), edges.end(),
-                   [](const E& x, const E& y) { return std::lexicographical_compare(x.ids, x.ids + 3, y.ids, y.ids + 3); });
-  std::stable_sort(verts.begin(), verts.end(),
-                   [](const Vx& x, const Vx& y) { return std::lexicographical_compare(x.ids, x.ids + 4, y.ids, y.ids + 4); });
+// The tessellation network (contact.rs:120-177, tessellation.rs:334-401) of
+// the run just completed on ctx, built on the device: the contours in
+// ctx->net and the record counts per pair scanned by enqueue_back become
+// edge and vertex records with canonical sorted ids, ordered by their ids
+// (stable), the first of each run kept; then two transfers to the host.
+int device_network(vltr_context* ctx, int64_t n, bool periodic, vltr_result* r) {
+  hipStream_t s = ctx->stream;
+  uint32_t* hw = static_cast<uint32_t*>(ctx->pinned.ptr);
+  if (hw[kReportStatus] & kStatNetTooLong)
+    return fail(VLTR_ERR_INTERNAL, "network output: a contour has more than 32 points");
+  const int64_t ne = checked_pair_total(hw[kReportEdges], hw[kReportEdgesBeyond]);
+  const int64_t nv = checked_pair_total(hw[kReportVerts], hw[kReportVertsBeyond]);
+  if (ne < 0 || nv < 0)
+    return fail(VLTR_ERR_INVALID_ARGUMENT, "more than 2147483647 network edges or vertices in one call: split the input");
   r->has_network = true;
-  for (size_t i = 0; i < edges.size(); ++i)
-    if (i == 0 || !std::equal(edges[i].ids, edges[i].ids + 3, edges[i - 1].ids)) {
-      vltr_cell_edge e;
-      for (int k = 0; k < 3; ++k) e.ball_indices[k] = edges[i].ids[k];
-      e.length = edges[i].len;
-      r->edges.push_back(e);
-    }
-  for (size_t i = 0; i < verts.size(); ++i)
-    if (i == 0 || !std::equal(verts[i].ids, verts[i].ids + 4, verts[i - 1].ids)) {
-      vltr_cell_vertex v;
-      for (int k = 0; k < 4; ++k) v.ball_indices[k] = verts[i].ids[k];
-      v.x = verts[i].x;
-      v.y = verts[i].y;
-      v.z = verts[i].z;
-      r->vertices.push_back(v);
-    }
+  r->edges.clear();
+  r->vertices.clear();
+  if (ne == 0) return VLTR_OK;  // every contact has an edge: no contacts
+  const NetLayout L = net_layout(n, periodic ? 1 : 0);
+  const int64_t m = std::max(ne, nv);
+  const size_t tmp_e = net_sort_tmp_bytes(ne, 3 * L.bits), tmp_v = net_sort_tmp_bytes(nv, 4 * L.bits);
+  if (tmp_e == SIZE_MAX || tmp_v == SIZE_MAX) return fail(VLTR_ERR_DEVICE, "network sort scratch query");
+  NetRecs E{}, V{};
+  NetSortWork W{};
+  uint32_t* flag = nullptr;
+  vltr_cell_edge* out_e = nullptr;
+  vltr_cell_vertex* out_v = nullptr;
+  auto carve = [&](Arena& A) {
+    E = NetRecs{A.take<unsigned long long>(ne), A.take<unsigned long long>(ne), A.take<uint32_t>(ne),
+                A.take<double>(ne)};
+    V = NetRecs{A.take<unsigned long long>(nv), A.take<unsigned long long>(nv), A.take<uint32_t>(nv),
+                A.take<double>(3 * nv)};
+    W.tmp_bytes = std::max(tmp_e, tmp_v);
+    W.tmp = A.take<unsigned char>(W.tmp_bytes);
+    W.keys_out = A.take<unsigned long long>(m);
+    W.keys_mid = A.take<unsigned long long>(m);
+    W.perm1 = A.take<uint32_t>(m);
+    W.perm2 = A.take<uint32_t>(m);
+    flag = A.take<uint32_t>(m + 1);
+    out_e = A.take<vltr_cell_edge>(ne);
+    out_v = A.take<vltr_cell_vertex>(nv);
+  };
+  Arena size{nullptr, SIZE_MAX};
+  carve(size);
+  VLTR_HIP(ctx->net_work.ensure(size.used));
+  Arena A{ctx->net_work.as<unsigned char>(), ctx->net_work.bytes};
+  carve(A);
+  if (!A.fits()) return fail(VLTR_ERR_INTERNAL, "network scratch");
+  VLTR_HIP(ctx->scan_tmp.ensure(scan_tmp_words(m + 64) * sizeof(uint32_t) * 2));
+  uint32_t* scan_tmp = ctx->scan_tmp.as<uint32_t>();
+
+  Pipeline p{};
+  fill_pointers(ctx, p);
+  p.n = n;
+  p.n_pairs = static_cast<int64_t>(ctx->counts[2]);
+  p.periodic = periodic ? 1 : 0;
+  p.net = ctx->net.as<ContactNet>();
+  VLTR_HIP(launch_net_records(p, ctx->net_eoff.as<uint32_t>(), ctx->net_voff.as<uint32_t>(), L, E, V, s));
+  const uint32_t* perm = nullptr;
+  VLTR_HIP(launch_net_sort(E, ne, 3 * L.bits, W, &perm, s));
+  VLTR_HIP(launch_net_runs(E, perm, ne, flag, s));
+  VLTR_HIP(launch_scan_u32(flag, ne, scan_tmp, s));
+  VLTR_HIP(launch_net_emit_edges(E, perm, flag, ne, L, out_e, s));
+  VLTR_HIP(launch_copy_word(p.status + kReportEdges, flag + ne, s));
+  if (nv > 0) {
+    VLTR_HIP(launch_net_sort(V, nv, 4 * L.bits, W, &perm, s));
+    VLTR_HIP(launch_net_runs(V, perm, nv, flag, s));
+    VLTR_HIP(launch_scan_u32(flag, nv, scan_tmp, s));
+    VLTR_HIP(launch_net_emit_vertices(V, perm, flag, nv, L, out_v, s));
+    VLTR_HIP(launch_copy_word(p.status + kReportVerts, flag + nv, s));
+  }
+  VLTR_HIP(hipMemcpyAsync(hw, p.status, kReportWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipStreamSynchronize(s));
+  const size_t ue = hw[kReportEdges], uv = nv > 0 ? hw[kReportVerts] : 0;
+  if (ue > static_cast<size_t>(ne) || uv > static_cast<size_t>(nv)) return fail(VLTR_ERR_INTERNAL, "network totals");
+  r->edges.resize(ue);
+  r->vertices.resize(uv);
+  if (ue) VLTR_HIP(hipMemcpyAsync(r->edges.data(), out_e, ue * sizeof(vltr_cell_edge), hipMemcpyDeviceToHost, s));
+  if (uv) VLTR_HIP(hipMemcpyAsync(r->vertices.data(), out_v, uv * sizeof(vltr_cell_vertex), hipMemcpyDeviceToHost, s));
+  VLTR_HIP(hipStreamSynchronize(s));
   return VLTR_OK;
 }
 
@@ -730,7 +756,8 @@ int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe
     const bool net = with_vertices && mode == 0;
     if (net) {
       ex.net_buf = &ctx->net;
-      ex.net_out = &ctx->net_out;
+      // the contours compacted per contact only for callers that keep them
+      if (keep_nets) ex.net_out = &ctx->net_out;
     }
     rc = run_pipeline(ctx, static_cast<int64_t>(n), probe, pbox, dg, dg ? static_cast<int64_t>(n_groups) : 0, mode, &nc,
                       &ncell, nullptr, 0, net ? &ex : nullptr);
@@ -748,7 +775,7 @@ int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe
       break;
     }
     std::vector<ContactNet> nets;
-    if (net && nc) {
+    if (net && keep_nets && nc) {
       nets.resize(nc);
       if (hipMemcpyAsync(nets.data(), ctx->net_out.ptr, nc * sizeof(ContactNet), hipMemcpyDeviceToHost, s) !=
           hipSuccess) {
@@ -761,15 +788,9 @@ int host_entry(vltr_context* ctx, const vltr_ball* balls, size_t n, double probe
       break;
     }
     if (net) {
-      rc = build_network(r, nets, pbox ? n : 0);
+      rc = device_network(ctx, static_cast<int64_t>(n), pbox != nullptr, r);
       if (rc != VLTR_OK) break;
       if (keep_nets) keep_nets->swap(nets);
-    }
-    if (pbox) {
-      if (mode == 0)
-        periodic_dedup_full(r->contacts, n);
-      else
-        periodic_dedup_contacts_only(r->contacts, n);
     }
   } while (false);
   if (rc != VLTR_OK) {
@@ -1395,7 +1416,8 @@ void vltr_context_destroy(vltr_context* ctx) {
                     &ctx->in_cnt, &ctx->incoming, &ctx->seg_out, &ctx->topo, &ctx->seed, &ctx->energy_part,
                     &ctx->types, &ctx->grad, &ctx->net, &ctx->net_out, &ctx->solv_cnt, &ctx->solv_comb,
                     &ctx->solv_parent, &ctx->solv_sih, &ctx->solv_ioff, &ctx->solv_cur, &ctx->solv_img,
-                    &ctx->nb_big, &ctx->dedup_keys, &ctx->dedup_first, &ctx->adj_list})
+                    &ctx->nb_big, &ctx->dedup_keys, &ctx->dedup_first, &ctx->adj_list, &ctx->net_eoff,
+                    &ctx->net_voff, &ctx->net_work})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)
@@ -1464,19 +1486,6 @@ namespace {
 static_assert(kReportWords * sizeof(uint32_t) <= kPinnedSegOff && kPinnedSegOff + sizeof(GridSeg) <= kPinnedBytes,
               "the pinned block holds the report words and one segment");
 
-// Regions of the fixed workspace, 256-byte aligned, taken in order.
-struct Arena {
-  unsigned char* base;
-  size_t cap;
-  size_t used = 0;
-  template <class T>
-  T* take(size_t count) {
-    const size_t off = (used + 255) & ~size_t(255);
-    used = off + count * sizeof(T);
-    return used <= cap ? reinterpret_cast<T*>(base + off) : nullptr;
-  }
-  bool fits() const { return used <= cap; }
-};
 
 // Lanes of the one-lane fallback kernel in the workspace (fewer than the
 // context's: the kernel walks its work list with a stride of its lane count).
@@ -1547,7 +1556,7 @@ int run_noalloc(vltr_context* ctx, const vltr_ball* d_balls, int64_t n, double p
     p.bbox = reinterpret_cast<unsigned long long*>(small + 256);
     p.clip_queue = reinterpret_cast<uint32_t*>(small + kSmallQueueOff);
     VLTR_HIP(hipMemcpyAsync(segs, seg_host, sizeof(GridSeg), hipMemcpyHostToDevice, s));
-    const RunBufs front{scan_front, nullptr, nullptr, nullptr, nullptr};
+    const RunBufs front{scan_front, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int rc = enqueue_front(ctx, p, front, false, s);
     if (rc != VLTR_OK) return rc;
     VLTR_HIP(hipMemcpyAsync(host_words, p.status, kReportWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1579,7 +1588,7 @@ int run_noalloc(vltr_context* ctx, const vltr_ball* d_balls, int64_t n, double p
                                                  " MiB workspace of the allocation-free entry point (" +
                                                  std::to_string(n) + " balls, " + std::to_string(P) +
                                                  " candidate pairs): use vltr_compute_tessellation_device");
-    const RunBufs back{scan_back, flags, d_contacts, d_cells, nullptr};
+    const RunBufs back{scan_back, flags, d_contacts, d_cells, nullptr, nullptr, nullptr};
     rc = enqueue_back(ctx, p, back, nullptr, false, false, host_words, s);
     if (rc != VLTR_OK) return rc;
     VLTR_HIP(hipStreamSynchronize(s));
@@ -2392,20 +2401,40 @@ int vltr_updateable_summary(const vltr_updateable* u, vltr_result** out) {
     VLTR_HIP(hipSetDevice(ctx->device));
     r->contacts.resize(u->st.n_contacts);
     r->cells.resize(u->st.n_cells);
+    const vltr_contact* src = ctx->out_contacts.as<const vltr_contact>();
+    const int64_t nc = static_cast<int64_t>(u->st.n_contacts);
+    if (u->st.periodic && nc > 0) {
+      // the run reported the first record of each canonical pair with
+      // canonical ids: order each pair's ids and list the contacts by their
+      // first ball, in pair order within a ball
+      const size_t tmp_bytes = contacts_sort_tmp_bytes(nc);
+      if (tmp_bytes == SIZE_MAX) return fail(VLTR_ERR_DEVICE, "contact sort scratch query");
+      ContactSortWork W{};
+      auto carve = [&](Arena& A) {
+        W.tmp_bytes = tmp_bytes;
+        W.tmp = A.take<unsigned char>(tmp_bytes);
+        W.key = A.take<uint32_t>(nc);
+        W.key_out = A.take<uint32_t>(nc);
+        W.idx = A.take<uint32_t>(nc);
+        W.perm = A.take<uint32_t>(nc);
+        W.out = A.take<vltr_contact>(nc);
+      };
+      Arena size{nullptr, SIZE_MAX};
+      carve(size);
+      VLTR_HIP(ctx->net_work.ensure(size.used));
+      Arena A{ctx->net_work.as<unsigned char>(), ctx->net_work.bytes};
+      carve(A);
+      if (!A.fits()) return fail(VLTR_ERR_INTERNAL, "contact sort scratch");
+      VLTR_HIP(launch_contacts_by_first(src, nc, static_cast<int64_t>(n), W, s));
+      src = W.out;
+    }
     if (!r->contacts.empty())
-      VLTR_HIP(hipMemcpyAsync(r->contacts.data(), ctx->out_contacts.ptr, r->contacts.size() * sizeof(vltr_contact),
+      VLTR_HIP(hipMemcpyAsync(r->contacts.data(), src, r->contacts.size() * sizeof(vltr_contact),
                               hipMemcpyDeviceToHost, s));
     if (!r->cells.empty())
       VLTR_HIP(hipMemcpyAsync(r->cells.data(), ctx->out_cells.ptr, r->cells.size() * sizeof(vltr_cell),
                               hipMemcpyDeviceToHost, s));
     VLTR_HIP(hipStreamSynchronize(s));
-    if (u->st.periodic) {
-      periodic_dedup_full(r->contacts, n);
-      for (auto& c : r->contacts)
-        if (c.id_a > c.id_b) std::swap(c.id_a, c.id_b);
-      std::stable_sort(r->contacts.begin(), r->contacts.end(),
-                       [](const vltr_contact& x, const vltr_contact& y) { return x.id_a < y.id_a; });
-    }
   }
   *out = r.release();
   return VLTR_OK;

@@ -39,6 +39,7 @@ constexpr uint32_t kStatNbOverflow = 1u;    // a neighbor list exceeded its capa
 constexpr uint32_t kStatPairOverflow = 2u;  // more pairs than the pair buffers hold
 constexpr uint32_t kStatContourOverflow = 4u;
 constexpr uint32_t kStatNbTooLong = 8u;        // a ball has more than kNbMax neighbors
+constexpr uint32_t kStatNetTooLong = 16u;      // a network contour has more than kTopoCap points
 constexpr int kNbMax = 4096;
 constexpr int kBboxStride = 32;  // 256 bytes
 // Work-queue counters of the persistent clip kernel: one per XCD, each on its
@@ -67,7 +68,11 @@ enum ReportWord : int {
   kReportMarksOld = 8,      // updateable step: balls marked as old neighbors (k_count_marks,
   kReportMarksChanged = 9,  // written before the run) and as changed
   kReportPairsBeyond = 10,  // 1: the exact pair total exceeds INT32_MAX (kReportPairs wrapped)
-  kReportWords = 11
+  kReportEdges = 11,        // network runs: edge records (the edge count scan's total)
+  kReportVerts = 12,        // network runs: vertex records (the vertex count scan's total)
+  kReportEdgesBeyond = 13,  // 1: the exact edge total exceeds INT32_MAX
+  kReportVertsBeyond = 14,  // 1: the exact vertex total exceeds INT32_MAX
+  kReportWords = 15
 };
 
 // Launch parameters of the tessellation pipeline for one call (one or many
@@ -236,9 +241,10 @@ hipError_t upload_hexagon_constants(hipStream_t s);
 // Slots of the canonical-pair table (periodic runs): a power of two >= 2 n_pairs.
 uint64_t dedup_slots(int64_t n_pairs);
 // Fills the canonical-pair table (dedup_slots(p.n_pairs) slots) from this
-// run's records: per canonical pair through an image, the smallest pair index
-// with a contact.
-hipError_t launch_dedup_table(const Pipeline& p, unsigned long long* keys, uint32_t* first, hipStream_t s);
+// run's records: per canonical pair, the smallest pair index with a contact,
+// among the contacts through an image (all == 0) or among all contacts
+// (all != 0, the contacts-only output of a periodic run).
+hipError_t launch_dedup_table(const Pipeline& p, unsigned long long* keys, uint32_t* first, int all, hipStream_t s);
 // Scratch of the adjoint: flags (n_pairs + 1 words) and scan_tmp as for
 // launch_scan_u32 over n_pairs, list (n_pairs entries): the contacts the
 // adjoint visits.
@@ -268,7 +274,13 @@ hipError_t launch_cells(const Pipeline& p, hipStream_t s);
 // Dedup cells: appends every first record through an image to the incoming
 // list of its second end's canonical ball (p.dd_keys set, table filled).
 hipError_t launch_dedup_incoming(const Pipeline& p, hipStream_t s);
-hipError_t launch_flag_contacts(const Pipeline& p, uint32_t* flag, hipStream_t s);
+// flag[i] = 1 for each pair with a contact, n_pairs words; with a
+// canonical-pair table (dd_keys non-null, periodic runs: launch_dedup_table
+// with all = p.mode) only for the first record of each canonical pair it holds.
+hipError_t launch_flag_contacts(const Pipeline& p, unsigned long long* dd_keys, uint32_t* dd_first, uint32_t* flag,
+                                hipStream_t s);
+// out[pos[i]] = the contact of pair i (pos = scanned flags); periodic runs
+// report canonical ids, ordered in contacts-only runs.
 hipError_t launch_emit_contacts(const Pipeline& p, const uint32_t* pos, vltr_contact* out, hipStream_t s);
 hipError_t launch_flag_cells(const Pipeline& p, uint32_t* flag, hipStream_t s);
 hipError_t launch_emit_cells(const Pipeline& p, const uint32_t* pos, vltr_cell* out, hipStream_t s);
@@ -279,6 +291,69 @@ hipError_t launch_seg_offsets(const Pipeline& p, const uint32_t* pos, int what, 
 // out[pos[i]] = p.net[i] for every pair i with a contact (pos = scanned contact
 // flags).
 hipError_t launch_emit_net(const Pipeline& p, const uint32_t* pos, ContactNet* out, hipStream_t s);
+// k_network.hip: the tessellation network (contact.rs:120-177,
+// tessellation.rs:334-401) from the contours in p.net.
+// Id packing of the network records: `bits` bits per id, the code n for none.
+struct NetLayout {
+  int64_t n;
+  int32_t periodic;
+  int32_t bits;
+};
+NetLayout net_layout(int64_t n, int periodic);
+// One kind of record (edges: 3 ids, vertices: 4): packed id keys, the
+// record's own index, values (edge: length; vertex: x, y, z).
+struct NetRecs {
+  unsigned long long* lo;
+  unsigned long long* hi;
+  uint32_t* idx;
+  double* val;
+};
+// Scratch of launch_net_sort for `count` records (net_sort_tmp_bytes bytes
+// of tmp).
+struct NetSortWork {
+  void* tmp;
+  size_t tmp_bytes;
+  unsigned long long* keys_out;  // count
+  unsigned long long* keys_mid;  // count
+  uint32_t* perm1;               // count
+  uint32_t* perm2;               // count
+};
+// ecnt[i], vcnt[i] = edge and vertex records of pair i (0 without a contact),
+// n_pairs words each; a contour past kTopoCap points sets kStatNetTooLong.
+hipError_t launch_net_count(const Pipeline& p, uint32_t* ecnt, uint32_t* vcnt, hipStream_t s);
+// The records of every contact at the scanned offsets eoff, voff.
+hipError_t launch_net_records(const Pipeline& p, const uint32_t* eoff, const uint32_t* voff, const NetLayout& L,
+                              const NetRecs& E, const NetRecs& V, hipStream_t s);
+// Bytes of rocprim scratch for sorting count keys of key_bits bits
+// (SIZE_MAX: the query failed).
+size_t net_sort_tmp_bytes(int64_t count, int key_bits);
+// Stable order of the records by their keys (key_bits = ids x bits):
+// *perm = the record indices in that order (W.perm1 or W.perm2).
+hipError_t launch_net_sort(const NetRecs& R, int64_t count, int key_bits, const NetSortWork& W, const uint32_t** perm,
+                           hipStream_t s);
+// flag[j] = 1 where the j-th record in perm order starts a run of equal ids.
+hipError_t launch_net_runs(const NetRecs& R, const uint32_t* perm, int64_t count, uint32_t* flag, hipStream_t s);
+// out[pos[j]] = record perm[j] for each run start j (pos = scanned flags).
+hipError_t launch_net_emit_edges(const NetRecs& R, const uint32_t* perm, const uint32_t* pos, int64_t count,
+                                 const NetLayout& L, vltr_cell_edge* out, hipStream_t s);
+hipError_t launch_net_emit_vertices(const NetRecs& R, const uint32_t* perm, const uint32_t* pos, int64_t count,
+                                    const NetLayout& L, vltr_cell_vertex* out, hipStream_t s);
+// Scratch of launch_contacts_by_first for `count` contacts.
+struct ContactSortWork {
+  void* tmp;  // contacts_sort_tmp_bytes(count) bytes
+  size_t tmp_bytes;
+  uint32_t* key;      // count
+  uint32_t* key_out;  // count
+  uint32_t* idx;      // count
+  uint32_t* perm;     // count
+  vltr_contact* out;  // count: the result
+};
+size_t contacts_sort_tmp_bytes(int64_t count);
+// Periodic updateable summary (updateable.rs:333-376): W.out = the contacts
+// c (ids < n) with their ids ordered, stably sorted by the smaller id.
+hipError_t launch_contacts_by_first(const vltr_contact* c, int64_t count, int64_t n, const ContactSortWork& W,
+                                    hipStream_t s);
+
 // Cells under the updateable's rules (updateable.rs:633-727), in place in
 // p.cell and copied to state: full = 1 after init or a full reinit, else only
 // balls with affected[i] != 0 take the new record (the rest keep state[i]);

@@ -89,8 +89,11 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   // (as 8-byte words: a PairRec local would live in scratch memory)
   constexpr int kRecWords = static_cast<int>(sizeof(PairRec) / 8);
   static_assert(sizeof(PairRec) % 8 == 0, "records are moved as 8-byte words");
+  // (own records without a contact, as pair_contact marks them, are not
+  // loaded: their window entries get flags 0)
+  const uint8_t* pc = p.pair_contact;
   uint64_t own_first[kRecWords] = {};
-  if (lane < nown) {
+  if (lane < nown && (pc == nullptr || pc[own0 + lane] != 0)) {
 #pragma unroll
     for (int k = 0; k < kRecWords; ++k) own_first[k] = reinterpret_cast<const uint64_t*>(p.rec + own0 + lane)[k];
   }
@@ -123,7 +126,10 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
         for (int k = 0; k < kRecWords; ++k) reinterpret_cast<uint64_t*>(&win[w][lane])[k] = own_first[k];
       }
     } else if (c0 + lane < nown) {
-      win[w][lane] = p.rec[own0 + c0 + lane];
+      if (pc == nullptr || pc[own0 + c0 + lane] != 0)
+        win[w][lane] = p.rec[own0 + c0 + lane];
+      else
+        win[w][lane].flags = 0u;
     }
     wave_sync();
     add_window(min(kWave, nown - c0), true, own0 + c0);

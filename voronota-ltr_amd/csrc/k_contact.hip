@@ -180,16 +180,11 @@ __device__ __forceinline__ unsigned long long lanes_below(int lane) {
 }
 
 // The pair list of ball a (tessellation.rs:298-319: its qualifying neighbors
-// in list order, from index base) and, in full mode, each pair (a, c) with a
-// canonical c appended to c's incoming list (tessellation.rs:624-631: an
-// image end receives nothing; the incoming order is free, k_cells puts each
-// list in pair order). The qualifying entries of 64 list slots take
-// consecutive indices by a ballot: one coalesced store and one atomic
-// instruction per 64 slots.
+// in list order, from index base). The qualifying entries of 64 list slots
+// take consecutive indices by a ballot: one coalesced store per 64 slots.
 __device__ __forceinline__ void write_pairs(const Pipeline& p, int64_t a, int64_t base, int cnt, const int32_t* nbl,
                                             int lane) {
   int64_t j = 0;
-  const bool in_lists = p.mode == 0;
   for (int k0 = 0; k0 < cnt; k0 += kWave) {
     const int k = k0 + lane;
     int32_t c = 0;
@@ -204,19 +199,29 @@ __device__ __forceinline__ void write_pairs(const Pipeline& p, int64_t a, int64_
       if (idx < p.pair_cap) {
         p.pair_a[idx] = static_cast<int32_t>(a);
         p.pair_b[idx] = c;
-        if (in_lists && c < p.n) {
-          const uint32_t slot = atomicAdd(&p.in_cnt[c], 1u);
-          if (slot < static_cast<uint32_t>(p.nb_cap))
-            p.incoming[static_cast<int64_t>(c) * p.nb_cap + slot] = static_cast<int32_t>(idx);
-          else
-            atomicOr(p.status, kStatNbOverflow);
-        }
       } else {
         atomicOr(p.status, kStatPairOverflow);
       }
     }
     j += __popcll(m);
   }
+}
+
+// Full runs whose clip writes the pair lists: a contact of pair pidx = (a, b)
+// joins b's incoming list when b is canonical (tessellation.rs:624-631: an
+// image end receives nothing; the incoming order is free, k_cells puts each
+// list in pair order). Pairs without a contact are not listed: the cells and
+// the solvent kernels read contacts only. (Updateable steps list every pair
+// in k_pair_fill before the clip.)
+__device__ __forceinline__ void push_incoming(const Pipeline& p, int64_t pidx, int32_t b, const PairRec& rec) {
+  if (!p.pairs_in_clip || p.mode != 0 || !(rec.flags & kRecValid) || !(rec.area > 0.0)) return;
+  if (p.pair_contact != nullptr) p.pair_contact[pidx] = 1;
+  if (b >= p.n) return;
+  const uint32_t slot = atomicAdd(&p.in_cnt[b], 1u);
+  if (slot < static_cast<uint32_t>(p.nb_cap))
+    p.incoming[static_cast<int64_t>(b) * p.nb_cap + slot] = static_cast<int32_t>(pidx);
+  else
+    atomicOr(p.status, kStatNbOverflow);
 }
 
 __device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
@@ -361,9 +366,12 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
   // uncut pairs: the full circle, one pair per lane
   if (lane < n_uncut) {
     const int it = w.item[kClipSlots - 1 - lane];
+    const int32_t b = w.cid[it & 0xff];
     PairGeom g;
-    pair_setup(A, load_sph(sph4, w.cid[it & 0xff]), &g);
-    p.rec[base + (it >> 8)] = record_uncut(g, p.mode);
+    pair_setup(A, load_sph(sph4, b), &g);
+    const PairRec rec = record_uncut(g, p.mode);
+    p.rec[base + (it >> 8)] = rec;
+    push_incoming(p, base + (it >> 8), b, rec);
     if (p.topo != nullptr) p.topo[base + (it >> 8)].m = 0;
     if (p.net != nullptr) p.net[base + (it >> 8)].m = 0;
   }
@@ -409,8 +417,9 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     if (idx < n_clip) {
       const int it = w.item[idx];
       const uint64_t M = w.mask[idx];
+      const int32_t b = w.cid[it & 0xff];
       PairGeom g;
-      pair_setup(A, load_sph(sph4, w.cid[it & 0xff]), &g);
+      pair_setup(A, load_sph(sph4, b), &g);
       int st = kNone;
       const int64_t pidx = base + (it >> 8);
       const TopoOut tout{p.topo != nullptr ? p.topo + pidx : nullptr, sph4, p.net != nullptr ? p.net + pidx : nullptr};
@@ -422,6 +431,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
           push_wide(p, pidx);
         } else {
           p.rec[pidx] = rec;
+          push_incoming(p, pidx, b, rec);
         }
       }
     }
@@ -543,7 +553,9 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
     }
     if (!any) {
       if (ln == 0) {
-        p.rec[pidx] = record_uncut(g, p.mode);
+        const PairRec rec = record_uncut(g, p.mode);
+        p.rec[pidx] = rec;
+        push_incoming(p, pidx, b, rec);
         if (p.topo != nullptr) p.topo[pidx].m = 0;
         if (p.net != nullptr) p.net[pidx].m = 0;
       }
@@ -570,6 +582,7 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
         push_slow(p, pidx);
       } else {
         p.rec[pidx] = rec;
+        push_incoming(p, pidx, b, rec);
       }
     }
   }
@@ -597,7 +610,9 @@ __global__ __launch_bounds__(64) void k_cut_slow(Pipeline p) {
     const double* sph4 = reinterpret_cast<const double*>(p.sph);
     ClipOut co;
     const int st = clip(sph4, a, b, nbl, p.nb_cnt[a], c_hex_s, c_hex_c, ct, out, &co);
-    p.rec[pi] = record_of(st, co, load_sph(sph4, a), load_sph(sph4, b), out, p.mode);
+    const PairRec rec = record_of(st, co, load_sph(sph4, a), load_sph(sph4, b), out, p.mode);
+    p.rec[pi] = rec;
+    push_incoming(p, pi, b, rec);
     if (p.topo != nullptr && st == kContact) {
       ContactTopo& tp = p.topo[pi];
       if (co.m > kTopoCap) {

@@ -725,10 +725,17 @@ hipError_t launch_neighbors(const Pipeline& p, hipStream_t s) {
 
 hipError_t launch_pairs(const Pipeline& p, hipStream_t s) {
   if (p.n > 0 && p.pairs_in_clip) {
-    // the clip kernel writes the lists; only the incoming counts start at 0
+    // the clip kernel writes the lists; only the incoming counts and the
+    // contact marks start at 0
     if (p.mode == 0) {
       const int64_t fb = blocks_for(p.n, 256) < 4096 ? blocks_for(p.n, 256) : 4096;
       hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, s, p.in_cnt, p.n, 0u);
+      if (p.pair_contact != nullptr && p.n_pairs > 0) {
+        const int64_t words = (p.n_pairs + 3) / 4;
+        const int64_t fw = blocks_for(words, 256) < 4096 ? blocks_for(words, 256) : 4096;
+        hipLaunchKernelGGL(k_fill_u32, dim3(fw), dim3(256), 0, s, reinterpret_cast<uint32_t*>(p.pair_contact), words,
+                           0u);
+      }
     }
     return hipGetLastError();
   }

@@ -135,7 +135,7 @@ struct vltr_context {
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
       in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad, net, net_out, solv_cnt, solv_comb, solv_parent,
       solv_sih, solv_ioff, solv_cur, solv_img, nb_big,
-      dedup_keys, dedup_first, adj_list, net_eoff, net_voff, net_work;
+      dedup_keys, dedup_first, adj_list, net_eoff, net_voff, net_work, pair_contact;
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
@@ -229,6 +229,7 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.pair_a = ctx->pair_a.as<int32_t>();
   p.pair_b = ctx->pair_b.as<int32_t>();
   p.rec = ctx->rec.as<PairRec>();
+  p.pair_contact = ctx->pair_contact.as<uint8_t>();
   p.cell = ctx->cell.as<CellRec>();
   p.overflow_list = ctx->overflow_list.as<int32_t>();
   p.wide_list = ctx->wide_list.as<int32_t>();
@@ -293,6 +294,7 @@ int ensure_back(vltr_context* ctx, Pipeline& p, int64_t cap, int64_t n, const Ru
   VLTR_HIP(ctx->pair_a.ensure(p.pair_cap * sizeof(int32_t)));
   VLTR_HIP(ctx->pair_b.ensure(p.pair_cap * sizeof(int32_t)));
   VLTR_HIP(ctx->rec.ensure(p.pair_cap * sizeof(PairRec)));
+  VLTR_HIP(ctx->pair_contact.ensure((p.pair_cap + 3) & ~int64_t(3)));
   VLTR_HIP(ctx->overflow_list.ensure(p.pair_cap * sizeof(int32_t)));
   VLTR_HIP(ctx->wide_list.ensure(p.pair_cap * sizeof(int32_t)));
   p.slow_ws_pairs = slow_lanes(p.nb_cap);
@@ -384,6 +386,7 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunBufs& b, const RunExtr
                  uint32_t* host_words, hipStream_t s) {
   const int64_t n = p.n;
   p.pairs_in_clip = (extras == nullptr || extras->dirty == nullptr) ? 1 : 0;
+  if (!p.pairs_in_clip || p.mode != 0) p.pair_contact = nullptr;
   VLTR_HIP(launch_pairs(p, s));
   if (extras && extras->pre_clip) VLTR_HIP(extras->pre_clip(p, s));
   if (extras && extras->dirty) {
@@ -1417,7 +1420,7 @@ void vltr_context_destroy(vltr_context* ctx) {
                     &ctx->types, &ctx->grad, &ctx->net, &ctx->net_out, &ctx->solv_cnt, &ctx->solv_comb,
                     &ctx->solv_parent, &ctx->solv_sih, &ctx->solv_ioff, &ctx->solv_cur, &ctx->solv_img,
                     &ctx->nb_big, &ctx->dedup_keys, &ctx->dedup_first, &ctx->adj_list, &ctx->net_eoff,
-                    &ctx->net_voff, &ctx->net_work})
+                    &ctx->net_voff, &ctx->net_work, &ctx->pair_contact})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)
@@ -1576,6 +1579,7 @@ int run_noalloc(vltr_context* ctx, const vltr_ball* d_balls, int64_t n, double p
     p.pair_a = A.take<int32_t>(p.pair_cap);
     p.pair_b = A.take<int32_t>(p.pair_cap);
     p.rec = A.take<PairRec>(p.pair_cap);
+    p.pair_contact = A.take<uint8_t>((p.pair_cap + 3) & ~int64_t(3));
     p.overflow_list = A.take<int32_t>(p.pair_cap);
     p.wide_list = A.take<int32_t>(p.pair_cap);
     p.slow_ws_pairs = kNoAllocSlowLanes;

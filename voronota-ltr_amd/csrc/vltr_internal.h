@@ -115,6 +115,10 @@ struct Pipeline {
   int32_t* pair_a;       // pair_cap
   int32_t* pair_b;       // pair_cap
   PairRec* rec;          // pair_cap
+  // pair_cap (rounded up to 4) or null: 1 where the pair has a contact,
+  // written by the clip kernels as they write the records (full runs whose
+  // clip writes the pair lists); k_cells then loads only those own records
+  uint8_t* pair_contact;
   CellRec* cell;         // n
   uint32_t* status;      // 1 word; the first of the run's report block (kReport* words)
   int32_t* overflow_list;  // pair_cap (pairs needing the slow kernel)
@@ -130,7 +134,10 @@ struct Pipeline {
   ContactNet* net;         // pair_cap or null: restricted contours (network output)
   double* seed;            // n or null: dE/d(solid angle) per cell (adjoint runs)
   double sas_weight, volume_weight;  // alpha, beta of the adjoint's energy
-  uint32_t* in_cnt;        // n: pairs (a, b), a < b, received by ball b
+  // Incoming lists: the pairs (a, b), a < b, received by ball b as their
+  // second end; only those with a contact when the clip writes the pair lists
+  // (pairs_in_clip), every pair in updateable steps (k_pair_fill).
+  uint32_t* in_cnt;        // n: list lengths
   int32_t* incoming;       // n * nb_cap: their pair indices (any order)
 
   // Updateable steps (updateable.rs:476-512): a pair is clipped only when
@@ -168,9 +175,10 @@ struct Pipeline {
   // context's own buffer); contacts beyond it are counted, not written.
   int64_t contacts_cap;
 
-  // 1: the clip kernel writes the pair lists (and incoming lists) itself, as
-  // it walks each ball's neighbors; 0: k_pair_fill writes them before the clip
-  // (updateable steps, whose clip reads the pair lists of the step first).
+  // 1: the clip kernels write the pair lists themselves, as they walk each
+  // ball's neighbors, and append each contact to its incoming list as they
+  // write its record; 0: k_pair_fill writes both before the clip (updateable
+  // steps, whose clip reads the pair lists of the step first).
   int32_t pairs_in_clip;
 };
 

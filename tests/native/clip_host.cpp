@@ -273,7 +273,8 @@ int64_t adjoint_host(const double* sph4, int64_t n, const int64_t* nb_off, const
     }
     vltr::PairRec rec[FiberGroup::W];
     int st[FiberGroup::W];
-    vltr::TopoOut tout{&topo[t], sph4};
+    uint32_t slot = 0u;  // a store of one record: topo[t]
+    vltr::TopoOut tout{&topo[t], &slot, 0u, sph4, nullptr};
     run_fibers([&](HostLane& lane) {
       struct {
         const double* sph4;
@@ -350,7 +351,7 @@ int64_t adjoint_host(const double* sph4, int64_t n, const int64_t* nb_off, const
     const double w = W16[4 * types[a] + types[b]];
     const double sga = r.solid_a < 0.0 ? -1.0 : 1.0, sgb = r.solid_b < 0.0 ? -1.0 : 1.0;
     run_fibers([&](HostLane& lane) {
-      vltr::group_contact_adjoint(lane, &lds, sph4, a, b, topo[t], w, seed[a], seed[b], sga, sgb, acc);
+      vltr::group_contact_adjoint(lane, &lds, sph4, a, b, topo[t], topo[t].m, w, seed[a], seed[b], sga, sgb, acc);
     });
   }
   return missing;

@@ -197,9 +197,11 @@ VLTR_HD void add_finite(Acc& acc, int32_t id, V3 v) {
   if (std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z)) acc.add(id, v);
 }
 
+// topo: the contact's cut topology with m points (m = 0: the uncut circle,
+// topo not read).
 template <class G, class Acc>
 VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t a_id, int32_t b_id,
-                                   const ContactTopo& topo, double w, double g_a, double g_b, double sgn_a,
+                                   const ContactTopo& topo, int m, double w, double g_a, double g_b, double sgn_a,
                                    double sgn_b, Acc& acc, double beta = 1.0) {
   constexpr int GS = G::kSize;
   static_assert(2 * GS <= kTopoCap, "two contour points per lane");
@@ -226,7 +228,6 @@ VLTR_HD void group_contact_adjoint(G& g, AdjLds* s, const double* sph4, int32_t 
   V3 icbar{0.0, 0.0, 0.0};
   double rbar = 0.0;
   double distbar[2] = {0.0, 0.0};
-  const int m = topo.m;
   double area;
   if (m == 0) {
     // uncut full circle (contact.rs:303-308, 803-805)

@@ -44,8 +44,9 @@ struct GPt {
 };
 
 // Where a clip also records the cut topology of its contour (for the adjoint):
-// the pair's topology record and the spheres (to tell the two candidate circle
-// points apart).
+// the store record reserved for the pair (written when the pair has a
+// contact, its index then put in the pair's slot), and the spheres (to tell
+// the two candidate circle points apart).
 constexpr int kTopoCap = kGroupR;
 // Cut topology of one contact: m points (m = 0: uncut full circle, m < 0: no
 // topology, so no gradient), ids l[k], r[k] of the pieces before and after
@@ -60,6 +61,14 @@ struct ContactTopo {
 constexpr int32_t kTopoNone = -1;
 constexpr int32_t kTopoTooLong = -2;
 
+// Topology slots of the contacts: an index into the run's store, or one of
+// these values, which need no store record. (Pairs without a contact are not
+// read.)
+constexpr uint32_t kTopoSlotUncut = 0xffffffffu;    // m = 0: the full circle
+constexpr uint32_t kTopoSlotTooLong = 0xfffffffeu;  // more than kTopoCap points: no gradient
+constexpr uint32_t kTopoSlotFull = 0xfffffffdu;     // the store region was full (the host reruns)
+VLTR_HD bool topo_slot_stored(uint32_t s) { return s < kTopoSlotFull; }
+
 // Restricted contour of one contact as computed (network output,
 // contact.rs:120-177): m points (0: uncut full circle, kTopoTooLong: more than
 // kTopoCap), their (left, right) ids, coordinates and arc angles.
@@ -72,9 +81,11 @@ struct ContactNet {
 };
 
 struct TopoOut {
-  ContactTopo* topo;  // may be null
+  ContactTopo* topo;  // the store, or null
+  uint32_t* slot;     // the pair's slot
+  uint32_t record;    // the store record reserved for the pair, or kTopoSlotFull
   const double* sph4;
-  ContactNet* net;    // may be null
+  ContactNet* net;    // the pair's contour, or null
 };
 
 // Radical plane of a and a cutting neighbor (plane_of), plus unit(pn) and the
@@ -419,13 +430,19 @@ template <class G, class S>
 VLTR_HD void write_topology(G& g, const S* s, const PairGeom& pg, int32_t a_id, int m, const TopoOut& tout) {
   constexpr int GS = G::kSize;
   const int j = g.lane();
+  const uint32_t slot = tout.record;
+  if (slot == kTopoSlotFull) {
+    if (j == 0) *tout.slot = kTopoSlotFull;
+    return;
+  }
+  ContactTopo* topo = tout.topo + slot;
   bool root[2] = {false, false};
   for (int h = 0; h < 2; ++h) {
     const int t = j + h * GS;
     if (t >= m) continue;
     const int32_t l = s->l[t], r = s->r[t];
-    tout.topo->l[t] = l;
-    tout.topo->r[t] = r;
+    topo->l[t] = l;
+    topo->r[t] = r;
     if (l == a_id || r == a_id) {
       const int32_t c = l == a_id ? r : l;
       root[h] = circle_root(lds_p(s, t), pg.a.c, pg.b.c, load_sph(tout.sph4, c).c);
@@ -433,8 +450,9 @@ VLTR_HD void write_topology(G& g, const S* s, const PairGeom& pg, int32_t a_id, 
   }
   const uint32_t r0 = g.ballot(root[0]), r1 = g.ballot(root[1]);
   if (j == 0) {
-    tout.topo->m = m;
-    tout.topo->roots = r0 | (r1 << GS);
+    topo->m = m;
+    topo->roots = r0 | (r1 << GS);
+    *tout.slot = slot;
   }
 }
 

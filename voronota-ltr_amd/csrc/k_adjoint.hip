@@ -81,13 +81,29 @@ __global__ void k_zero(double* a, int64_t n) {
     a[i] = 0.0;
 }
 
+// The topology of contact t: its store record and point count (m = 0: the
+// uncut circle, no record read; m < 0: none, no gradient).
+__device__ __forceinline__ const ContactTopo* contact_topo(const Pipeline& p, int64_t t, int* m) {
+  const uint32_t s = p.topo_slot[t];
+  const bool stored = topo_slot_stored(s) && static_cast<int64_t>(s) < kTopoRegions * p.topo_region;
+  const ContactTopo* tp = p.topo + (stored ? s : 0u);
+  *m = stored ? tp->m : (s == kTopoSlotUncut ? 0 : kTopoNone);
+  return tp;
+}
+
 // The contacts with a topology (valid record, positive area, m >= 0), listed
 // in pair order so that every group of the adjoint takes a real contact.
 __global__ void k_adj_flag(Pipeline p, uint32_t* flag) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= p.n_pairs) return;
   const PairRec rec = p.rec[i];
-  flag[i] = (rec.flags & kRecValid) && rec.area > 0.0 && p.topo[i].m >= 0 ? 1u : 0u;
+  bool with = (rec.flags & kRecValid) && rec.area > 0.0;
+  if (with) {
+    int m = 0;
+    contact_topo(p, i, &m);
+    with = m >= 0;
+  }
+  flag[i] = with ? 1u : 0u;
 }
 __global__ void k_adj_list(Pipeline p, const uint32_t* pos, int32_t* list) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -122,7 +138,8 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(VLTR_
   for (int64_t k = static_cast<int64_t>(blockIdx.x) * kAdjGroups + gl; k < n_list; k += groups) {
     const int64_t t = list[k];
     const PairRec rec = p.rec[t];
-    const ContactTopo& topo = p.topo[t];
+    int m = 0;
+    const ContactTopo& topo = *contact_topo(p, t, &m);  // listed by k_adj_flag: m >= 0
     const int32_t a = p.pair_a[t], b = p.pair_b[t];
     const int64_t ca = a % p.n, cb = b % p.n;
     const bool image = p.periodic && b >= p.n;
@@ -137,7 +154,7 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(VLTR_
       b_cell = first && cb != ca;
     }
     if (ln == 0) e += w * rec.area;
-    group_contact_adjoint(G, &lds[gl], sph4, a, b, topo, w, a_cell ? p.seed[ca] : 0.0, b_cell ? p.seed[cb] : 0.0,
+    group_contact_adjoint(G, &lds[gl], sph4, a, b, topo, m, w, a_cell ? p.seed[ca] : 0.0, b_cell ? p.seed[cb] : 0.0,
                           a_cell ? (rec.solid_a < 0.0 ? -1.0 : 1.0) : 0.0,
                           b_cell ? (rec.solid_b < 0.0 ? -1.0 : 1.0) : 0.0, acc, p.volume_weight);
   }

@@ -112,7 +112,35 @@ __device__ __forceinline__ void push_wide(const Pipeline& p, int64_t pidx) {
 constexpr int kRecWords = static_cast<int>(sizeof(PairRec) / 8);
 constexpr int kTopoWords = static_cast<int>(sizeof(ContactTopo) / 8);
 static_assert(sizeof(PairRec) % 8 == 0 && sizeof(ContactTopo) % 8 == 0, "records are copied as 8-byte words");
-static_assert(kRecWords + kTopoWords <= kWave, "one pair's words fit one wavefront");
+static_assert(kTopoWords <= kWave, "one topology's words fit one wavefront");
+
+__device__ __forceinline__ unsigned long long lanes_below(int lane) {
+  return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+}
+
+__device__ __forceinline__ bool is_contact(const PairRec& r) { return (r.flags & kRecValid) && r.area > 0.0; }
+
+// One lane copies a stored topology (word by word: a whole-record copy
+// would hold it in registers).
+__device__ __forceinline__ void copy_topo(ContactTopo* dst, const ContactTopo* src) {
+  const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+  uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+#pragma unroll 1
+  for (int k = 0; k < kTopoWords; ++k) d[k] = s[k];
+}
+
+// k topology records of the region of this block's XCD: the first of their
+// indices within the region (counted past the region's end).
+__device__ __forceinline__ uint32_t topo_reserve(const Pipeline& p, uint32_t k) {
+  return atomicAdd(p.topo_counts + (blockIdx.x % kTopoRegions) * kClipQueueStride, k);
+}
+// The store record of index l within this block's region, or kTopoSlotFull
+// past its end (the run is then redone with larger regions).
+__device__ __forceinline__ uint32_t topo_record(const Pipeline& p, uint32_t l) {
+  return static_cast<int64_t>(l) < p.topo_region
+             ? static_cast<uint32_t>((blockIdx.x % kTopoRegions) * p.topo_region + l)
+             : kTopoSlotFull;
+}
 
 // Position of partner b in the previous run's pair list [ob, ob + on) of the
 // same ball, or -1. Every lane of the wavefront takes part (b uniform).
@@ -132,30 +160,51 @@ __device__ int64_t prev_find_lane(const Pipeline& p, int64_t ob, int on, int32_t
 
 // Pair `to` takes the previous run's record (and topology) of pair `from`,
 // or an empty record when from < 0; the wavefront's lanes copy one word each.
+// A kept contact's stored topology is copied to a new record of this run's
+// store.
+template <bool kTopo>
 __device__ void keep_pair_wave(const Pipeline& p, int64_t from, int64_t to, int lane) {
   if (from < 0) {
     if (lane == 0) p.rec[to] = empty_record();
     return;
   }
-  if (lane < kRecWords) {
+  if (lane < kRecWords)
     reinterpret_cast<uint64_t*>(p.rec + to)[lane] = reinterpret_cast<const uint64_t*>(p.prev_rec + from)[lane];
-  } else if (p.topo != nullptr && lane < kRecWords + kTopoWords) {
-    reinterpret_cast<uint64_t*>(p.topo + to)[lane - kRecWords] =
-        reinterpret_cast<const uint64_t*>(p.prev_topo + from)[lane - kRecWords];
+  if (!kTopo || p.topo == nullptr || !is_contact(p.prev_rec[from])) return;
+  const uint32_t ps = p.prev_topo_slot[from];
+  uint32_t slot = ps;
+  if (topo_slot_stored(ps)) {
+    uint32_t got = 0u;
+    if (lane == 0) got = topo_reserve(p, 1u);
+    slot = topo_record(p, static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(got))));
+    if (slot != kTopoSlotFull && lane < kTopoWords)
+      reinterpret_cast<uint64_t*>(p.topo + slot)[lane] = reinterpret_cast<const uint64_t*>(p.prev_topo + ps)[lane];
   }
+  if (lane == 0) p.topo_slot[to] = slot;
 }
+template <bool kTopo>
 __device__ void keep_pair_lane(const Pipeline& p, int64_t from, int64_t to) {
   if (from < 0) {
     p.rec[to] = empty_record();
     return;
   }
-  p.rec[to] = p.prev_rec[from];
-  if (p.topo != nullptr) p.topo[to] = p.prev_topo[from];
+  const PairRec rec = p.prev_rec[from];
+  p.rec[to] = rec;
+  if (!kTopo || p.topo == nullptr || !is_contact(rec)) return;
+  const uint32_t ps = p.prev_topo_slot[from];
+  uint32_t slot = ps;
+  if (topo_slot_stored(ps)) {
+    slot = topo_record(p, topo_reserve(p, 1u));
+    if (slot != kTopoSlotFull) copy_topo(p.topo + slot, p.prev_topo + ps);
+  }
+  p.topo_slot[to] = slot;
 }
 
 // Ball a is not dirty: all its pairs keep their previous records. When its
 // pair list is the previous one (the usual case: no neighbor of a moved),
-// the records are copied as one block.
+// the records are copied as one block, and the stored topologies of 64
+// contacts at a time take consecutive new records (one atomic per 64).
+template <bool kTopo>
 __device__ void keep_ball(const Pipeline& p, int64_t base, int npairs, int64_t ob, int on, int lane) {
   bool diff = on != npairs;
   if (!diff)
@@ -164,19 +213,32 @@ __device__ void keep_ball(const Pipeline& p, int64_t base, int npairs, int64_t o
     const uint64_t* rs = reinterpret_cast<const uint64_t*>(p.prev_rec + ob);
     uint64_t* rd = reinterpret_cast<uint64_t*>(p.rec + base);
     for (int t = lane; t < npairs * kRecWords; t += kWave) rd[t] = rs[t];
-    if (p.topo != nullptr) {
-      const uint64_t* ts = reinterpret_cast<const uint64_t*>(p.prev_topo + ob);
-      uint64_t* td = reinterpret_cast<uint64_t*>(p.topo + base);
-      for (int t = lane; t < npairs * kTopoWords; t += kWave) td[t] = ts[t];
+    if (kTopo && p.topo != nullptr) {
+      for (int j0 = 0; j0 < npairs; j0 += kWave) {
+        const int j = j0 + lane;
+        uint32_t ps = kTopoSlotUncut;
+        bool stored = false;
+        if (j < npairs && is_contact(p.prev_rec[ob + j])) {
+          ps = p.prev_topo_slot[ob + j];
+          stored = topo_slot_stored(ps);
+          if (!stored) p.topo_slot[base + j] = ps;
+        }
+        const unsigned long long m = __ballot(stored);
+        if (m == 0ull) continue;
+        uint32_t got = 0u;
+        if (lane == 0) got = topo_reserve(p, static_cast<uint32_t>(__popcll(m)));
+        got = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(got)));
+        if (stored) {
+          const uint32_t slot = topo_record(p, got + static_cast<uint32_t>(__popcll(m & lanes_below(lane))));
+          if (slot != kTopoSlotFull) copy_topo(p.topo + slot, p.prev_topo + ps);
+          p.topo_slot[base + j] = slot;
+        }
+      }
     }
     return;
   }
   for (int j = 0; j < npairs; ++j)
-    keep_pair_wave(p, prev_find_wave(p, ob, on, p.pair_b[base + j], lane), base + j, lane);
-}
-
-__device__ __forceinline__ unsigned long long lanes_below(int lane) {
-  return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+    keep_pair_wave<kTopo>(p, prev_find_wave(p, ob, on, p.pair_b[base + j], lane), base + j, lane);
 }
 
 // The pair list of ball a (tessellation.rs:298-319: its qualifying neighbors
@@ -233,7 +295,7 @@ __device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
 // ball goes to wave jj % kWavesPerBall; each of them builds the plane table,
 // classifies and clips only its own pairs). Updateable steps (p.dirty set)
 // run whole balls in share 0.
-template <int kClipGroup, bool kKeepNormals, int kWavesPerBall>
+template <int kClipGroup, bool kKeepNormals, int kWavesPerBall, bool kOut>
 __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup, kKeepNormals>& w, const int lane,
                                           const int64_t q, const int wsub) {
   using ClipGroup = WaveGroupT<kClipGroup>;
@@ -257,7 +319,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     ob = p.prev_pair_cnt[a];
     on = static_cast<int>(static_cast<int64_t>(p.prev_pair_cnt[a + 1]) - ob);
     if (!p.dirty[a]) {
-      keep_ball(p, base, npairs, ob, on, lane);
+      keep_ball<kOut>(p, base, npairs, ob, on, lane);
       return;
     }
     // the previous partners, one per lane, for the kept pairs' look-ups
@@ -268,7 +330,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     for (int j = lane; j < npairs; j += kWave) {
       const int32_t b = p.pair_b[base + j];
       if (incremental && !is_dirty(p, b))
-        keep_pair_lane(p, prev_find_lane(p, ob, on, b), base + j);
+        keep_pair_lane<kOut>(p, prev_find_lane(p, ob, on, b), base + j);
       else
         push_wide(p, base + j);
     }
@@ -333,7 +395,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
       } else {
         from = prev_find_wave(p, ob, on, b, lane);
       }
-      keep_pair_wave(p, from, base + jj, lane);
+      keep_pair_wave<kOut>(p, from, base + jj, lane);
       continue;
     }
     if (!((ok_mask >> k) & 1ull)) {
@@ -372,8 +434,8 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     const PairRec rec = record_uncut(g, p.mode);
     p.rec[base + (it >> 8)] = rec;
     push_incoming(p, base + (it >> 8), b, rec);
-    if (p.topo != nullptr) p.topo[base + (it >> 8)].m = 0;
-    if (p.net != nullptr) p.net[base + (it >> 8)].m = 0;
+    if (kOut && p.topo != nullptr) p.topo_slot[base + (it >> 8)] = kTopoSlotUncut;
+    if (kOut && p.net != nullptr) p.net[base + (it >> 8)].m = 0;
   }
 
   // Pairs to clip in decreasing cutting-set size, so that the four pairs
@@ -399,6 +461,15 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     wave_sync();
   }
 
+  // one topology record of this XCD's region per pair to clip, reserved at
+  // once (adjoint runs)
+  uint32_t tbase = 0u;
+  if (kOut && p.topo != nullptr && n_clip > 0) {
+    uint32_t got = 0u;
+    if (lane == 0) got = topo_reserve(p, static_cast<uint32_t>(n_clip));
+    tbase = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(got)));
+  }
+
   // ---- phase 2: kWave / kClipGroup pairs at a time, kClipGroup lanes each
   const int grp = lane / kClipGroup;
   ClipGroup G{grp * kClipGroup, lane & (kClipGroup - 1)};
@@ -422,8 +493,10 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
       pair_setup(A, load_sph(sph4, b), &g);
       int st = kNone;
       const int64_t pidx = base + (it >> 8);
-      const TopoOut tout{p.topo != nullptr ? p.topo + pidx : nullptr, sph4, p.net != nullptr ? p.net + pidx : nullptr};
-      const bool want = p.topo != nullptr || p.net != nullptr;
+      const TopoOut tout{p.topo, p.topo != nullptr ? p.topo_slot + pidx : nullptr,
+                         p.topo != nullptr ? topo_record(p, tbase + static_cast<uint32_t>(idx)) : kTopoSlotFull, sph4,
+                         p.net != nullptr ? p.net + pidx : nullptr};
+      const bool want = kOut && (p.topo != nullptr || p.net != nullptr);
       const PairRec rec = group_clip(G, gs, g, static_cast<int32_t>(a), M, 0ull, fetch, c_hex_s, c_hex_c, p.mode, &st,
                                      want ? &tout : nullptr);
       if (G.lane() == 0) {
@@ -440,14 +513,14 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
 
 // One wavefront per clip task, tasks dealt by block index (remapped so
 // that each XCD walks a contiguous range of balls).
-template <int kClipGroup, int kClipWaves, int kWavesPerEu, bool kKeepNormals, int kWavesPerBall>
+template <int kClipGroup, int kClipWaves, int kWavesPerEu, bool kKeepNormals, int kWavesPerBall, bool kOut>
 __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu))) void k_clip(Pipeline p) {
   __shared__ ClipLdsT<kClipGroup, kKeepNormals> lds[kClipWaves];
   const int wid = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t t = xcd_remap(blockIdx.x, gridDim.x) * kClipWaves + wid;
   if (t >= p.n * kWavesPerBall || pairs_overflowed(p)) return;
-  clip_ball<kClipGroup, kKeepNormals, kWavesPerBall>(p, lds[wid], lane, t / kWavesPerBall,
+  clip_ball<kClipGroup, kKeepNormals, kWavesPerBall, kOut>(p, lds[wid], lane, t / kWavesPerBall,
                                                      static_cast<int>(t % kWavesPerBall));
 }
 
@@ -458,7 +531,7 @@ __global__ __launch_bounds__(kClipWaves* kWave) __attribute__((amdgpu_waves_per_
 // counter on its own cache line (block b runs on XCD b % 8): a wavefront
 // takes one task at a time from its XCD's range, then from the other ranges
 // in turn. Every wavefront leaves once all ranges are taken.
-template <int kClipGroup, int kWavesPerEu, bool kKeepNormals, int kWavesPerBall>
+template <int kClipGroup, int kWavesPerEu, bool kKeepNormals, int kWavesPerBall, bool kOut>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPerEu, kWavesPerEu))) void k_clip_queue(Pipeline p) {
   __shared__ ClipLdsT<kClipGroup, kKeepNormals> lds;
   const int lane = threadIdx.x;
@@ -477,7 +550,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPer
       ++v;
       continue;
     }
-    clip_ball<kClipGroup, kKeepNormals, kWavesPerBall>(p, lds, lane, t / kWavesPerBall,
+    clip_ball<kClipGroup, kKeepNormals, kWavesPerBall, kOut>(p, lds, lane, t / kWavesPerBall,
                                                        static_cast<int>(t % kWavesPerBall));
     wave_sync();  // the table of this task is read before the next one's is written
   }
@@ -556,7 +629,7 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
         const PairRec rec = record_uncut(g, p.mode);
         p.rec[pidx] = rec;
         push_incoming(p, pidx, b, rec);
-        if (p.topo != nullptr) p.topo[pidx].m = 0;
+        if (p.topo != nullptr) p.topo_slot[pidx] = kTopoSlotUncut;
         if (p.net != nullptr) p.net[pidx].m = 0;
       }
       continue;
@@ -572,7 +645,14 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       }
     } fetch{sph4, nbl, A};
     int st = kNone;
-    const TopoOut tout{p.topo != nullptr ? p.topo + pidx : nullptr, sph4, p.net != nullptr ? p.net + pidx : nullptr};
+    uint32_t trec = kTopoSlotFull;
+    if (p.topo != nullptr) {
+      uint32_t got = 0u;
+      if (ln == 0) got = topo_reserve(p, 1u);
+      trec = topo_record(p, static_cast<uint32_t>(G.shfl(static_cast<int32_t>(got), 0)));
+    }
+    const TopoOut tout{p.topo, p.topo != nullptr ? p.topo_slot + pidx : nullptr, trec, sph4,
+                       p.net != nullptr ? p.net + pidx : nullptr};
     const bool want = p.topo != nullptr || p.net != nullptr;
     const PairRec rec = group_clip_words(G, gs, g, a, nw, WideMask{cw}, fetch, c_hex_s, c_hex_c, p.mode, &st,
                                          want ? &tout : nullptr);
@@ -614,10 +694,10 @@ __global__ __launch_bounds__(64) void k_cut_slow(Pipeline p) {
     p.rec[pi] = rec;
     push_incoming(p, pi, b, rec);
     if (p.topo != nullptr && st == kContact) {
-      ContactTopo& tp = p.topo[pi];
-      if (co.m > kTopoCap) {
-        tp.m = kTopoTooLong;
-      } else {
+      const uint32_t slot = co.m > kTopoCap ? kTopoSlotTooLong : topo_record(p, topo_reserve(p, 1u));
+      p.topo_slot[pi] = slot;
+      if (topo_slot_stored(slot)) {
+        ContactTopo& tp = p.topo[slot];
         uint32_t roots = 0u;
         for (int k = 0; k < co.m; ++k) {
           tp.l[k] = out.l[k];
@@ -675,6 +755,9 @@ struct ClipVariant {
   bool queue;  // persistent blocks on the device work queue
   int wpb;     // wavefronts per ball
   void (*kernel)(Pipeline);
+  // the same launch shape writing cut topologies and contours (adjoint and
+  // network runs); the plain kernel carries none of that code
+  void (*kernel_out)(Pipeline);
 };
 // Launch shapes: lanes per pair in phase 2 (g8/g16), wavefronts per block
 // (w), the occupancy target in wavefronts per SIMD (o), unit normals kept in
@@ -689,38 +772,39 @@ struct ClipVariant {
 // two wavefronts per ball build the plane table twice and leave the clip
 // rounds half full (3.88 ms).
 const ClipVariant kClipVariants[] = {
-    {"g8w1o4q", 1, true, 1, k_clip_queue<8, 4, false, 1>},
-    {"g8w1o4n", 1, false, 1, k_clip<8, 1, 4, true, 1>},
-    {"g8w1o4", 1, false, 1, k_clip<8, 1, 4, false, 1>},
-    {"g8w2o4", 2, false, 1, k_clip<8, 2, 4, false, 1>},
-    {"g8w4o4", 4, false, 1, k_clip<8, 4, 4, false, 1>},
-    {"g8w4o3", 4, false, 1, k_clip<8, 4, 3, false, 1>},
-    {"g16w2o4", 2, false, 1, k_clip<16, 2, 4, false, 1>},
-    {"g16w4o4", 4, false, 1, k_clip<16, 4, 4, false, 1>},
-    {"g8w1o3n", 1, false, 1, k_clip<8, 1, 3, true, 1>},
-    {"g8w1o4b2", 1, false, 2, k_clip<8, 1, 4, false, 2>},
-    {"g8w1o4nq", 1, true, 1, k_clip_queue<8, 4, true, 1>},
-    {"g8w1o4qb2", 1, true, 2, k_clip_queue<8, 4, false, 2>},
-    {"g8w1o3q", 1, true, 1, k_clip_queue<8, 3, false, 1>},
-    {"g8w1o3nq", 1, true, 1, k_clip_queue<8, 3, true, 1>},
+    {"g8w1o4q", 1, true, 1, k_clip_queue<8, 4, false, 1, false>, k_clip_queue<8, 4, false, 1, true>},
+    {"g8w1o4n", 1, false, 1, k_clip<8, 1, 4, true, 1, false>, k_clip<8, 1, 4, true, 1, true>},
+    {"g8w1o4", 1, false, 1, k_clip<8, 1, 4, false, 1, false>, k_clip<8, 1, 4, false, 1, true>},
+    {"g8w2o4", 2, false, 1, k_clip<8, 2, 4, false, 1, false>, k_clip<8, 2, 4, false, 1, true>},
+    {"g8w4o4", 4, false, 1, k_clip<8, 4, 4, false, 1, false>, k_clip<8, 4, 4, false, 1, true>},
+    {"g8w4o3", 4, false, 1, k_clip<8, 4, 3, false, 1, false>, k_clip<8, 4, 3, false, 1, true>},
+    {"g16w2o4", 2, false, 1, k_clip<16, 2, 4, false, 1, false>, k_clip<16, 2, 4, false, 1, true>},
+    {"g16w4o4", 4, false, 1, k_clip<16, 4, 4, false, 1, false>, k_clip<16, 4, 4, false, 1, true>},
+    {"g8w1o3n", 1, false, 1, k_clip<8, 1, 3, true, 1, false>, k_clip<8, 1, 3, true, 1, true>},
+    {"g8w1o4b2", 1, false, 2, k_clip<8, 1, 4, false, 2, false>, k_clip<8, 1, 4, false, 2, true>},
+    {"g8w1o4nq", 1, true, 1, k_clip_queue<8, 4, true, 1, false>, k_clip_queue<8, 4, true, 1, true>},
+    {"g8w1o4qb2", 1, true, 2, k_clip_queue<8, 4, false, 2, false>, k_clip_queue<8, 4, false, 2, true>},
+    {"g8w1o3q", 1, true, 1, k_clip_queue<8, 3, false, 1, false>, k_clip_queue<8, 3, false, 1, true>},
+    {"g8w1o3nq", 1, true, 1, k_clip_queue<8, 3, true, 1, false>, k_clip_queue<8, 3, true, 1, true>},
 };
 constexpr int kNumClipVariants = static_cast<int>(sizeof(kClipVariants) / sizeof(kClipVariants[0]));
 
 // Blocks of a persistent variant: what the device holds at once (occupancy
 // of the kernel times the compute units), computed once per variant.
-int64_t resident_blocks(int v) {
-  static int64_t cache[kNumClipVariants] = {};
-  if (cache[v] > 0) return cache[v];
+int64_t resident_blocks(int v, bool out) {
+  static int64_t cache[2][kNumClipVariants] = {};
+  int64_t& c = cache[out ? 1 : 0][v];
+  if (c > 0) return c;
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kClipVariants[v].kernel),
-                                                    kClipVariants[v].waves * kWave, 0) != hipSuccess ||
+  const void* k = reinterpret_cast<const void*>(out ? kClipVariants[v].kernel_out : kClipVariants[v].kernel);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kClipVariants[v].waves * kWave, 0) != hipSuccess ||
       per_cu <= 0)
     per_cu = 16;
-  cache[v] = static_cast<int64_t>(cus) * per_cu;
-  return cache[v];
+  c = static_cast<int64_t>(cus) * per_cu;
+  return c;
 }
 }  // namespace
 
@@ -733,8 +817,9 @@ hipError_t launch_clip(const Pipeline& p, hipStream_t s) {
   const ClipVariant& v = kClipVariants[vi];
   const int64_t tasks = p.n * v.wpb;
   int64_t blocks = (tasks + v.waves - 1) / v.waves;
-  if (v.queue) blocks = blocks < resident_blocks(vi) ? blocks : resident_blocks(vi);
-  hipLaunchKernelGGL(v.kernel, dim3(blocks), dim3(v.waves * kWave), 0, s, p);
+  const bool out = p.topo != nullptr || p.net != nullptr;
+  if (v.queue) blocks = blocks < resident_blocks(vi, out) ? blocks : resident_blocks(vi, out);
+  hipLaunchKernelGGL(out ? v.kernel_out : v.kernel, dim3(blocks), dim3(v.waves * kWave), 0, s, p);
   return hipGetLastError();
 }
 

@@ -104,6 +104,7 @@ __global__ void k_init(Pipeline p) {
       p.bbox[(3 + d) * kBboxStride] = 0ull;
     }
     for (int x = 0; x < kClipQueues; ++x) p.clip_queue[x * kClipQueueStride] = 0u;
+    for (int x = 0; x < kTopoRegions; ++x) p.topo_counts[x * kClipQueueStride] = 0u;
     *p.status = 0u;
     *p.overflow_cnt = 0u;
     *p.wide_cnt = 0u;
@@ -114,6 +115,10 @@ __global__ void k_init(Pipeline p) {
 
 __global__ void k_copy_word(uint32_t* dst, const uint32_t* src) {
   if (threadIdx.x == 0) *dst = *src;
+}
+
+__global__ void k_copy_strided(uint32_t* dst, const uint32_t* src, int count, int stride) {
+  if (static_cast<int>(threadIdx.x) < count) dst[threadIdx.x] = src[threadIdx.x * stride];
 }
 
 __global__ void k_fill_u32(uint32_t* a, int64_t n, uint32_t v) {
@@ -693,6 +698,19 @@ hipError_t launch_gather_rows(const Pipeline& p, const int32_t* ids, int64_t k, 
                               hipStream_t s) {
   if (k > 0)
     hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(k * p.nb_cap, 256)), dim3(256), 0, s, p, ids, k, cnt, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_strided(uint32_t* dst, const uint32_t* src, int count, int stride, hipStream_t s) {
+  if (count > 0) hipLaunchKernelGGL(k_copy_strided, dim3(1), dim3(64), 0, s, dst, src, count, stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_words(uint32_t* a, int64_t n, uint32_t v, hipStream_t s) {
+  if (n > 0) {
+    const int64_t b = blocks_for(n, 256) < 4096 ? blocks_for(n, 256) : 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3(b), dim3(256), 0, s, a, n, v);
+  }
   return hipGetLastError();
 }
 

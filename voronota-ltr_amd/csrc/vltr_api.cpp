@@ -135,7 +135,8 @@ struct vltr_context {
       pair_cnt, pair_a, pair_b, rec, cell, overflow_list, wide_list, flags, scan_tmp, out_contacts, out_cells, slow_ws,
       in_cnt, incoming, seg_out, topo, seed, energy_part, types, grad, net, net_out, solv_cnt, solv_comb, solv_parent,
       solv_sih, solv_ioff, solv_cur, solv_img, nb_big,
-      dedup_keys, dedup_first, adj_list, net_eoff, net_voff, net_work, pair_contact;
+      dedup_keys, dedup_first, adj_list, net_eoff, net_voff, net_work, pair_contact, topo_slot;
+  int64_t topo_cap = 0;  // records of the topology store (adjoint runs)
   HostPinned pinned;
   // last device-resident outputs
   size_t last_contacts = 0, last_cells = 0;
@@ -221,6 +222,7 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
   p.nb_big_cnt = p.status + kReportNbBig;
   p.bbox = reinterpret_cast<unsigned long long*>(small + 256);  // 6 words, kBboxStride apart
   p.clip_queue = reinterpret_cast<uint32_t*>(small + kSmallQueueOff);
+  p.topo_counts = reinterpret_cast<uint32_t*>(small + kSmallTopoOff);
   p.nb_big = ctx->nb_big.as<int32_t>();
   p.nb = ctx->nb.as<int32_t>();
   p.nb_cnt = ctx->nb_cnt.as<int32_t>();
@@ -246,9 +248,13 @@ void fill_pointers(vltr_context* ctx, Pipeline& p) {
 struct RunExtras {
   const uint8_t* user_excl = nullptr;
   std::function<hipError_t(const Pipeline&, hipStream_t)> post_cells;
-  // adjoint runs: cut topologies (sized here once the pair count is known),
-  // cell seeds (n) and the energy weights
+  // adjoint runs: the cut topologies (a store of *topo_cap records, one per
+  // contact with a stored topology, and a slot per pair; sized here once the
+  // pair count is known, the capacity raised and the run redone when the
+  // store is too small), cell seeds (n) and the energy weights
   DevBuf* topo_buf = nullptr;
+  DevBuf* topo_slot_buf = nullptr;
+  int64_t* topo_cap = nullptr;
   // network output: restricted contours per pair, compacted per contact
   DevBuf* net_buf = nullptr;
   DevBuf* net_out = nullptr;
@@ -263,6 +269,7 @@ struct RunExtras {
   const int32_t* prev_pair_b = nullptr;
   const PairRec* prev_rec = nullptr;
   const ContactTopo* prev_topo = nullptr;
+  const uint32_t* prev_topo_slot = nullptr;
   // called with the report block once the neighbor lists are complete; false
   // ends the run there (run_pipeline returns kRunAborted)
   std::function<bool(const uint32_t*)> front_check;
@@ -271,6 +278,9 @@ struct RunExtras {
   DevBuf* dd_keys = nullptr;
   DevBuf* dd_first = nullptr;
 };
+
+// Records per region of a topology store of at least tcap records.
+int64_t topo_region_for(int64_t tcap) { return (tcap + kTopoRegions - 1) / kTopoRegions; }
 
 // run_pipeline's return value when RunExtras::front_check ended the run.
 constexpr int kRunAborted = -1000;
@@ -308,8 +318,15 @@ int ensure_back(vltr_context* ctx, Pipeline& p, int64_t cap, int64_t n, const Ru
   VLTR_HIP(ctx->out_cells.ensure(std::max<int64_t>(n, 1) * sizeof(vltr_cell)));
   fill_pointers(ctx, p);
   if (extras && extras->topo_buf) {
-    VLTR_HIP(extras->topo_buf->ensure(p.pair_cap * sizeof(ContactTopo)));
+    // one record per clipped pair (contacts are about 30% of the candidate
+    // pairs); a region too small makes the run start again (run_pipeline)
+    int64_t& tcap = *extras->topo_cap;
+    if (tcap < p.pair_cap / 2 + 8192) tcap = p.pair_cap / 2 + 8192;
+    p.topo_region = topo_region_for(tcap);
+    VLTR_HIP(extras->topo_buf->ensure(kTopoRegions * p.topo_region * sizeof(ContactTopo)));
+    VLTR_HIP(extras->topo_slot_buf->ensure(p.pair_cap * sizeof(uint32_t)));
     p.topo = extras->topo_buf->as<ContactTopo>();
+    p.topo_slot = extras->topo_slot_buf->as<uint32_t>();
   }
   if (extras && extras->dd_keys && extras->dd_first) {
     const uint64_t slots = dedup_slots(p.pair_cap);
@@ -395,7 +412,10 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunBufs& b, const RunExtr
     p.prev_pair_b = extras->prev_pair_b;
     p.prev_rec = extras->prev_rec;
     p.prev_topo = extras->prev_topo;
+    p.prev_topo_slot = extras->prev_topo_slot;
   }
+  // a contact whose topology no clip path records has none (no gradient)
+  if (p.topo != nullptr) VLTR_HIP(launch_fill_words(p.topo_slot, p.n_pairs, kTopoSlotTooLong, s));
   if (events) VLTR_HIP(hipEventRecord(ctx->ev[3], s));
   VLTR_HIP(launch_clip(p, s));
   if (events) VLTR_HIP(hipEventRecord(ctx->ev[4], s));
@@ -454,6 +474,8 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunBufs& b, const RunExtr
     VLTR_HIP(launch_copy_word(p.status + kReportCells, flags + n, s));
     if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 1, seg_out + p.n_segs + 1, s));
   }
+  if (p.topo != nullptr)
+    VLTR_HIP(launch_copy_strided(p.status + kReportTopos, p.topo_counts, kTopoRegions, kClipQueueStride, s));
   VLTR_HIP(hipMemcpyAsync(host_words, p.status, kReportWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   if (events) VLTR_HIP(hipEventRecord(ctx->ev[6], s));
   return VLTR_OK;
@@ -625,6 +647,15 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   }
   if (host_words[kReportStatus] & kStatPairOverflow)
     return fail(VLTR_ERR_INTERNAL, "pair lists outgrew the buffers sized from the pair total");
+  if (p.topo != nullptr) {
+    int64_t most = 0;
+    for (int x = 0; x < kTopoRegions; ++x) most = std::max<int64_t>(most, host_words[kReportTopos + x]);
+    if (most > p.topo_region) {
+      // a region of the topology store was too small: redo with larger ones
+      *extras->topo_cap = kTopoRegions * (most + most / 8 + 1024);
+      goto restart;
+    }
+  }
   finish_run(ctx, p, P, batched ? static_cast<size_t>(n_structs) : 0, n_contacts, n_cells);
   float ms = 0.0f;
   for (int k = 0; k < kPhases - 1; ++k) {
@@ -1283,6 +1314,8 @@ int vltr_energy_gradient_device(vltr_context* ctx, const vltr_ball* d_balls, siz
   VLTR_HIP(ctx->energy_part.ensure(adjoint_blocks() * sizeof(double)));
   RunExtras ex;
   ex.topo_buf = &ctx->topo;
+  ex.topo_slot_buf = &ctx->topo_slot;
+  ex.topo_cap = &ctx->topo_cap;
   ex.seed = ctx->seed.as<double>();
   ex.sas_weight = sas_weight;
   ex.volume_weight = volume_weight;
@@ -1420,7 +1453,7 @@ void vltr_context_destroy(vltr_context* ctx) {
                     &ctx->types, &ctx->grad, &ctx->net, &ctx->net_out, &ctx->solv_cnt, &ctx->solv_comb,
                     &ctx->solv_parent, &ctx->solv_sih, &ctx->solv_ioff, &ctx->solv_cur, &ctx->solv_img,
                     &ctx->nb_big, &ctx->dedup_keys, &ctx->dedup_first, &ctx->adj_list, &ctx->net_eoff,
-                    &ctx->net_voff, &ctx->net_work, &ctx->pair_contact})
+                    &ctx->net_voff, &ctx->net_work, &ctx->pair_contact, &ctx->topo_slot})
     b->release();
   ctx->pinned.release();
   for (auto& e : ctx->ev)
@@ -1558,6 +1591,7 @@ int run_noalloc(vltr_context* ctx, const vltr_ball* d_balls, int64_t n, double p
     p.nb_big_cnt = p.status + kReportNbBig;
     p.bbox = reinterpret_cast<unsigned long long*>(small + 256);
     p.clip_queue = reinterpret_cast<uint32_t*>(small + kSmallQueueOff);
+    p.topo_counts = reinterpret_cast<uint32_t*>(small + kSmallTopoOff);
     VLTR_HIP(hipMemcpyAsync(segs, seg_host, sizeof(GridSeg), hipMemcpyHostToDevice, s));
     const RunBufs front{scan_front, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int rc = enqueue_front(ctx, p, front, false, s);
@@ -1762,18 +1796,20 @@ struct vltr_updateable {
   double energy = 0.0, sas_weight = 1.0, volume_weight = 1.0;
   std::vector<double> w16;
   size_t n_types = 0;
-  DevBuf types, grad, part, topo, seed, dedup_keys, dedup_first;
-  // pair lists, records and topologies of the last run: the next step's
-  // pairs that are not clipped again (one end not affected) take them
-  DevBuf prev_pair_cnt, prev_pair_b, prev_rec, prev_topo;
+  DevBuf types, grad, part, topo, topo_slot, seed, dedup_keys, dedup_first;
+  int64_t topo_cap = 0;
+  // pair lists, records and topologies (store, slots, records in the store)
+  // of the last run: the next step's pairs that are not clipped again (one
+  // end not affected) take them
+  DevBuf prev_pair_cnt, prev_pair_b, prev_rec, prev_topo, prev_topo_slot;
   bool prev_ok = false, prev_topo_ok = false;
   size_t prev_n = 0;
-  int64_t prev_pairs = 0;
+  int64_t prev_pairs = 0, prev_topos = 0;
   // the same for the backup state: a restore keeps every pair's record
-  DevBuf bk_pair_cnt, bk_pair_b, bk_rec, bk_topo;
+  DevBuf bk_pair_cnt, bk_pair_b, bk_rec, bk_topo, bk_topo_slot;
   bool bk_ok = false, bk_topo_ok = false;
   size_t bk_n = 0;
-  int64_t bk_pairs = 0;
+  int64_t bk_pairs = 0, bk_topos = 0;
 };
 
 namespace {
@@ -1915,6 +1951,8 @@ int upd_run(Upd* u, const RunOpts& o) {
     VLTR_HIP(u->grad.ensure(3 * n * sizeof(double)));
     VLTR_HIP(u->part.ensure(adjoint_blocks() * sizeof(double)));
     ex.topo_buf = &u->topo;
+    ex.topo_slot_buf = &u->topo_slot;
+    ex.topo_cap = &u->topo_cap;
     ex.seed = u->seed.as<double>();
     ex.sas_weight = u->sas_weight;
     ex.volume_weight = u->volume_weight;
@@ -1934,6 +1972,7 @@ int upd_run(Upd* u, const RunOpts& o) {
     ex.prev_pair_b = u->prev_pair_b.as<int32_t>();
     ex.prev_rec = u->prev_rec.as<PairRec>();
     ex.prev_topo = adj ? u->prev_topo.as<ContactTopo>() : nullptr;
+    ex.prev_topo_slot = adj ? u->prev_topo_slot.as<uint32_t>() : nullptr;
   }
   // updateable.rs:691-727: an incremental step recomputes the affected cells
   // from the deduplicated per-ball contact lists (in a periodic box, of each
@@ -1986,7 +2025,11 @@ int upd_run(Upd* u, const RunOpts& o) {
   std::swap(ctx->pair_cnt, u->prev_pair_cnt);
   std::swap(ctx->pair_b, u->prev_pair_b);
   std::swap(ctx->rec, u->prev_rec);
-  if (adj) std::swap(u->topo, u->prev_topo);
+  if (adj) {
+    std::swap(u->topo, u->prev_topo);
+    std::swap(u->topo_slot, u->prev_topo_slot);
+    u->prev_topos = kTopoRegions * topo_region_for(u->topo_cap);
+  }
   u->prev_ok = true;
   u->prev_topo_ok = adj;
   u->prev_n = n;
@@ -2021,6 +2064,7 @@ int upd_prepare(Upd* u) {
     u->bk_topo_ok = u->prev_topo_ok;
     u->bk_n = u->prev_n;
     u->bk_pairs = u->prev_pairs;
+    u->bk_topos = u->prev_topos;
     if (u->prev_ok) {
       const size_t np = static_cast<size_t>(u->prev_pairs);
       VLTR_HIP(u->bk_pair_cnt.ensure((u->prev_n + 1) * sizeof(uint32_t)));
@@ -2032,9 +2076,15 @@ int upd_prepare(Upd* u) {
         VLTR_HIP(hipMemcpyAsync(u->bk_pair_b.ptr, u->prev_pair_b.ptr, np * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
         VLTR_HIP(hipMemcpyAsync(u->bk_rec.ptr, u->prev_rec.ptr, np * sizeof(PairRec), hipMemcpyDeviceToDevice, s));
         if (u->prev_topo_ok) {
-          VLTR_HIP(u->bk_topo.ensure(np * sizeof(ContactTopo)));
-          VLTR_HIP(hipMemcpyAsync(u->bk_topo.ptr, u->prev_topo.ptr, np * sizeof(ContactTopo), hipMemcpyDeviceToDevice,
-                                  s));
+          const size_t nt = static_cast<size_t>(u->prev_topos);
+          VLTR_HIP(u->bk_topo_slot.ensure(np * sizeof(uint32_t)));
+          VLTR_HIP(hipMemcpyAsync(u->bk_topo_slot.ptr, u->prev_topo_slot.ptr, np * sizeof(uint32_t),
+                                  hipMemcpyDeviceToDevice, s));
+          if (nt > 0) {
+            VLTR_HIP(u->bk_topo.ensure(nt * sizeof(ContactTopo)));
+            VLTR_HIP(hipMemcpyAsync(u->bk_topo.ptr, u->prev_topo.ptr, nt * sizeof(ContactTopo),
+                                    hipMemcpyDeviceToDevice, s));
+          }
         }
       }
     }
@@ -2191,7 +2241,8 @@ void vltr_updateable_destroy(vltr_updateable* u) {
     (void)hipStreamSynchronize(u->ctx->stream);
     for (DevBuf* b : {&u->cells_state, &u->cells_backup, &u->excl, &u->excl_backup, &u->affected_mask, &u->ids, &u->rows,
                       &u->rows_cnt, &u->moved_vals, &u->types, &u->grad, &u->part, &u->topo, &u->seed, &u->dedup_keys,
-                      &u->dedup_first, &u->prev_pair_cnt, &u->prev_pair_b, &u->prev_rec, &u->prev_topo,
+                      &u->dedup_first, &u->prev_pair_cnt, &u->prev_pair_b, &u->prev_rec, &u->prev_topo, &u->topo_slot,
+                      &u->prev_topo_slot, &u->bk_topo_slot,
                       &u->bk_pair_cnt, &u->bk_pair_b, &u->bk_rec, &u->bk_topo})
       b->release();
     u->stage.release();
@@ -2317,6 +2368,8 @@ int vltr_updateable_restore(vltr_updateable* u, int* ok) {
     std::swap(u->prev_pair_b, u->bk_pair_b);
     std::swap(u->prev_rec, u->bk_rec);
     std::swap(u->prev_topo, u->bk_topo);
+    std::swap(u->prev_topo_slot, u->bk_topo_slot);
+    u->prev_topos = u->bk_topos;
     u->prev_ok = true;
     u->prev_topo_ok = u->bk_topo_ok;
     u->prev_n = u->bk_n;

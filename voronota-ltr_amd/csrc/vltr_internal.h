@@ -47,10 +47,17 @@ constexpr int kBboxStride = 32;  // 256 bytes
 constexpr int kClipQueues = 8;
 constexpr int kClipQueueStride = 32;  // 32 words
 // The run's small device block: the box size word, the report words (from
-// byte 8), the bounding box (from byte 256, kBboxStride words apart) and the
-// clip queue counters (from kSmallQueueOff).
+// byte 8), the bounding box (from byte 256, kBboxStride words apart), the
+// clip queue counters (from kSmallQueueOff) and the topology store counters
+// (from kSmallTopoOff).
 constexpr size_t kSmallQueueOff = 256 + 6 * kBboxStride * sizeof(unsigned long long);
-constexpr size_t kSmallBytes = kSmallQueueOff + kClipQueues * kClipQueueStride * sizeof(uint32_t);
+// Adjoint runs: the topology store is split into kTopoRegions equal regions,
+// one per XCD, each with its record counter on its own line (from
+// kSmallTopoOff, kClipQueueStride words apart), so that the clip kernels'
+// reservations of different XCDs do not meet on one address.
+constexpr int kTopoRegions = 8;
+constexpr size_t kSmallTopoOff = kSmallQueueOff + kClipQueues * kClipQueueStride * sizeof(uint32_t);
+constexpr size_t kSmallBytes = kSmallTopoOff + kTopoRegions * kClipQueueStride * sizeof(uint32_t);
 
 // The run's report block: consecutive device words from p.status, copied to
 // the host in one transfer. The totals are copied in by k_copy_word after the
@@ -72,7 +79,9 @@ enum ReportWord : int {
   kReportVerts = 12,        // network runs: vertex records (the vertex count scan's total)
   kReportEdgesBeyond = 13,  // 1: the exact edge total exceeds INT32_MAX
   kReportVertsBeyond = 14,  // 1: the exact vertex total exceeds INT32_MAX
-  kReportWords = 15
+  kReportTopos = 15,        // adjoint runs: records reserved in each of the kTopoRegions
+                            // regions of the topology store (words 15 .. 22; may exceed a region)
+  kReportWords = kReportTopos + kTopoRegions
 };
 
 // Launch parameters of the tessellation pipeline for one call (one or many
@@ -130,7 +139,16 @@ struct Pipeline {
   double* slow_ws;         // workspace of the slow kernel
   int64_t slow_ws_pairs;   // its lanes (one workspace region each)
 
-  ContactTopo* topo;       // pair_cap or null: cut topology per pair (adjoint runs)
+  // Adjoint runs: the cut topologies of the contacts in a store of
+  // kTopoRegions regions of topo_region records (records reserved through
+  // the region counters topo_counts, kClipQueueStride words apart: a clip
+  // wavefront reserves one record per pair it clips, of its XCD's region),
+  // found through the pair's slot (topo_slot, pair_cap; kTopoSlot* values
+  // need no record). Null topo: no topologies.
+  ContactTopo* topo;
+  uint32_t* topo_slot;
+  uint32_t* topo_counts;
+  int64_t topo_region;
   ContactNet* net;         // pair_cap or null: restricted contours (network output)
   double* seed;            // n or null: dE/d(solid angle) per cell (adjoint runs)
   double sas_weight, volume_weight;  // alpha, beta of the adjoint's energy
@@ -149,7 +167,8 @@ struct Pipeline {
   const uint32_t* prev_pair_cnt;  // n + 1: pair bases of the previous run
   const int32_t* prev_pair_b;     // its pair partners
   const PairRec* prev_rec;        // its records
-  const ContactTopo* prev_topo;   // its topologies (read when topo is set)
+  const ContactTopo* prev_topo;   // its topology store and slots (read when topo is set)
+  const uint32_t* prev_topo_slot;
 
   int32_t clip_variant;  // launch shape of k_clip (index into its variant table)
   // Graph replays: the pair buffers hold pair_cap pairs (a capacity from an
@@ -219,6 +238,10 @@ hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream
 const uint32_t* scan_beyond_word(const uint32_t* tmp, int64_t n);
 // *dst = *src on the device (one word, stream-ordered).
 hipError_t launch_copy_word(uint32_t* dst, const uint32_t* src, hipStream_t s);
+// a[0 .. n) = v.
+hipError_t launch_fill_words(uint32_t* a, int64_t n, uint32_t v, hipStream_t s);
+// dst[k] = src[k * stride], k < count <= 64.
+hipError_t launch_copy_strided(uint32_t* dst, const uint32_t* src, int count, int stride, hipStream_t s);
 int64_t scan_tmp_words(int64_t n);
 
 // vltr_noalloc.hip: the fixed device workspace of the allocation-free entry

@@ -94,7 +94,7 @@ def test_clip_variants_give_identical_records(oracle):
     from voronota_ltr import _native, workload
 
     names = _native.clip_variants()
-    assert len(names) >= 2 and names[0] == "g8w1o4q"
+    assert len(names) >= 2 and names[0] == "g8w1o4nq"
     structs = [s for _, s, _ in _structures()] + [workload.rsa_sphere(20000, seed=20000)[0]]
     before = _native.clip_variant()
     try:

@@ -763,15 +763,16 @@ struct ClipVariant {
 // (w), the occupancy target in wavefronts per SIMD (o), unit normals kept in
 // shared memory (n), wavefronts per ball (b2), and the persistent form on the
 // device work queue (q). The first is the default: the fastest in the
-// autotuner's passes on the 100k-ball plain workload (pipeline 3.18 ms
-// against 3.29 for g8w1o4n, the fastest one-shot launch) and on the mixed
-// batch of 4,096 200-20000-ball structures (478 ms against 501). One
-// wavefront per block lets each ball's wavefront release its shared memory as
-// soon as it ends; keeping the unit normals costs 1.5 KB of it (3.5
-// wavefronts per SIMD) and saves their recomputation at every applied plane;
-// two wavefronts per ball build the plane table twice and leave the clip
-// rounds half full (3.88 ms).
+// autotuner's passes on the 100k-ball plain workload (pipeline 2.93 ms
+// against 3.05 for g8w1o4q and 3.06 for g8w1o4n, the fastest one-shot
+// launch) and level with g8w1o4q on the mixed batch of 4,096 200-20000-ball
+// structures (458 ms). One wavefront per block lets each ball's wavefront
+// release its shared memory as soon as it ends; keeping the unit normals
+// costs 1.5 KB of it (3.5 wavefronts per SIMD) and saves their recomputation
+// at every applied plane; two wavefronts per ball build the plane table twice
+// and leave the clip rounds half full (3.7 ms).
 const ClipVariant kClipVariants[] = {
+    {"g8w1o4nq", 1, true, 1, k_clip_queue<8, 4, true, 1, false>, k_clip_queue<8, 4, true, 1, true>},
     {"g8w1o4q", 1, true, 1, k_clip_queue<8, 4, false, 1, false>, k_clip_queue<8, 4, false, 1, true>},
     {"g8w1o4n", 1, false, 1, k_clip<8, 1, 4, true, 1, false>, k_clip<8, 1, 4, true, 1, true>},
     {"g8w1o4", 1, false, 1, k_clip<8, 1, 4, false, 1, false>, k_clip<8, 1, 4, false, 1, true>},
@@ -782,7 +783,6 @@ const ClipVariant kClipVariants[] = {
     {"g16w4o4", 4, false, 1, k_clip<16, 4, 4, false, 1, false>, k_clip<16, 4, 4, false, 1, true>},
     {"g8w1o3n", 1, false, 1, k_clip<8, 1, 3, true, 1, false>, k_clip<8, 1, 3, true, 1, true>},
     {"g8w1o4b2", 1, false, 2, k_clip<8, 1, 4, false, 2, false>, k_clip<8, 1, 4, false, 2, true>},
-    {"g8w1o4nq", 1, true, 1, k_clip_queue<8, 4, true, 1, false>, k_clip_queue<8, 4, true, 1, true>},
     {"g8w1o4qb2", 1, true, 2, k_clip_queue<8, 4, false, 2, false>, k_clip_queue<8, 4, false, 2, true>},
     {"g8w1o3q", 1, true, 1, k_clip_queue<8, 3, false, 1, false>, k_clip_queue<8, 3, false, 1, true>},
     {"g8w1o3nq", 1, true, 1, k_clip_queue<8, 3, true, 1, false>, k_clip_queue<8, 3, true, 1, true>},

@@ -336,10 +336,11 @@ def grid_build(grid_ms, counts):
             "frac": round(gbs / HBM_PEAK_GBS, 4), "buckets": counts["buckets"]}
 
 
-def autotune(lib, ctx, dptr, offsets, probe):
-    """The clip kernel's launch shapes timed on this workload (one warm-up,
-    two timed pipeline runs each, outside every timed region); the fastest
-    stays set on the context."""
+def autotune(lib, ctx, dptr, offsets, probe, repeats=3):
+    """The clip kernel's launch shapes timed on this workload (an untimed
+    round over all shapes, then `repeats` rounds over all of them; the median
+    pipeline time per shape; outside every timed region); the fastest stays
+    set on the context."""
     import ctypes as C
 
     import numpy as np
@@ -350,7 +351,7 @@ def autotune(lib, ctx, dptr, offsets, probe):
     ms = np.zeros(nv)
     chosen = C.c_int(0)
     _native.check(lib.vltr_context_autotune(ctx, dptr, offsets.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                            len(offsets) - 1, probe, 0, 2, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                            len(offsets) - 1, probe, 0, repeats, ms.ctypes.data_as(C.POINTER(C.c_double)),
                                             nv, C.byref(chosen)), "vltr_context_autotune")
     names = _native.clip_variants()
     return {"chosen": names[chosen.value], "pipeline_ms": {names[v]: round(float(ms[v]), 4) for v in range(nv)}}
@@ -382,7 +383,8 @@ def batched_leg(args, lib, ctx):
         _native.check(lib.vltr_compute_tessellation_batched_device(ctx, dbuf.ptr, offp, len(sizes), args.probe, 0),
                       "vltr_compute_tessellation_batched_device")
 
-    tune = None if (args.no_autotune or args.clip_shape is not None) else autotune(lib, ctx, dbuf.ptr, off, args.probe)
+    tune = (None if (args.no_autotune or args.clip_shape is not None)
+            else autotune(lib, ctx, dbuf.ptr, off, args.probe, repeats=2))
     batched_once()
     _hip.synchronize()
     reps = 2

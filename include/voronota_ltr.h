@@ -229,8 +229,9 @@ int vltr_context_set_clip_variant(vltr_context* ctx, int variant);
 int vltr_context_clip_variant(const vltr_context* ctx);
 /* Times every shape on a device-resident batch (arguments as
  * vltr_compute_tessellation_batched_device; one structure is a batch of one):
- * one warm-up run, then `repeats` runs whose mean pipeline time (HIP events,
- * ms) goes to ms[variant] (max_entries entries, ms may be NULL). The fastest
+ * one untimed round over all shapes, then `repeats` rounds each running
+ * every shape once; the median pipeline time of a shape (HIP events, ms)
+ * goes to ms[variant] (max_entries entries, ms may be NULL). The fastest
  * shape stays set on the context and is returned in *chosen. */
 int vltr_context_autotune(vltr_context* ctx, const vltr_ball* d_balls, const uint64_t* offsets, size_t n_structs,
                           double probe, int contacts_only, int repeats, double* ms, int max_entries, int* chosen);

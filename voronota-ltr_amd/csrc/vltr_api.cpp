@@ -1221,24 +1221,32 @@ int vltr_context_autotune(vltr_context* ctx, const vltr_ball* d_balls, const uin
   // direct runs only: each shape would otherwise be captured in a graph of its own
   const bool graphs = ctx->use_graphs;
   ctx->use_graphs = false;
-  double best = 0.0;
-  int best_v = before;
-  for (int v = 0; v < nv; ++v) {
-    ctx->clip_variant = v;
-    double sum = 0.0;
-    for (int r = 0; r <= repeats; ++r) {
+  // an untimed round over every shape, then `repeats` timed rounds, each
+  // over every shape in turn, so that no shape is timed while the device is
+  // still settling or always right after the same neighbor
+  std::vector<std::vector<double>> t(static_cast<size_t>(nv));
+  for (int r = 0; r <= repeats; ++r) {
+    for (int v = 0; v < nv; ++v) {
+      ctx->clip_variant = v;
       const int rc = vltr_compute_tessellation_batched_device(ctx, d_balls, offsets, n_structs, probe, contacts_only);
       if (rc != VLTR_OK) {
         ctx->clip_variant = before;
         ctx->use_graphs = graphs;
         return rc;
       }
-      if (r > 0) sum += ctx->timings[kPhases - 1];
+      if (r > 0) t[v].push_back(ctx->timings[kPhases - 1]);
     }
-    const double mean = sum / repeats;
-    if (ms && v < max_entries) ms[v] = mean;
-    if (v == 0 || mean < best) {
-      best = mean;
+  }
+  double best = 0.0;
+  int best_v = before;
+  for (int v = 0; v < nv; ++v) {
+    std::vector<double>& x = t[v];
+    std::sort(x.begin(), x.end());
+    const size_t k = x.size();
+    const double med = k % 2 ? x[k / 2] : 0.5 * (x[k / 2 - 1] + x[k / 2]);
+    if (ms && v < max_entries) ms[v] = med;
+    if (v == 0 || med < best) {
+      best = med;
       best_v = v;
     }
   }

@@ -363,7 +363,7 @@ def clip_variant(device=0):
 def autotune(structures, probe, contacts_only=False, repeats=2, device=0, d_balls=None):
     """Times every clip launch shape on the batch (a list of structures, or
     one structure) and keeps the fastest on the device's context; returns
-    (name, {name: mean pipeline ms}). d_balls: optional device pointer that
+    (name, {name: median pipeline ms}). d_balls: optional device pointer that
     already holds the concatenated balls."""
     from . import _hip
 

@@ -203,3 +203,33 @@ def test_pair_total_guard(n, per, extra, ok):
         assert got == exact
     else:
         assert got == -1, f"total {exact} (32-bit {t32.value}) was accepted"
+
+
+def test_canonical_id_without_division():
+    """canon_mod (vltr_internal.h): c mod n for sphere ids c < 27 n from the
+    double reciprocal of n, truncated to int32 and corrected by one either
+    way, equals the integer remainder: every id for small n, and ids around
+    every multiple of n (where the truncated quotient can be off by one) for
+    n up to the 2^31 / 27 balls the library accepts."""
+    import numpy as np
+
+    def canon(c, n):
+        inv = np.float64(1.0) / np.float64(n)
+        q = (c.astype(np.float64) * inv).astype(np.int64)  # truncation, as the int32 conversion
+        r = c - q * n
+        r = np.where(r < 0, r + n, np.where(r >= n, r - n, r))
+        assert np.all(q <= 27) and np.all(q * n < 2**31)
+        return r
+
+    for n in range(1, 300):
+        c = np.arange(27 * n, dtype=np.int64)
+        assert np.array_equal(canon(c, n), c % n)
+    rng = np.random.default_rng(27)
+    cap = (1 << 31) // 27
+    for n in [cap, cap - 1, 1 << 26, (1 << 26) + 1, 3, 1000003] + list(rng.integers(2, cap, 200)):
+        n = int(n)
+        k = np.arange(28, dtype=np.int64)
+        c = (k[:, None] * n + np.arange(-3, 4)[None, :]).ravel()
+        c = c[(c >= 0) & (c < 27 * n)]
+        c = np.concatenate([c, rng.integers(0, 27 * n, 2000)])
+        assert np.array_equal(canon(c, n), c % n)

@@ -29,7 +29,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 }
 
 __device__ __forceinline__ unsigned long long pair_key(const Pipeline& p, int64_t t) {
-  const int64_t ca = p.pair_a[t] % p.n, cb = p.pair_b[t] % p.n;
+  const int64_t ca = canon_id(p, p.pair_a[t]), cb = canon_id(p, p.pair_b[t]);
   const int64_t lo = ca < cb ? ca : cb, hi = ca < cb ? cb : ca;
   return static_cast<unsigned long long>(lo) * static_cast<unsigned long long>(p.n) + hi + 1ull;
 }

@@ -33,8 +33,9 @@ struct W16 {
 struct DevAcc {
   double* grad;
   int64_t n;
+  double inv_n;
   __device__ void add(int64_t id, V3 v) {
-    const int64_t i = id % n;
+    const int64_t i = canon_mod(id, n, inv_n);
     atomicAdd(grad + 3 * i, v.x);
     atomicAdd(grad + 3 * i + 1, v.y);
     atomicAdd(grad + 3 * i + 2, v.z);
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(VLTR_
   const int ln = threadIdx.x & (kGroup - 1);
   WaveGroup G{(gl % (64 / kGroup)) * kGroup, ln};
   const double* sph4 = reinterpret_cast<const double*>(p.sph);
-  DevAcc acc{grad, p.n};
+  DevAcc acc{grad, p.n, p.inv_n};
   double e = 0.0;
   const int64_t groups = static_cast<int64_t>(gridDim.x) * kAdjGroups;
   const int64_t n_list = *count;
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(VLTR_
     int m = 0;
     const ContactTopo& topo = *contact_topo(p, t, &m);  // listed by k_adj_flag: m >= 0
     const int32_t a = p.pair_a[t], b = p.pair_b[t];
-    const int64_t ca = a % p.n, cb = b % p.n;
+    const int64_t ca = canon_id(p, a), cb = canon_id(p, b);
     const bool image = p.periodic && b >= p.n;
     const bool first = image && dedup_first(p, T, t);
     const double w = !image || first ? ws[4 * types[ca] + types[cb]] : 0.0;

@@ -200,7 +200,7 @@ __global__ void k_dedup_incoming(Pipeline p, uint64_t mask) {
   if (b < p.n) return;
   const PairRec& r = p.rec[t];
   if (!(r.flags & kRecValid) || !(r.area > 0.0)) return;
-  const int64_t cb = b % p.n;
+  const int64_t cb = canon_id(p, b);
   if (cb == p.pair_a[t] || !dedup_first(p, DedupTable{p.dd_keys, p.dd_first, mask}, t)) return;
   const uint32_t slot = atomicAdd(&p.in_cnt[cb], 1u);
   if (slot < static_cast<uint32_t>(p.nb_cap))
@@ -240,7 +240,7 @@ __global__ void k_emit_contacts(Pipeline p, const uint32_t* pos, vltr_contact* o
   vltr_contact c;
   if (p.periodic) {
     // canonical ids; contacts-only runs order them (tessellation.rs:176-186)
-    int64_t ca = a % p.n, cb = p.pair_b[i] % p.n;
+    int64_t ca = canon_id(p, a), cb = canon_id(p, p.pair_b[i]);
     if (p.mode != 0 && ca > cb) {
       const int64_t t = ca;
       ca = cb;
@@ -385,7 +385,7 @@ __global__ void k_mark_neighbors(Pipeline p, const int32_t* changed, int64_t k, 
   const int64_t r = t / p.nb_cap, j = t - r * p.nb_cap;
   const int32_t id = changed[r];
   if (id < 0 || id >= p.n || j >= p.nb_cnt[id]) return;
-  uint8_t* m = affected + p.nb[static_cast<int64_t>(id) * p.nb_cap + j] % p.n;
+  uint8_t* m = affected + canon_id(p, p.nb[static_cast<int64_t>(id) * p.nb_cap + j]);
   if (*m == 0) *m = value;
 }
 

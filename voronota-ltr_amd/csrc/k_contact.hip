@@ -287,7 +287,7 @@ __device__ __forceinline__ void push_incoming(const Pipeline& p, int64_t pidx, i
 }
 
 __device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
-  return p.dirty[p.periodic ? id % p.n : id] != 0;
+  return p.dirty[p.periodic ? canon_id(p, id) : id] != 0;
 }
 
 // One clip task of a wavefront: ball q (in cell order), share wsub of its

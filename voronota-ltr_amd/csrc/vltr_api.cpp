@@ -518,7 +518,7 @@ int run_pipeline(vltr_context* ctx, int64_t n, double probe, const vltr_periodic
   const bool periodic = pbox != nullptr;
   const int64_t n_pop = periodic ? 27 * n : n;
   Pipeline p{};
-  p.n = n;
+  set_balls(p, n);
   p.n_pop = n_pop;
   const bool batched = seg_offsets != nullptr && n_structs > 0;
   p.n_segs = batched ? static_cast<int32_t>(n_structs) : 1;
@@ -718,7 +718,7 @@ int device_network(vltr_context* ctx, int64_t n, bool periodic, vltr_result* r) 
 
   Pipeline p{};
   fill_pointers(ctx, p);
-  p.n = n;
+  set_balls(p, n);
   p.n_pairs = static_cast<int64_t>(ctx->counts[2]);
   p.periodic = periodic ? 1 : 0;
   p.net = ctx->net.as<ContactNet>();
@@ -1083,7 +1083,8 @@ int run_batched_graph(vltr_context* ctx, const vltr_ball* d_balls, const uint64_
     return run_batched(ctx, offsets, n_structs, probe, mode, nc, ncell);
   }
   Pipeline p{};
-  p.n = p.n_pop = static_cast<int64_t>(n);
+  set_balls(p, static_cast<int64_t>(n));
+  p.n_pop = p.n;
   p.mode = mode;
   p.nb_cap = ctx->graph_nb_cap;
   p.n_buckets = ctx->graph_buckets;
@@ -1549,7 +1550,7 @@ int run_noalloc(vltr_context* ctx, const vltr_ball* d_balls, int64_t n, double p
   hipStream_t s = ctx->stream;
   uint32_t* host_words = static_cast<uint32_t*>(ctx->pinned.ptr);
   Pipeline p{};
-  p.n = n;
+  set_balls(p, n);
   p.n_pop = n;
   p.n_segs = 1;
   p.mode = mode;
@@ -1843,7 +1844,7 @@ int upd_rows(Upd* u, const std::vector<uint64_t>& ids, std::vector<std::vector<u
   hipStream_t s = ctx->stream;
   VLTR_HIP(hipMemcpyAsync(u->ids.ptr, id32.data(), id32.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
   Pipeline p{};
-  p.n = static_cast<int64_t>(n);
+  set_balls(p, static_cast<int64_t>(n));
   p.nb = ctx->nb.as<int32_t>();
   p.nb_cnt = ctx->nb_cnt.as<int32_t>();
   p.nb_cap = cap;
@@ -2173,7 +2174,7 @@ int upd_update(Upd* u, const vltr_ball* balls, size_t nb, const uint64_t* change
   VLTR_HIP(launch_fill_u8(am, static_cast<int64_t>(n), 0, s));
   VLTR_HIP(launch_mark_ids(am, static_cast<int64_t>(n), d_ids, static_cast<int64_t>(k), 3, s));
   Pipeline last{};
-  last.n = static_cast<int64_t>(n);
+  set_balls(last, static_cast<int64_t>(n));
   last.nb = ctx->nb.as<int32_t>();
   last.nb_cnt = ctx->nb_cnt.as<int32_t>();
   last.nb_cap = ctx->last_nb_cap;

@@ -88,6 +88,7 @@ enum ReportWord : int {
 // structures). All pointers are device pointers.
 struct Pipeline {
   int64_t n;         // balls (canonical)
+  double inv_n;      // 1 / n (canon_id), set with n by set_balls
   int64_t n_pop;     // spheres searched (n, or 27 n with periodic images)
   int32_t n_segs;
   int32_t periodic;  // 1: sph holds 27 n images, ids >= n are images
@@ -218,8 +219,28 @@ __device__ __forceinline__ bool same_group(const Pipeline& p, int64_t a, int64_t
 // tessellation.rs:298-319 (and updateable.rs:516-561 without involvement):
 // (a, c) is a candidate pair when not in the same group and a < c, or c is a
 // periodic image.
+// p.n and its reciprocal (canon_id).
+inline void set_balls(Pipeline& p, int64_t n) {
+  p.n = n;
+  p.inv_n = n > 0 ? 1.0 / static_cast<double>(n) : 0.0;
+}
+
+// c mod n for a sphere id c (0 <= c < 27 n < 2^31: a ball or one of its
+// images) without an integer division: the quotient from the reciprocal,
+// which is off by at most one either way (c / n <= 27 carries a relative
+// error of a few ulp), then corrected.
+__device__ __forceinline__ int64_t canon_mod(int64_t c, int64_t n64, double inv_n) {
+  const int32_t n = static_cast<int32_t>(n64);
+  const int32_t ci = static_cast<int32_t>(c);
+  const int32_t q = static_cast<int32_t>(static_cast<double>(ci) * inv_n);
+  int32_t r = ci - q * n;
+  r = r < 0 ? r + n : (r >= n ? r - n : r);
+  return r;
+}
+__device__ __forceinline__ int64_t canon_id(const Pipeline& p, int64_t c) { return canon_mod(c, p.n, p.inv_n); }
+
 __device__ __forceinline__ bool pair_qualifies(const Pipeline& p, int64_t a, int64_t c) {
-  const int64_t cc = p.periodic ? c % p.n : c;
+  const int64_t cc = p.periodic ? canon_id(p, c) : c;
   if (same_group(p, a, cc)) return false;
   if (p.user_excl != nullptr && (p.user_excl[a] || p.user_excl[cc])) return false;
   return (p.periodic && c >= p.n) || a < c;

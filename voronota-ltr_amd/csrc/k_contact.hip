@@ -158,18 +158,36 @@ __device__ int64_t prev_find_lane(const Pipeline& p, int64_t ob, int on, int32_t
   return -1;
 }
 
-// Pair `to` takes the previous run's record (and topology) of pair `from`,
-// or an empty record when from < 0; the wavefront's lanes copy one word each.
-// A kept contact's stored topology is copied to a new record of this run's
-// store.
+// Full runs: a contact of pair pidx = (a, b), clipped or kept from the
+// previous run, joins b's incoming list when b is canonical
+// (tessellation.rs:624-631: an image end receives nothing; the incoming order
+// is free, k_cells puts each list in pair order) and is marked in the pair's
+// contact byte. Pairs without a contact are not listed: the cells and the
+// solvent kernels read contacts only.
+__device__ __forceinline__ void push_incoming(const Pipeline& p, int64_t pidx, int32_t b, const PairRec& rec) {
+  if (p.mode != 0 || !(rec.flags & kRecValid) || !(rec.area > 0.0)) return;
+  if (p.pair_contact != nullptr) p.pair_contact[pidx] = 1;
+  if (b >= p.n) return;
+  const uint32_t slot = atomicAdd(&p.in_cnt[b], 1u);
+  if (slot < static_cast<uint32_t>(p.nb_cap))
+    p.incoming[static_cast<int64_t>(b) * p.nb_cap + slot] = static_cast<int32_t>(pidx);
+  else
+    atomicOr(p.status, kStatNbOverflow);
+}
+
+// Pair `to` = (a, b) takes the previous run's record (and topology) of pair
+// `from`, or an empty record when from < 0; the wavefront's lanes copy one
+// word each. A kept contact joins b's incoming list, and its stored topology
+// is copied to a new record of this run's store.
 template <bool kTopo>
-__device__ void keep_pair_wave(const Pipeline& p, int64_t from, int64_t to, int lane) {
+__device__ void keep_pair_wave(const Pipeline& p, int64_t from, int64_t to, int32_t b, int lane) {
   if (from < 0) {
     if (lane == 0) p.rec[to] = empty_record();
     return;
   }
   if (lane < kRecWords)
     reinterpret_cast<uint64_t*>(p.rec + to)[lane] = reinterpret_cast<const uint64_t*>(p.prev_rec + from)[lane];
+  if (lane == 0) push_incoming(p, to, b, p.prev_rec[from]);
   if (!kTopo || p.topo == nullptr || !is_contact(p.prev_rec[from])) return;
   const uint32_t ps = p.prev_topo_slot[from];
   uint32_t slot = ps;
@@ -183,13 +201,14 @@ __device__ void keep_pair_wave(const Pipeline& p, int64_t from, int64_t to, int 
   if (lane == 0) p.topo_slot[to] = slot;
 }
 template <bool kTopo>
-__device__ void keep_pair_lane(const Pipeline& p, int64_t from, int64_t to) {
+__device__ void keep_pair_lane(const Pipeline& p, int64_t from, int64_t to, int32_t b) {
   if (from < 0) {
     p.rec[to] = empty_record();
     return;
   }
   const PairRec rec = p.prev_rec[from];
   p.rec[to] = rec;
+  push_incoming(p, to, b, rec);
   if (!kTopo || p.topo == nullptr || !is_contact(rec)) return;
   const uint32_t ps = p.prev_topo_slot[from];
   uint32_t slot = ps;
@@ -213,6 +232,7 @@ __device__ void keep_ball(const Pipeline& p, int64_t base, int npairs, int64_t o
     const uint64_t* rs = reinterpret_cast<const uint64_t*>(p.prev_rec + ob);
     uint64_t* rd = reinterpret_cast<uint64_t*>(p.rec + base);
     for (int t = lane; t < npairs * kRecWords; t += kWave) rd[t] = rs[t];
+    for (int j = lane; j < npairs; j += kWave) push_incoming(p, base + j, p.prev_pair_b[ob + j], p.prev_rec[ob + j]);
     if (kTopo && p.topo != nullptr) {
       for (int j0 = 0; j0 < npairs; j0 += kWave) {
         const int j = j0 + lane;
@@ -237,8 +257,10 @@ __device__ void keep_ball(const Pipeline& p, int64_t base, int npairs, int64_t o
     }
     return;
   }
-  for (int j = 0; j < npairs; ++j)
-    keep_pair_wave<kTopo>(p, prev_find_wave(p, ob, on, p.pair_b[base + j], lane), base + j, lane);
+  for (int j = 0; j < npairs; ++j) {
+    const int32_t b = p.pair_b[base + j];
+    keep_pair_wave<kTopo>(p, prev_find_wave(p, ob, on, b, lane), base + j, b, lane);
+  }
 }
 
 // The pair list of ball a (tessellation.rs:298-319: its qualifying neighbors
@@ -269,23 +291,6 @@ __device__ __forceinline__ void write_pairs(const Pipeline& p, int64_t a, int64_
   }
 }
 
-// Full runs whose clip writes the pair lists: a contact of pair pidx = (a, b)
-// joins b's incoming list when b is canonical (tessellation.rs:624-631: an
-// image end receives nothing; the incoming order is free, k_cells puts each
-// list in pair order). Pairs without a contact are not listed: the cells and
-// the solvent kernels read contacts only. (Updateable steps list every pair
-// in k_pair_fill before the clip.)
-__device__ __forceinline__ void push_incoming(const Pipeline& p, int64_t pidx, int32_t b, const PairRec& rec) {
-  if (!p.pairs_in_clip || p.mode != 0 || !(rec.flags & kRecValid) || !(rec.area > 0.0)) return;
-  if (p.pair_contact != nullptr) p.pair_contact[pidx] = 1;
-  if (b >= p.n) return;
-  const uint32_t slot = atomicAdd(&p.in_cnt[b], 1u);
-  if (slot < static_cast<uint32_t>(p.nb_cap))
-    p.incoming[static_cast<int64_t>(b) * p.nb_cap + slot] = static_cast<int32_t>(pidx);
-  else
-    atomicOr(p.status, kStatNbOverflow);
-}
-
 __device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
   return p.dirty[p.periodic ? canon_id(p, id) : id] != 0;
 }
@@ -311,7 +316,16 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
   const int32_t* nbl = p.nb + a * static_cast<int64_t>(p.nb_cap);
   const double* sph4 = reinterpret_cast<const double*>(p.sph);
   const bool incremental = p.dirty != nullptr;
-  if (p.pairs_in_clip && wsub == 0) write_pairs(p, a, base, cnt, nbl, lane);
+  if (wsub == 0) {
+    write_pairs(p, a, base, cnt, nbl, lane);
+    if (incremental) {
+      // the keep paths below read this ball's pair list back: its stores
+      // complete before those loads
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+  }
   int64_t ob = 0;
   int on = 0;
   int32_t prev_b = -1;
@@ -330,7 +344,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     for (int j = lane; j < npairs; j += kWave) {
       const int32_t b = p.pair_b[base + j];
       if (incremental && !is_dirty(p, b))
-        keep_pair_lane<kOut>(p, prev_find_lane(p, ob, on, b), base + j);
+        keep_pair_lane<kOut>(p, prev_find_lane(p, ob, on, b), base + j, b);
       else
         push_wide(p, base + j);
     }
@@ -395,7 +409,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
       } else {
         from = prev_find_wave(p, ob, on, b, lane);
       }
-      keep_pair_wave<kOut>(p, from, base + jj, lane);
+      keep_pair_wave<kOut>(p, from, base + jj, b, lane);
       continue;
     }
     if (!((ok_mask >> k) & 1ull)) {

@@ -497,79 +497,6 @@ __global__ __launch_bounds__(kBigBlock) void k_neighbors_big(Pipeline p) {
   }
 }
 
-// Pair list in reference order: by first ball, then neighbor order. Each pair
-// (a, c) whose second ball is canonical is also appended to c's incoming list
-// (tessellation.rs:624-631: an image end receives nothing); the incoming
-// order is free, k_cells puts each list in pair order. A lane per ball: the
-// device-scope atomic appends (one per pair) set the pace, and spreading each
-// lane's appends over its list keeps fewer of them in flight at once than a
-// wavefront per ball does (0.16 vs 0.21 ms for the 2.45 M pairs of 100k balls).
-__global__ void k_pair_fill(Pipeline p) {
-  const int64_t a = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (a >= p.n) return;
-  const int64_t base = p.pair_cnt[a];
-  const int cnt = p.nb_cnt[a];
-  const int32_t* lst = p.nb + a * static_cast<int64_t>(p.nb_cap);
-  int64_t j = 0;
-  // list entries read four at a time (the rows are 16-byte aligned: nb_cap is
-  // a multiple of 4): a quarter of the dependent loads per lane
-  const int4* lst4 = reinterpret_cast<const int4*>(lst);
-  int4 q = cnt > 0 ? lst4[0] : make_int4(0, 0, 0, 0);
-  for (int k = 0; k < cnt; ++k) {
-    if ((k & 3) == 0 && k > 0) q = lst4[k >> 2];
-    const int32_t c = (k & 3) == 0 ? q.x : (k & 3) == 1 ? q.y : (k & 3) == 2 ? q.z : q.w;
-    if (!pair_qualifies(p, a, c)) continue;
-    if (base + j < p.pair_cap) {
-      p.pair_a[base + j] = static_cast<int32_t>(a);
-      p.pair_b[base + j] = c;
-      if (c < p.n) {
-        const uint32_t slot = atomicAdd(&p.in_cnt[c], 1u);
-        if (slot < static_cast<uint32_t>(p.nb_cap))
-          p.incoming[static_cast<int64_t>(c) * p.nb_cap + slot] = static_cast<int32_t>(base + j);
-        else
-          atomicOr(p.status, kStatNbOverflow);
-      }
-    } else {
-      atomicOr(p.status, kStatPairOverflow);
-    }
-    ++j;
-  }
-}
-
-// The same pair list without incoming lists (contacts only: no cells). One
-// wavefront per ball, a lane per list entry; the qualifying entries take
-// consecutive slots by a ballot, so the stores are coalesced.
-constexpr int kFillBlock = 256;
-__global__ __launch_bounds__(kFillBlock) void k_pair_fill_wave(Pipeline p) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t a = blockIdx.x * static_cast<int64_t>(kFillBlock / kWave) + threadIdx.x / kWave;
-  if (a >= p.n) return;  // the whole wavefront
-  const int64_t base = p.pair_cnt[a];
-  const int cnt = p.nb_cnt[a];
-  const int32_t* lst = p.nb + a * static_cast<int64_t>(p.nb_cap);
-  int64_t j = 0;
-  for (int k0 = 0; k0 < cnt; k0 += kWave) {
-    const int k = k0 + lane;
-    int32_t c = 0;
-    bool q = false;
-    if (k < cnt) {
-      c = lst[k];
-      q = pair_qualifies(p, a, c);
-    }
-    const unsigned long long m = __ballot(q);
-    if (q) {
-      const int64_t idx = base + j + __popcll(m & lanemask_lt());
-      if (idx < p.pair_cap) {
-        p.pair_a[idx] = static_cast<int32_t>(a);
-        p.pair_b[idx] = c;
-      } else {
-        atomicOr(p.status, kStatPairOverflow);
-      }
-    }
-    j += __popcll(m);
-  }
-}
-
 __global__ void k_gather_rows(Pipeline p, const int32_t* ids, int64_t k, int32_t* cnt, int32_t* rows) {
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (t >= k * p.nb_cap) return;
@@ -742,29 +669,17 @@ hipError_t launch_neighbors(const Pipeline& p, hipStream_t s) {
 }
 
 hipError_t launch_pairs(const Pipeline& p, hipStream_t s) {
-  if (p.n > 0 && p.pairs_in_clip) {
-    // the clip kernel writes the lists; only the incoming counts and the
-    // contact marks start at 0
-    if (p.mode == 0) {
-      const int64_t fb = blocks_for(p.n, 256) < 4096 ? blocks_for(p.n, 256) : 4096;
-      hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, s, p.in_cnt, p.n, 0u);
-      if (p.pair_contact != nullptr && p.n_pairs > 0) {
-        const int64_t words = (p.n_pairs + 3) / 4;
-        const int64_t fw = blocks_for(words, 256) < 4096 ? blocks_for(words, 256) : 4096;
-        hipLaunchKernelGGL(k_fill_u32, dim3(fw), dim3(256), 0, s, reinterpret_cast<uint32_t*>(p.pair_contact), words,
-                           0u);
-      }
-    }
-    return hipGetLastError();
-  }
-  if (p.n > 0) {
-    if (p.mode != 0) {
-      hipLaunchKernelGGL(k_pair_fill_wave, dim3(blocks_for(p.n, kFillBlock / kWave)), dim3(kFillBlock), 0, s, p);
-      return hipGetLastError();
-    }
+  // the clip kernels write the pair lists and the incoming lists; the
+  // incoming counts and the contact marks start at 0
+  if (p.n > 0 && p.mode == 0) {
     const int64_t fb = blocks_for(p.n, 256) < 4096 ? blocks_for(p.n, 256) : 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, s, p.in_cnt, p.n, 0u);
-    hipLaunchKernelGGL(k_pair_fill, dim3(blocks_for(p.n, 256)), dim3(256), 0, s, p);
+    if (p.pair_contact != nullptr && p.n_pairs > 0) {
+      const int64_t words = (p.n_pairs + 3) / 4;
+      const int64_t fw = blocks_for(words, 256) < 4096 ? blocks_for(words, 256) : 4096;
+      hipLaunchKernelGGL(k_fill_u32, dim3(fw), dim3(256), 0, s, reinterpret_cast<uint32_t*>(p.pair_contact), words,
+                         0u);
+    }
   }
   return hipGetLastError();
 }

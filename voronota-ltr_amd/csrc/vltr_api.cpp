@@ -402,8 +402,7 @@ int checked_pairs(const uint32_t* host_words, int64_t* P) {
 int enqueue_back(vltr_context* ctx, Pipeline& p, const RunBufs& b, const RunExtras* extras, bool batched, bool events,
                  uint32_t* host_words, hipStream_t s) {
   const int64_t n = p.n;
-  p.pairs_in_clip = (extras == nullptr || extras->dirty == nullptr) ? 1 : 0;
-  if (!p.pairs_in_clip || p.mode != 0) p.pair_contact = nullptr;
+  if (p.mode != 0) p.pair_contact = nullptr;
   VLTR_HIP(launch_pairs(p, s));
   if (extras && extras->pre_clip) VLTR_HIP(extras->pre_clip(p, s));
   if (extras && extras->dirty) {

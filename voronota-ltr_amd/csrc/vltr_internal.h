@@ -126,8 +126,8 @@ struct Pipeline {
   int32_t* pair_b;       // pair_cap
   PairRec* rec;          // pair_cap
   // pair_cap (rounded up to 4) or null: 1 where the pair has a contact,
-  // written by the clip kernels as they write the records (full runs whose
-  // clip writes the pair lists); k_cells then loads only those own records
+  // written by the clip kernels as they write or keep the records (full
+  // runs); k_cells then loads only those own records
   uint8_t* pair_contact;
   CellRec* cell;         // n
   uint32_t* status;      // 1 word; the first of the run's report block (kReport* words)
@@ -153,9 +153,10 @@ struct Pipeline {
   ContactNet* net;         // pair_cap or null: restricted contours (network output)
   double* seed;            // n or null: dE/d(solid angle) per cell (adjoint runs)
   double sas_weight, volume_weight;  // alpha, beta of the adjoint's energy
-  // Incoming lists: the pairs (a, b), a < b, received by ball b as their
-  // second end; only those with a contact when the clip writes the pair lists
-  // (pairs_in_clip), every pair in updateable steps (k_pair_fill).
+  // Incoming lists (full runs): the contacts (a, b), a < b, received by ball
+  // b as their second end, appended by the clip kernels as they write or keep
+  // each contact's record (the pair lists are written by the clip kernels
+  // too, as they walk each ball's neighbors).
   uint32_t* in_cnt;        // n: list lengths
   int32_t* incoming;       // n * nb_cap: their pair indices (any order)
 
@@ -194,12 +195,6 @@ struct Pipeline {
   // Contacts the output buffer holds (0: one per candidate pair, the
   // context's own buffer); contacts beyond it are counted, not written.
   int64_t contacts_cap;
-
-  // 1: the clip kernels write the pair lists themselves, as they walk each
-  // ball's neighbors, and append each contact to its incoming list as they
-  // write its record; 0: k_pair_fill writes both before the clip (updateable
-  // steps, whose clip reads the pair lists of the step first).
-  int32_t pairs_in_clip;
 };
 
 // A graph replay whose pair total outgrew the capacity it was captured with

@@ -56,8 +56,7 @@ __global__ void k_dedup_insert(Pipeline p, DedupTable T, int all) {
   if (t >= p.n_pairs) return;
   // graph replays: pairs past the device-side total hold no record of this run
   if (p.n_pairs_dev != nullptr && t >= static_cast<int64_t>(*p.n_pairs_dev)) return;
-  const PairRec rec = p.rec[t];
-  if (!(rec.flags & kRecValid) || !(rec.area > 0.0) || (all == 0 && p.pair_b[t] < p.n)) return;
+  if ((all == 0 && p.pair_b[t] < p.n) || !has_contact(p, t)) return;
   const unsigned long long key = pair_key(p, t);
   uint64_t slot = mix64(key) & T.mask;
   for (;;) {
@@ -97,8 +96,7 @@ __device__ __forceinline__ const ContactTopo* contact_topo(const Pipeline& p, in
 __global__ void k_adj_flag(Pipeline p, uint32_t* flag) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= p.n_pairs) return;
-  const PairRec rec = p.rec[i];
-  bool with = (rec.flags & kRecValid) && rec.area > 0.0;
+  bool with = has_contact(p, i);
   if (with) {
     int m = 0;
     contact_topo(p, i, &m);

@@ -198,8 +198,7 @@ __global__ void k_dedup_incoming(Pipeline p, uint64_t mask) {
   if (t >= p.n_pairs) return;
   const int64_t b = p.pair_b[t];
   if (b < p.n) return;
-  const PairRec& r = p.rec[t];
-  if (!(r.flags & kRecValid) || !(r.area > 0.0)) return;
+  if (!has_contact(p, t)) return;
   const int64_t cb = canon_id(p, b);
   if (cb == p.pair_a[t] || !dedup_first(p, DedupTable{p.dd_keys, p.dd_first, mask}, t)) return;
   const uint32_t slot = atomicAdd(&p.in_cnt[cb], 1u);
@@ -223,7 +222,7 @@ __global__ void k_flag_contacts(Pipeline p, DedupTable T, uint32_t* flag) {
     flag[i] = 0u;
     return;
   }
-  bool keep = (p.rec[i].flags & kRecValid) && p.rec[i].area > 0.0;
+  bool keep = has_contact(p, i);
   if (keep && T.keys != nullptr && (p.mode != 0 || p.pair_b[i] >= p.n)) keep = dedup_first(p, T, i);
   flag[i] = keep ? 1u : 0u;
 }

@@ -129,6 +129,15 @@ __device__ __forceinline__ void copy_topo(ContactTopo* dst, const ContactTopo* s
   for (int k = 0; k < kTopoWords; ++k) d[k] = s[k];
 }
 
+// The record of pair pidx, stored when it holds a contact or when the run
+// keeps every pair's record (updateable steps: the next step copies the
+// records of the pairs it does not clip). Where a run keeps contact bytes the
+// readers ask those which pairs have a contact (has_contact), so the records
+// of the other pairs are not written.
+__device__ __forceinline__ void put_record(const Pipeline& p, int64_t pidx, const PairRec& rec) {
+  if (p.records_all || p.pair_contact == nullptr || is_contact(rec)) p.rec[pidx] = rec;
+}
+
 // k topology records of the region of this block's XCD: the first of their
 // indices within the region (counted past the region's end).
 __device__ __forceinline__ uint32_t topo_reserve(const Pipeline& p, uint32_t k) {
@@ -165,9 +174,9 @@ __device__ int64_t prev_find_lane(const Pipeline& p, int64_t ob, int on, int32_t
 // contact byte. Pairs without a contact are not listed: the cells and the
 // solvent kernels read contacts only.
 __device__ __forceinline__ void push_incoming(const Pipeline& p, int64_t pidx, int32_t b, const PairRec& rec) {
-  if (p.mode != 0 || !(rec.flags & kRecValid) || !(rec.area > 0.0)) return;
+  if (!is_contact(rec)) return;
   if (p.pair_contact != nullptr) p.pair_contact[pidx] = 1;
-  if (b >= p.n) return;
+  if (p.mode != 0 || b >= p.n) return;
   const uint32_t slot = atomicAdd(&p.in_cnt[b], 1u);
   if (slot < static_cast<uint32_t>(p.nb_cap))
     p.incoming[static_cast<int64_t>(b) * p.nb_cap + slot] = static_cast<int32_t>(pidx);
@@ -413,7 +422,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
       continue;
     }
     if (!((ok_mask >> k) & 1ull)) {
-      if (lane == 0) p.rec[base + jj] = empty_record();
+      if (lane == 0) put_record(p, base + jj, empty_record());
       continue;
     }
     // classify reads the circle, its direction from a, and b
@@ -425,7 +434,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     const bool kill = __ballot(cl == kKill) != 0ull;
     const uint64_t M = __ballot(cl == kCut);
     if (kill) {
-      if (lane == 0) p.rec[base + jj] = empty_record();
+      if (lane == 0) put_record(p, base + jj, empty_record());
     } else if (M == 0ull) {
       if (lane == 0) w.item[kClipSlots - 1 - n_uncut] = (jj << 8) | k;
       ++n_uncut;
@@ -517,7 +526,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
         if (st == kOverflow) {
           push_wide(p, pidx);
         } else {
-          p.rec[pidx] = rec;
+          put_record(p, pidx, rec);
           push_incoming(p, pidx, b, rec);
         }
       }
@@ -607,7 +616,7 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
     const Sph A = load_sph(sph4, a);
     PairGeom g;
     if (!pair_setup(A, load_sph(sph4, b), &g)) {
-      if (ln == 0) p.rec[pidx] = empty_record();
+      if (ln == 0) put_record(p, pidx, empty_record());
       continue;
     }
     bool kill = false;
@@ -635,7 +644,7 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
     }
     G.sync();
     if (kill) {
-      if (ln == 0) p.rec[pidx] = empty_record();
+      if (ln == 0) put_record(p, pidx, empty_record());
       continue;
     }
     if (!any) {
@@ -675,7 +684,7 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       if (st == kOverflow) {
         push_slow(p, pidx);
       } else {
-        p.rec[pidx] = rec;
+        put_record(p, pidx, rec);
         push_incoming(p, pidx, b, rec);
       }
     }
@@ -705,7 +714,7 @@ __global__ __launch_bounds__(64) void k_cut_slow(Pipeline p) {
     ClipOut co;
     const int st = clip(sph4, a, b, nbl, p.nb_cnt[a], c_hex_s, c_hex_c, ct, out, &co);
     const PairRec rec = record_of(st, co, load_sph(sph4, a), load_sph(sph4, b), out, p.mode);
-    p.rec[pi] = rec;
+    put_record(p, pi, rec);
     push_incoming(p, pi, b, rec);
     if (p.topo != nullptr && st == kContact) {
       const uint32_t slot = co.m > kTopoCap ? kTopoSlotTooLong : topo_record(p, topo_reserve(p, 1u));

@@ -674,6 +674,8 @@ hipError_t launch_pairs(const Pipeline& p, hipStream_t s) {
   if (p.n > 0 && p.mode == 0) {
     const int64_t fb = blocks_for(p.n, 256) < 4096 ? blocks_for(p.n, 256) : 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, s, p.in_cnt, p.n, 0u);
+  }
+  if (p.n > 0) {
     if (p.pair_contact != nullptr && p.n_pairs > 0) {
       const int64_t words = (p.n_pairs + 3) / 4;
       const int64_t fw = blocks_for(words, 256) < 4096 ? blocks_for(words, 256) : 4096;

@@ -28,7 +28,7 @@ inline int64_t blocks_for(int64_t n, int64_t per) { return (n + per - 1) / per; 
 
 __device__ __forceinline__ bool net_contact(const Pipeline& p, int64_t i) {
   if (p.n_pairs_dev != nullptr && i >= static_cast<int64_t>(*p.n_pairs_dev)) return false;
-  return (p.rec[i].flags & kRecValid) && p.rec[i].area > 0.0;
+  return has_contact(p, i);
 }
 
 __global__ void k_net_count(Pipeline p, uint32_t* ecnt, uint32_t* vcnt) {

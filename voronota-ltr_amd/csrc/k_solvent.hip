@@ -26,19 +26,17 @@ __device__ __forceinline__ bool exposed(const Pipeline& p, int64_t a) {
   return c.stage == 2 && c.sas > 0.0;
 }
 
-__device__ __forceinline__ bool valid_rec(const PairRec& r) { return (r.flags & kRecValid) && r.area > 0.0; }
-
 // Periodic box: per canonical ball, the first ends of valid pairs whose second
 // end is one of its images (count, then fill after a scan of the counts).
 __global__ void k_solv_img_count(Pipeline p, uint32_t* icnt) {
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (t >= p.n_pairs || p.pair_b[t] < p.n || !valid_rec(p.rec[t])) return;
+  if (t >= p.n_pairs || p.pair_b[t] < p.n || !has_contact(p, t)) return;
   atomicAdd(&icnt[canon_id(p, p.pair_b[t])], 1u);
 }
 
 __global__ void k_solv_img_fill(Pipeline p, const uint32_t* ioff, uint32_t* cur, int32_t* img) {
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (t >= p.n_pairs || p.pair_b[t] < p.n || !valid_rec(p.rec[t])) return;
+  if (t >= p.n_pairs || p.pair_b[t] < p.n || !has_contact(p, t)) return;
   const int64_t b = canon_id(p, p.pair_b[t]);
   img[ioff[b] + atomicAdd(&cur[b], 1u)] = p.pair_a[t];
 }
@@ -58,13 +56,13 @@ __device__ bool sample_ok(const Pipeline& p, const uint32_t* ioff, const int32_t
                           double d_atom) {
   const int64_t own0 = p.pair_cnt[a], own1 = p.pair_cnt[a + 1];
   for (int64_t t = own0; t < own1; ++t) {
-    if (valid_rec(p.rec[t]) && closer(p, canon_id(p, p.pair_b[t]), pt, d_atom)) return false;
+    if (has_contact(p, t) && closer(p, canon_id(p, p.pair_b[t]), pt, d_atom)) return false;
   }
   const int nin = min(static_cast<int>(p.in_cnt[a]), p.nb_cap);
   const int32_t* inc = p.incoming + a * static_cast<int64_t>(p.nb_cap);
   for (int k = 0; k < nin; ++k) {
     const int32_t t = inc[k];
-    if (valid_rec(p.rec[t]) && closer(p, p.pair_a[t], pt, d_atom)) return false;
+    if (has_contact(p, t) && closer(p, p.pair_a[t], pt, d_atom)) return false;
   }
   if (ioff != nullptr)
     for (uint32_t k = ioff[a]; k < ioff[a + 1]; ++k)

@@ -270,6 +270,8 @@ struct RunExtras {
   const PairRec* prev_rec = nullptr;
   const ContactTopo* prev_topo = nullptr;
   const uint32_t* prev_topo_slot = nullptr;
+  // every pair's record written (updateable runs)
+  bool records_all = false;
   // called with the report block once the neighbor lists are complete; false
   // ends the run there (run_pipeline returns kRunAborted)
   std::function<bool(const uint32_t*)> front_check;
@@ -402,7 +404,7 @@ int checked_pairs(const uint32_t* host_words, int64_t* P) {
 int enqueue_back(vltr_context* ctx, Pipeline& p, const RunBufs& b, const RunExtras* extras, bool batched, bool events,
                  uint32_t* host_words, hipStream_t s) {
   const int64_t n = p.n;
-  if (p.mode != 0) p.pair_contact = nullptr;
+  p.records_all = (extras != nullptr && extras->records_all) ? 1 : 0;
   VLTR_HIP(launch_pairs(p, s));
   if (extras && extras->pre_clip) VLTR_HIP(extras->pre_clip(p, s));
   if (extras && extras->dirty) {
@@ -1940,6 +1942,7 @@ int upd_run(Upd* u, const RunOpts& o) {
     if (o.refresh) VLTR_HIP(launch_mask_clear(u->excl.as<uint8_t>(), am, static_cast<int64_t>(n), s));
   }
   RunExtras ex;
+  ex.records_all = true;  // the next step copies the records of the pairs it does not clip
   ex.user_excl = o.full ? nullptr : u->excl.as<uint8_t>();
   ex.front_check = o.front_check;
   CellRec* state = u->cells_state.as<CellRec>();

@@ -126,9 +126,14 @@ struct Pipeline {
   int32_t* pair_b;       // pair_cap
   PairRec* rec;          // pair_cap
   // pair_cap (rounded up to 4) or null: 1 where the pair has a contact,
-  // written by the clip kernels as they write or keep the records (full
-  // runs); k_cells then loads only those own records
+  // written by the clip kernels as they write or keep the records; the
+  // readers ask it which pairs have a contact (has_contact), k_cells loads
+  // only those own records
   uint8_t* pair_contact;
+  // 1: every pair's record is written (updateable steps: the next step
+  // copies the records of the pairs it does not clip); 0: with contact bytes,
+  // only the records of pairs with a contact
+  int32_t records_all;
   CellRec* cell;         // n
   uint32_t* status;      // 1 word; the first of the run's report block (kReport* words)
   int32_t* overflow_list;  // pair_cap (pairs needing the slow kernel)
@@ -214,6 +219,14 @@ __device__ __forceinline__ bool same_group(const Pipeline& p, int64_t a, int64_t
 // tessellation.rs:298-319 (and updateable.rs:516-561 without involvement):
 // (a, c) is a candidate pair when not in the same group and a < c, or c is a
 // periodic image.
+// Whether pair t has a contact: its contact byte where the run keeps them,
+// else its record.
+__device__ __forceinline__ bool has_contact(const Pipeline& p, int64_t t) {
+  if (p.pair_contact != nullptr) return p.pair_contact[t] != 0;
+  const PairRec& r = p.rec[t];
+  return (r.flags & kRecValid) && r.area > 0.0;
+}
+
 // p.n and its reciprocal (canon_id).
 inline void set_balls(Pipeline& p, int64_t n) {
   p.n = n;

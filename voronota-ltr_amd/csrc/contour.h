@@ -385,13 +385,22 @@ VLTR_HD bool quotient_reaches(double l, double t, double r) {
   return l / sqrt(t) >= r;
 }
 
+// intersects(g.b, c) and contains(c, g.b) share the squared distance of the
+// two centres (norm2(c - b) and norm2(b - c) are equal bit for bit), and the
+// halfspace side of the circle centre is the sign of the distance l is taken
+// from.
 VLTR_HD int classify(const PairGeom& g, const Sph& c, const Plane& pl, bool c_contains_a) {
-  if (!intersects(g.ic, c) || !intersects(g.b, c)) return kSkip;
-  if (c_contains_a || contains(c, g.b)) return kKill;
+  if (!intersects(g.ic, c)) return kSkip;
+  const double dbc = norm2(c.c - g.b.c);
+  const double sbc = g.b.r + c.r;
+  if (!f_lt(dbc, sbc * sbc)) return kSkip;
+  const double dr = c.r - g.b.r;
+  if (c_contains_a || (f_ge(c.r, g.b.r) && f_le(dbc, dr * dr))) return kKill;
   const double cosv = dot(g.ic_dir, pl.pdir);
-  if (fabs(cosv) >= 1.0) return halfspace(pl.pc, pl.pn, g.ic.c) > 0 ? kKill : kSkip;
-  const double l = fabs(dot(pl.pn, g.ic.c - pl.pc));
-  if (quotient_reaches(l, fma(cosv, -cosv, 1.0), g.ic.r)) return halfspace(pl.pc, pl.pn, g.ic.c) >= 0 ? kKill : kSkip;
+  const double sd = dot(pl.pn, g.ic.c - pl.pc);
+  const int side = f_gt(sd, 0.0) ? 1 : (f_lt(sd, 0.0) ? -1 : 0);  // halfspace(pl.pc, pl.pn, g.ic.c)
+  if (fabs(cosv) >= 1.0) return side > 0 ? kKill : kSkip;
+  if (quotient_reaches(fabs(sd), fma(cosv, -cosv, 1.0), g.ic.r)) return side >= 0 ? kKill : kSkip;
   return kCut;
 }
 

@@ -127,32 +127,25 @@ VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
     return kNone;
   }
   if (n < 3) return kContact;
-  int start = ctz32(M), end = 31 - clz32(M);
-  if (start == 0 && end == n - 1) {
-    // marked run wraps around point 0: it ends before the first unmarked
-    // point and starts after the last one
-    const uint32_t full = (1u << n) - 1u;
-    end = ctz32(~M) - 1;
-    start = 32 - clz32(~M & full);
-  }
-  int src = j, nn = n;
-  if (start == end) {
-    if (n + 1 > GS) return kOverflow;
-    src = j > start ? j - 1 : j;
-    nn = n + 1;
-    end = start + 1;
-  } else if (start < end) {
-    const int d = end - (start + 1);
-    src = j >= start + 1 ? j + d : j;
-    nn = n - d;
-    end = start + 1;
-  } else {
-    nn = start + 1 - end;
-    src = j + end;
-    start = nn - 1;
-    end = 0;
-  }
-  if (nn < 3) return kOverflow;
+  // The marked run is [start, end]; when it wraps around point 0 it ends
+  // before the first unmarked point and starts after the last one. The run
+  // is replaced by two points, start and start + 1 (one point out: one more
+  // point, the others shift up; several: the points after the run shift
+  // down), or, for a wrapped run, the unmarked points move to the front and
+  // the two new points are the last and the first. Every lane computes the
+  // group's indices, so the cases are selects rather than branches.
+  const uint32_t full = (1u << n) - 1u;
+  const int first = ctz32(M), last = 31 - clz32(M);
+  const bool wraps = first == 0 && last == n - 1;
+  int start = wraps ? 32 - clz32(~M & full) : first;
+  const int stop = wraps ? ctz32(~M) - 1 : last;
+  const bool run = !wraps;  // start <= stop
+  const int d = stop - (start + 1);  // -1 when a single point is out
+  const int nn = run ? n - d : start + 1 - stop;
+  const int src = run ? (j > start ? j + d : j) : j + stop;
+  if (nn > GS || nn < 3) return kOverflow;
+  start = run ? start : nn - 1;
+  const int end = run ? start + 1 : 0;
   c = gshfl(g, c, src & (GS - 1));
   n = nn;
   const int il = start > 0 ? start - 1 : n - 1;
@@ -161,15 +154,12 @@ VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
   // points as they were before either is written
   const GPt other = gshfl(g, c, j == start ? il : ir);
   const V3 x = plane_segment(cp.pc, cp.nn, c.p, other.p);
-  if (j == start) {
-    c.r = c.l;
-    c.l = other.r;
-    c.p = x;
-  } else if (j == end) {
-    c.l = c.r;
-    c.r = other.l;
-    c.p = x;
-  }
+  const bool is_s = j == start, is_e = j == end;
+  const int32_t nl = is_s ? other.r : (is_e ? c.r : c.l);
+  const int32_t nr = is_s ? c.l : (is_e ? other.l : c.r);
+  c.l = nl;
+  c.r = nr;
+  if (is_s || is_e) c.p = x;
   const V3 pe = gshfl3(g, c.p, end);
   if (g.ballot(j == start && !f_gt(norm2(pe - c.p), 0.0)) != 0u) {
     const int32_t re = g.shfl(c.r, end);
@@ -386,13 +376,22 @@ template <class G, class S>
 VLTR_HD double group_solid_angle(G& g, const Sph& a, const Sph& b, const Sph& ic, const S* s, int m) {
   constexpr int GS = G::kSize;
   const int j = g.lane();
-  double t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0};
+  // entries j and j + GS of the lane (registers, not an indexed array)
+  double t1a = 0.0, t2a = 0.0, t1b = 0.0, t2b = 0.0;
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
     const int t = j + h * GS;
-    if (t < m) turn_terms(a, b, ic, s, t, m, &t1[h], &t2[h]);
+    double u1 = 0.0, u2 = 0.0;
+    if (t < m) turn_terms(a, b, ic, s, t, m, &u1, &u2);
+    if (h == 0) {
+      t1a = u1;
+      t2a = u2;
+    } else {
+      t1b = u1;
+      t2b = u2;
+    }
   }
-  const double turn = group_sum2(g, t1[0], t1[1], t2[0], t2[1], m);
+  const double turn = group_sum2(g, t1a, t1b, t2a, t2b, m);
   double solid = kTau - turn;
   const V3 ic_a = ic.c - a.c;
   const V3 ic_b = ic.c - b.c;
@@ -469,33 +468,32 @@ VLTR_HD PairRec group_clip_words(G& g, S* s, const PairGeom& pg, int32_t a_id, i
   const int j = g.lane();
   GPt c{pg.ic.c, a_id, a_id};
   int n = 0;
-  bool init = false;
   double d2 = 0.0;  // squared distance of this lane's point from a's centre
   *status = kNone;
+  bool any = false;
+  for (int w = 0; w < nw; ++w) any = any || words(w) != 0ull;
+  if (any) {
+    // contact.rs:455-469: the first cutting plane lays the hexagon of
+    // circumradius 1.19 r, which that plane then cuts
+    const V3 first = (any_normal(pg.axis) * pg.ic.r) * kHexagonScale;
+    if (j == 0) {
+      c.p = pg.ic.c + first;
+    } else if (j < 6) {
+      c.p = pg.ic.c + rotate(pg.axis, hex_s[j - 1], hex_c[j - 1], first);
+    }
+    n = 6;
+  }
+  bool fresh = true;
   for (int w = 0; w < nw; ++w) {
     uint64_t mm = words(w);
     while (mm) {
       const int k = w * 64 + ctz64(mm);
       mm &= mm - 1;
-      if (init) {
-        // contact.rs:472-481: some point at least as far from a's centre as
-        // the plane point
-        if (g.ballot(j < n && d2 >= f.thr(k)) == 0u) continue;
-      }
+      // contact.rs:472-481: a later plane cuts only when some point is at
+      // least as far from a's centre as the plane point
+      if (!fresh && g.ballot(j < n && d2 >= f.thr(k)) == 0u) continue;
+      fresh = false;
       const CutPlane cp = f.plane(k);
-      if (!init) {
-        // contact.rs:455-469: hexagon of circumradius 1.19 r
-        const V3 first = (any_normal(pg.axis) * pg.ic.r) * kHexagonScale;
-        if (j == 0) {
-          c.p = pg.ic.c + first;
-        } else if (j < 6) {
-          c.p = pg.ic.c + rotate(pg.axis, hex_s[j - 1], hex_c[j - 1], first);
-        }
-        c.l = a_id;
-        c.r = a_id;
-        n = 6;
-        init = true;
-      }
       const int st = group_cut(g, c, n, cp);
       if (st != kContact) {
         *status = st;
@@ -515,19 +513,27 @@ VLTR_HD PairRec group_clip_words(G& g, S* s, const PairGeom& pg, int32_t a_id, i
   V3 sum{0.0, 0.0, 0.0};
   for (int k = 0; k < m; ++k) sum = sum + lds_p(s, k);
   const V3 bary = sum / static_cast<double>(m);
-  double tri[2] = {0.0, 0.0}, seg[2] = {0.0, 0.0};
+  double tri_a = 0.0, seg_a = 0.0, tri_b = 0.0, seg_b = 0.0;
   bool reflex = false, fail = false;
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
     const int t = j + h * GS;
     if (t < m) {
+      double tr, sg;
       bool rf, fl;
-      area_terms(s, t, m, pg.ic, bary, &tri[h], &seg[h], &rf, &fl);
+      area_terms(s, t, m, pg.ic, bary, &tr, &sg, &rf, &fl);
       reflex = reflex || rf;
       fail = fail || fl;
+      if (h == 0) {
+        tri_a = tr;
+        seg_a = sg;
+      } else {
+        tri_b = tr;
+        seg_b = sg;
+      }
     }
   }
-  const double area = group_sum2(g, tri[0], tri[1], seg[0], seg[1], m);
+  const double area = group_sum2(g, tri_a, tri_b, seg_a, seg_b, m);
   const bool central = g.ballot(reflex) != 0u || g.ballot(fail) == 0u;
   if (area <= 0.0) return empty_record();
   *status = kContact;
@@ -554,11 +560,8 @@ VLTR_HD PairRec group_clip_words(G& g, S* s, const PairGeom& pg, int32_t a_id, i
   rec.arc_length = sum_arc * pg.ic.r;
   rec.flags = kRecValid | (central ? kRecCentral : 0u);
   if (mode == 0) {
-    double solid[2];
-#pragma unroll 1
-    for (int h = 0; h < 2; ++h) solid[h] = group_solid_angle(g, h == 0 ? pg.a : pg.b, h == 0 ? pg.b : pg.a, pg.ic, s, m);
-    rec.solid_a = solid[0];
-    rec.solid_b = solid[1];
+    rec.solid_a = group_solid_angle(g, pg.a, pg.b, pg.ic, s, m);
+    rec.solid_b = group_solid_angle(g, pg.b, pg.a, pg.ic, s, m);
     const double sa = rec.solid_a < 0.0 ? -1.0 : 1.0;
     const double sb = rec.solid_b < 0.0 ? -1.0 : 1.0;
     rec.pyr_a = norm(pg.ic.c - pg.a.c) * area / 3.0 * sa;

@@ -374,34 +374,50 @@ enum NbClass : int { kSkip = 0, kCut = 1, kKill = 2 };
 // 2^-23) and one product (2^-24) put it within 5e-7 of the exact quotient
 // (relative), so outside a 1e-5 margin around r its answer is the exact one.
 // Near r, and for l or t outside [1e-30, 1e15], the exact quotient decides.
-VLTR_HD bool quotient_reaches(double l, double t, double r) {
+// `need` false skips the test (returns false): lanes whose answer is not
+// used never take the exact path.
+VLTR_HD bool quotient_reaches(double l, double t, double r, bool need = true) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  if (t >= 1e-30 && l >= 1e-30 && l <= 1e15) {
-    const double xa = static_cast<double>(static_cast<float>(l) * __builtin_amdgcn_rsqf(static_cast<float>(t)));
-    if (xa > r * (1.0 + 1e-5)) return true;
-    if (xa < r * (1.0 - 1e-5)) return false;
-  }
+  const bool fast = t >= 1e-30 && l >= 1e-30 && l <= 1e15;
+  const double xa = static_cast<double>(static_cast<float>(l) * __builtin_amdgcn_rsqf(static_cast<float>(t)));
+  const bool above = xa > r * (1.0 + 1e-5), below = xa < r * (1.0 - 1e-5);
+  bool reach = fast & above;
+  if (need & !(fast & (above | below))) reach = l / sqrt(t) >= r;
+  return reach;
+#else
+  return need && l / sqrt(t) >= r;
 #endif
-  return l / sqrt(t) >= r;
 }
 
-// intersects(g.b, c) and contains(c, g.b) share the squared distance of the
-// two centres (norm2(c - b) and norm2(b - c) are equal bit for bit), and the
-// halfspace side of the circle centre is the sign of the distance l is taken
-// from.
+// What neighbor c does to the contact of a pair (contact.rs:224-276), in
+// the reference's order of tests: no intersection with the circle or with b
+// skips, c containing a or b kills, and otherwise the circle centre's side
+// of the plane decides when the plane is parallel or misses the circle;
+// the rest cut. Every predicate is computed and the answer selected, so a
+// wavefront classifying one neighbor per lane runs no branches (only the
+// rare exact quotient). intersects(g.b, c) and contains(c, g.b) share the
+// squared distance of the two centres (norm2(c - b) and norm2(b - c) are
+// equal bit for bit), and the side of the circle centre is the sign of the
+// distance l is taken from.
 VLTR_HD int classify(const PairGeom& g, const Sph& c, const Plane& pl, bool c_contains_a) {
-  if (!intersects(g.ic, c)) return kSkip;
+  const bool meets_ic = intersects(g.ic, c);
   const double dbc = norm2(c.c - g.b.c);
   const double sbc = g.b.r + c.r;
-  if (!f_lt(dbc, sbc * sbc)) return kSkip;
+  const bool meets_b = f_lt(dbc, sbc * sbc);
   const double dr = c.r - g.b.r;
-  if (c_contains_a || (f_ge(c.r, g.b.r) && f_le(dbc, dr * dr))) return kKill;
+  const bool kill = c_contains_a | (f_ge(c.r, g.b.r) & f_le(dbc, dr * dr));
   const double cosv = dot(g.ic_dir, pl.pdir);
   const double sd = dot(pl.pn, g.ic.c - pl.pc);
-  const int side = f_gt(sd, 0.0) ? 1 : (f_lt(sd, 0.0) ? -1 : 0);  // halfspace(pl.pc, pl.pn, g.ic.c)
-  if (fabs(cosv) >= 1.0) return side > 0 ? kKill : kSkip;
-  if (quotient_reaches(fabs(sd), fma(cosv, -cosv, 1.0), g.ic.r)) return side >= 0 ? kKill : kSkip;
-  return kCut;
+  const bool pos = f_gt(sd, 0.0);       // halfspace(pl.pc, pl.pn, g.ic.c) > 0
+  const bool nonneg = !f_lt(sd, 0.0);   // halfspace(...) >= 0
+  const bool par = fabs(cosv) >= 1.0;
+  const bool open = meets_ic & meets_b & !kill & !par;
+  const bool reach = quotient_reaches(fabs(sd), fma(cosv, -cosv, 1.0), g.ic.r, open);
+  int cl = kCut;
+  cl = reach ? (nonneg ? kKill : kSkip) : cl;
+  cl = par ? (pos ? kKill : kSkip) : cl;
+  cl = kill ? kKill : cl;
+  return (meets_ic & meets_b) ? cl : kSkip;
 }
 
 // One cutting neighbor (contact.rs:278-297): the first one lays the hexagon

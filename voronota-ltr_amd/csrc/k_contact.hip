@@ -404,27 +404,38 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
   }
   wave_sync();
   const uint64_t ok_mask = __ballot(gk_ok);
-  int n_clip = 0, n_uncut = 0, jp = 0;
-  for (int k = 0; k < cnt; ++k) {
+  // The ball's pairs in list order (tessellation.rs:298-319): slot k holds a
+  // pair when its neighbor qualifies; the pair's offset jj within a's pairs
+  // is the number of qualifying slots before k. Each lane tests its own
+  // neighbor, so the loops below walk wavefront-uniform masks.
+  bool qualifies = false;
+  if (lane < cnt) qualifies = pair_qualifies(p, a, c0);
+  const uint64_t qual = __ballot(qualifies);
+  const int jj_lane = __popcll(qual & lanes_below(lane));
+  const bool mine = qualifies && (wpb == 1 || jj_lane % wpb == wsub);
+  const uint64_t kept = incremental ? __ballot(mine && !is_dirty(p, c0)) : 0ull;
+  const uint64_t todo = __ballot(mine) & ~kept;
+  // pairs of an updateable step whose two balls are not both affected keep
+  // their previous records
+  for (uint64_t r = kept; r != 0ull; r &= r - 1) {
+    const int k = __builtin_ctzll(r);
+    const int jj = __popcll(qual & lanes_below(k));
     const int32_t b = __builtin_amdgcn_readlane(c0, k);
-    if (!pair_qualifies(p, a, b)) continue;
-    const int jj = jp++;
-    if (wpb > 1 && jj % wpb != wsub) continue;
-    if (incremental && !is_dirty(p, b)) {
-      int64_t from;
-      if (on <= kWave) {
-        const uint64_t hit = __ballot(prev_b == b);
-        from = hit != 0ull ? ob + __builtin_ctzll(hit) : -1;
-      } else {
-        from = prev_find_wave(p, ob, on, b, lane);
-      }
-      keep_pair_wave<kOut>(p, from, base + jj, b, lane);
-      continue;
+    int64_t from;
+    if (on <= kWave) {
+      const uint64_t hit = __ballot(prev_b == b);
+      from = hit != 0ull ? ob + __builtin_ctzll(hit) : -1;
+    } else {
+      from = prev_find_wave(p, ob, on, b, lane);
     }
-    if (!((ok_mask >> k) & 1ull)) {
-      if (lane == 0) put_record(p, base + jj, empty_record());
-      continue;
-    }
+    keep_pair_wave<kOut>(p, from, base + jj, b, lane);
+  }
+  // pairs that cannot have a contact (no intersection circle)
+  if (((todo & ~ok_mask) >> lane) & 1ull) put_record(p, base + jj_lane, empty_record());
+  int n_clip = 0, n_uncut = 0;
+  for (uint64_t r = todo & ok_mask; r != 0ull; r &= r - 1) {
+    const int k = __builtin_ctzll(r);
+    const int jj = __popcll(qual & lanes_below(k));
     // classify reads the circle, its direction from a, and b
     PairGeom g;
     g.ic = Sph{V3{pg[0 * kWave + k], pg[1 * kWave + k], pg[2 * kWave + k]}, pg[3 * kWave + k]};

@@ -590,13 +590,21 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kWavesPer
   }
 }
 
-// Pairs of balls with more than 64 neighbors (listed by k_clip): one 16-lane
-// group per pair classifies the pair's neighbors itself, 16 at a time, into a
-// cutting mask in shared memory (one bit per neighbor slot, up to kNbMax;
-// the first killing neighbor ends the pair) and clips with group_clip_words,
-// computing each cutting plane when it is needed. Contour overflows go to
-// k_cut_slow. A fixed grid walks the device-side count.
+// Pairs listed by k_clip (balls with more than 64 neighbors, and contours
+// that outgrew its 8-lane groups): one 16-lane group per pair, four groups per
+// single-wavefront block. The group classifies the pair's neighbors itself,
+// 16 at a time (the first killing neighbor ends the pair), and keeps each
+// cutting neighbor's plane (point, normal, threshold, id) in its shared
+// memory, compacted in list order, so the clip walks a dense mask over that
+// list without touching global memory. Pairs with more than kWidePlanes
+// cutting neighbors classify them again into a cutting mask (one bit per
+// neighbor slot, up to kNbMax, in the same shared memory) and compute each
+// plane when it is needed.
+// Contours beyond 16 points go to k_cut_slow. A fixed grid walks the
+// device-side count.
 constexpr int kWideBlocks = 2048;
+constexpr int kWideGroups = kWave / kGroup;  // 16-lane groups per block
+constexpr int kWidePlanes = 48;              // cutting planes a group keeps
 constexpr int kWideWords = kNbMax / 64;
 
 struct WideMask {
@@ -604,18 +612,30 @@ struct WideMask {
   __device__ uint64_t operator()(int k) const { return w[k]; }
 };
 
-__global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
-  __shared__ GroupLds grp[256 / kGroup];
-  __shared__ uint64_t cutw[256 / kGroup][kWideWords];
+union WidePlanes {
+  struct {
+    double pcx[kWidePlanes], pcy[kWidePlanes], pcz[kWidePlanes];
+    double pnx[kWidePlanes], pny[kWidePlanes], pnz[kWidePlanes];
+    double thr[kWidePlanes];
+    int32_t cid[kWidePlanes];
+  };
+  uint64_t cw[kWideWords];  // cutting mask of a pair with more cutting neighbors
+};
+static_assert(sizeof(uint64_t) * kWideWords <= sizeof(double) * 7 * kWidePlanes, "the mask overlays the planes");
+
+__global__ __launch_bounds__(kWave) void k_clip_wide(Pipeline p) {
+  __shared__ GroupLds grp[kWideGroups];
+  __shared__ WidePlanes planes[kWideGroups];
   const int gl = threadIdx.x / kGroup;
   const int ln = threadIdx.x & (kGroup - 1);
-  WaveGroup G{(gl % (kWave / kGroup)) * kGroup, ln};
+  WaveGroup G{gl * kGroup, ln};
   GroupLds* gs = &grp[gl];
-  uint64_t* cw = cutw[gl];
+  WidePlanes& pl = planes[gl];
+  uint64_t* cw = pl.cw;
   const double* sph4 = reinterpret_cast<const double*>(p.sph);
   const int64_t n_wide = *p.wide_cnt;
-  const int64_t groups = static_cast<int64_t>(gridDim.x) * (256 / kGroup);
-  for (int64_t t = static_cast<int64_t>(blockIdx.x) * (256 / kGroup) + gl; t < n_wide; t += groups) {
+  const int64_t groups = static_cast<int64_t>(gridDim.x) * kWideGroups;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * kWideGroups + gl; t < n_wide; t += groups) {
     const int64_t pidx = p.wide_list[t];
     const int32_t a = p.pair_a[pidx], b = p.pair_b[pidx];
     const int cnt = p.nb_cnt[a];
@@ -631,34 +651,46 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       continue;
     }
     bool kill = false;
-    bool any = false;
-    uint64_t word = 0;
-    const int nw = (cnt + 63) / 64;
+    int ncut = 0;  // cutting neighbors so far (those beyond kWidePlanes are counted, not kept)
     for (int s0 = 0; s0 < cnt; s0 += kGroup) {
       const int sl = s0 + ln;
       int cl = kSkip;
-      if (sl < cnt && nbl[sl] != b) {
-        const Sph C = load_sph(sph4, nbl[sl]);
-        cl = classify(g, C, plane_of(A, C), contains(C, A));
+      Plane P{};
+      int32_t c = -1;
+      if (sl < cnt) {
+        c = nbl[sl];
+        if (c != b) {
+          const Sph C = load_sph(sph4, c);
+          P = plane_of(A, C);
+          cl = classify(g, C, P, contains(C, A));
+        }
       }
       if (G.ballot(cl == kKill) != 0u) {
         kill = true;
         break;
       }
-      const uint64_t bits = static_cast<uint64_t>(G.ballot(cl == kCut));
-      word |= bits << (s0 & 63);
-      if ((s0 & 63) + kGroup == 64 || s0 + kGroup >= cnt) {
-        if (ln == 0) cw[s0 >> 6] = word;
-        any = any || word != 0ull;
-        word = 0;
+      const uint32_t bits = G.ballot(cl == kCut);
+      if (cl == kCut) {
+        const int idx = ncut + popc32(bits & ((1u << ln) - 1u));
+        if (idx < kWidePlanes) {
+          pl.pcx[idx] = P.pc.x;
+          pl.pcy[idx] = P.pc.y;
+          pl.pcz[idx] = P.pc.z;
+          pl.pnx[idx] = P.pn.x;
+          pl.pny[idx] = P.pn.y;
+          pl.pnz[idx] = P.pn.z;
+          pl.thr[idx] = norm2(P.pc - A.c);
+          pl.cid[idx] = c;
+        }
       }
+      ncut += popc32(bits);
     }
     G.sync();
     if (kill) {
       if (ln == 0) put_record(p, pidx, empty_record());
       continue;
     }
-    if (!any) {
+    if (ncut == 0) {
       if (ln == 0) {
         const PairRec rec = record_uncut(g, p.mode);
         p.rec[pidx] = rec;
@@ -668,16 +700,47 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
       }
       continue;
     }
+    // the kept planes in list order: bit k of the dense mask = kept plane k
+    struct {
+      const WidePlanes& pl;
+      __device__ double thr(int k) const { return pl.thr[k]; }
+      __device__ CutPlane plane(int k) const {
+        const V3 pn{pl.pnx[k], pl.pny[k], pl.pnz[k]};
+        return CutPlane{V3{pl.pcx[k], pl.pcy[k], pl.pcz[k]}, pn, unit(pn), pl.cid[k]};
+      }
+    } fetch{pl};
+    // more cutting neighbors than kept: slot k of a's list, its plane computed
     struct {
       const double* sph4;
       const int32_t* nbl;
       Sph A;
       __device__ double thr(int k) const { return norm2(circle_center(A, load_sph(sph4, nbl[k])) - A.c); }
       __device__ CutPlane plane(int k) const {
-        const Plane pl = plane_of(A, load_sph(sph4, nbl[k]));
-        return CutPlane{pl.pc, pl.pn, unit(pl.pn), nbl[k]};
+        const Plane q = plane_of(A, load_sph(sph4, nbl[k]));
+        return CutPlane{q.pc, q.pn, unit(q.pn), nbl[k]};
       }
-    } fetch{sph4, nbl, A};
+    } fetch_slots{sph4, nbl, A};
+    const bool kept = ncut <= kWidePlanes;
+    if (!kept) {
+      // the cutting mask over a's slots, in place of the planes (no neighbor
+      // kills: the pass above found none)
+      uint64_t word = 0;
+      for (int s0 = 0; s0 < cnt; s0 += kGroup) {
+        const int sl = s0 + ln;
+        int cl = kSkip;
+        if (sl < cnt && nbl[sl] != b) {
+          const Sph C = load_sph(sph4, nbl[sl]);
+          cl = classify(g, C, plane_of(A, C), contains(C, A));
+        }
+        word |= static_cast<uint64_t>(G.ballot(cl == kCut)) << (s0 & 63);
+        if ((s0 & 63) + kGroup == 64 || s0 + kGroup >= cnt) {
+          if (ln == 0) cw[s0 >> 6] = word;
+          word = 0;
+        }
+      }
+      G.sync();
+    }
+    const uint64_t dense = ncut >= 64 ? ~0ull : ((1ull << ncut) - 1ull);
     int st = kNone;
     uint32_t trec = kTopoSlotFull;
     if (p.topo != nullptr) {
@@ -688,9 +751,11 @@ __global__ __launch_bounds__(256) void k_clip_wide(Pipeline p) {
     const TopoOut tout{p.topo, p.topo != nullptr ? p.topo_slot + pidx : nullptr, trec, sph4,
                        p.net != nullptr ? p.net + pidx : nullptr};
     const bool want = p.topo != nullptr || p.net != nullptr;
-    const PairRec rec = group_clip_words(G, gs, g, a, nw, WideMask{cw}, fetch, c_hex_s, c_hex_c, p.mode, &st,
-                                         want ? &tout : nullptr);
-    G.sync();
+    const PairRec rec =
+        kept ? group_clip(G, gs, g, a, dense, 0ull, fetch, c_hex_s, c_hex_c, p.mode, &st, want ? &tout : nullptr)
+             : group_clip_words(G, gs, g, a, (cnt + 63) / 64, WideMask{cw}, fetch_slots, c_hex_s, c_hex_c, p.mode,
+                                &st, want ? &tout : nullptr);
+    G.sync();  // the group's planes, mask and contour are read before the next pair writes them
     if (ln == 0) {
       if (st == kOverflow) {
         push_slow(p, pidx);
@@ -867,7 +932,7 @@ int64_t slow_lanes(int nb_cap) {
 
 hipError_t launch_fallbacks(const Pipeline& p, hipStream_t s) {
   if (p.n <= 0 || p.n_pairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_clip_wide, dim3(kWideBlocks), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(k_clip_wide, dim3(kWideBlocks), dim3(kWave), 0, s, p);
   hipLaunchKernelGGL(k_cut_slow, dim3(p.slow_ws_pairs / 64), dim3(64), 0, s, p);
   return hipGetLastError();
 }

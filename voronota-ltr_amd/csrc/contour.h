@@ -405,7 +405,7 @@ VLTR_HD int classify(const PairGeom& g, const Sph& c, const Plane& pl, bool c_co
   const double sbc = g.b.r + c.r;
   const bool meets_b = f_lt(dbc, sbc * sbc);
   const double dr = c.r - g.b.r;
-  const bool kill = c_contains_a | (f_ge(c.r, g.b.r) & f_le(dbc, dr * dr));
+  const bool kill = c_contains_a || (f_ge(c.r, g.b.r) && f_le(dbc, dr * dr));
   const double cosv = dot(g.ic_dir, pl.pdir);
   const double sd = dot(pl.pn, g.ic.c - pl.pc);
   const bool pos = f_gt(sd, 0.0);       // halfspace(pl.pc, pl.pn, g.ic.c) > 0

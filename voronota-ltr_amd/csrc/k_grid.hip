@@ -278,68 +278,96 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
     r_start[w][lane] = rs;
     r_pre[w][lane] = inc - rc;
   }
-  // the 27 cell prefixes as wavefront-uniform values: a candidate's cell is
-  // the number of cells 1..26 whose prefix it has reached (prefixes never
-  // decrease; an empty cell has its successor's prefix)
-  uint32_t pre_s[27];
-#pragma unroll
-  for (int k = 0; k < 27; ++k) pre_s[k] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(inc - rc), k));
   __syncthreads();
 
-  // candidate j's id and sphere; the loop fetches the next chunk's before it
-  // tests the current one, so the loads overlap the tests
+  // candidate j's id and sphere: its cell t is the last of the 27 whose prefix
+  // it has reached (prefixes never decrease; an empty cell has its
+  // successor's prefix), found by a binary search over the prefixes in shared
+  // memory. The loops fetch the next chunk's candidates before they test the
+  // current one, so the loads overlap the tests.
   auto fetch = [&](uint32_t j, int32_t* c, double4* C4) {
     if (j >= total) return;
     int t = 0;
 #pragma unroll
-    for (int k = 1; k < 27; ++k) t += pre_s[k] <= j ? 1 : 0;
+    for (int step = 16; step > 0; step >>= 1)
+      if (t + step < 27 && r_pre[w][t + step] <= j) t += step;
     const uint32_t pos = r_start[w][t] + (j - r_pre[w][t]);
     *c = p.sorted_id[pos];
     *C4 = p.sorted_sph[pos];
   };
+  // Pass 1: the candidates that intersect a (spheres_searcher.rs:281-290),
+  // listed in any order.
   int32_t c_next = 0;
   double4 C_next = make_double4(0.0, 0.0, 0.0, 0.0);
   fetch(lane, &c_next, &C_next);
-  int cnt = 0;
-  bool hidden = false;
+  int cnt1 = 0;
   for (uint32_t base = 0; base < total; base += kWave) {
     const uint32_t j = base + lane;
     const int32_t c = c_next;
     const double4 C4 = C_next;
     fetch(j + kWave, &c_next, &C_next);
-    bool emit = false;
-    NbEntry e{0.0, 0};
-    if (j < total) {
-      if (c != a) {
+    const bool hit = j < total && c != a && intersects(A, Sph{V3{C4.x, C4.y, C4.z}, C4.w});
+    const unsigned long long m = __ballot(hit);
+    if (hit) {
+      const int slot = cnt1 + __popcll(m & lanemask_lt());
+      if (slot < kListCap) list[w][slot].id = c;
+    }
+    cnt1 += __popcll(m);
+  }
+  __syncthreads();
+  if (!active) return;
+  bool hidden = false;
+  int cnt = 0;
+  if (cnt1 > kListCap) {
+    // more intersecting candidates than the list holds: the hidden test over
+    // all of them (spheres_searcher.rs:295-303), then k_neighbors_big lists
+    // the ball
+    fetch(lane, &c_next, &C_next);
+    for (uint32_t base = 0; base < total; base += kWave) {
+      const uint32_t j = base + lane;
+      const int32_t c = c_next;
+      const double4 C4 = C_next;
+      fetch(j + kWave, &c_next, &C_next);
+      if (j < total && c != a) {
         const Sph Cs{V3{C4.x, C4.y, C4.z}, C4.w};
-        if (intersects(A, Cs)) {
-          if (contains(Cs, A) && (!equals(Cs, A) || a > c)) {
-            hidden = true;
-          } else if (!contains(A, Cs)) {
-            emit = true;
-            e.id = c;  // its sort key follows once the list is complete
-          }
-        }
+        if (intersects(A, Cs) && contains(Cs, A) && (!equals(Cs, A) || a > c)) hidden = true;
       }
     }
-    const unsigned long long m = __ballot(emit);
-    if (emit) {
-      const int slot = cnt + __popcll(m & lanemask_lt());
-      if (slot < kListCap) list[w][slot] = e;
+    if (__ballot(hidden) == 0ull) {
+      if (lane == 0) p.nb_big[atomicAdd(p.nb_big_cnt, 1u)] = static_cast<int32_t>(a);
+      return;
     }
-    cnt += __popcll(m);
+  } else {
+    // Pass 2 over the intersecting candidates (spheres_searcher.rs:295-310):
+    // a candidate containing a (the equal one of lower index) hides a; one
+    // that a contains is skipped; the others stay in the list, compacted in
+    // place, with their sort keys.
+    for (int e0 = 0; e0 < cnt1; e0 += kWave) {
+      const int e = e0 + lane;
+      bool keep = false;
+      int32_t c = 0;
+      double key = 0.0;
+      if (e < cnt1) {
+        c = list[w][e].id;
+        const double4 C4 = p.sph[c];
+        const Sph Cs{V3{C4.x, C4.y, C4.z}, C4.w};
+        if (contains(Cs, A) && (!equals(Cs, A) || a > c)) {
+          hidden = true;
+        } else if (!contains(A, Cs)) {
+          keep = true;
+          key = circle_center_distance(A, Cs);
+        }
+      }
+      const unsigned long long m = __ballot(keep);
+      if (keep) {
+        const int slot = cnt + __popcll(m & lanemask_lt());
+        list[w][slot].id = c;
+        list[w][slot].key = key;
+      }
+      cnt += __popcll(m);
+    }
   }
-  const bool any_hidden = __ballot(hidden) != 0ull;
-  const int stored = cnt < kListCap ? cnt : kListCap;
-  __syncthreads();
-
-  if (!active) return;
-  if (!any_hidden && cnt > kListCap) {
-    // too many for this wavefront's list: k_neighbors_big lists the ball
-    if (lane == 0) p.nb_big[atomicAdd(p.nb_big_cnt, 1u)] = static_cast<int32_t>(a);
-    return;
-  }
-  if (any_hidden) {
+  if (__ballot(hidden) != 0ull) {
     if (lane == 0) {
       p.nb_cnt[a] = 0;
       p.excluded[a] = 1;
@@ -347,13 +375,7 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
     }
     return;
   }
-  // sort keys of the listed neighbors only (about one candidate in nine
-  // collides), every lane busy
-  for (int e = lane; e < stored; e += kWave) {
-    const int32_t c = list[w][e].id;
-    const double4 C4 = p.sph[c];
-    list[w][e].key = circle_center_distance(A, Sph{V3{C4.x, C4.y, C4.z}, C4.w});
-  }
+  const int stored = cnt;
   // the wavefront's list is private to it: a wavefront-level barrier
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

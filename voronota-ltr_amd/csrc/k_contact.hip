@@ -75,6 +75,16 @@ struct UnitNormals<true> {
   __device__ V3 get(int k, V3) const { return V3{x[k], y[k], z[k]}; }
 };
 
+// Restricted contour capacity of a clip group: twice the group size, except
+// 12 for 8-lane groups (contours of 9-12 points are 0.02% of the clipped
+// pairs on protein-like input; longer ones go to k_clip_wide), which keeps the
+// wavefront's shared memory under 10 KB with the unit normals (four
+// wavefronts per SIMD).
+template <int G>
+constexpr int restrict_cap() {
+  return G == 8 ? 12 : 2 * G;
+}
+
 // Shared memory of one k_clip wavefront.
 template <int G, bool NN>
 struct ClipLdsT {
@@ -87,7 +97,7 @@ struct ClipLdsT {
   // from the front (with their cutting masks), uncut pairs from the back
   int32_t item[kClipSlots];
   uint64_t mask[kClipSlots];
-  GroupLdsT<2 * G> grp[kWave / G];
+  GroupLdsT<restrict_cap<G>()> grp[kWave / G];
 };
 
 // Value of v in lane k (k uniform over the wavefront).
@@ -313,7 +323,7 @@ template <int kClipGroup, bool kKeepNormals, int kWavesPerBall, bool kOut>
 __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup, kKeepNormals>& w, const int lane,
                                           const int64_t q, const int wsub) {
   using ClipGroup = WaveGroupT<kClipGroup>;
-  using ClipGroupLds = GroupLdsT<2 * kClipGroup>;
+  using ClipGroupLds = GroupLdsT<restrict_cap<kClipGroup>()>;
   using ClipLds = ClipLdsT<kClipGroup, kKeepNormals>;
   const int wpb = p.dirty != nullptr ? 1 : kWavesPerBall;
   if (wsub >= wpb) return;
@@ -387,20 +397,19 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     w.cid[lane] = c0;
   }
   // the pair geometries, staged in the clip groups' shared memory (unused
-  // until phase 2) for uniform-address reads in the loop below
-  static_assert(sizeof(double) * 10 * kWave <= sizeof(w.grp), "pair geometry fits the clip groups' buffers");
+  // until phase 2) for uniform-address reads in the loop below; the circle's
+  // centre is the plane point already in the table (intersection_circle and
+  // circle_center take the same steps)
+  static_assert(sizeof(double) * 7 * kWave <= sizeof(w.grp), "pair geometry fits the clip groups' buffers");
   double* pg = reinterpret_cast<double*>(&w.grp[0]);
   if (lane < cnt) {
-    pg[0 * kWave + lane] = gk.ic.c.x;
-    pg[1 * kWave + lane] = gk.ic.c.y;
-    pg[2 * kWave + lane] = gk.ic.c.z;
-    pg[3 * kWave + lane] = gk.ic.r;
-    pg[4 * kWave + lane] = gk.ic_dir.x;
-    pg[5 * kWave + lane] = gk.ic_dir.y;
-    pg[6 * kWave + lane] = gk.ic_dir.z;
-    pg[7 * kWave + lane] = C.c.x;
-    pg[8 * kWave + lane] = C.c.y;
-    pg[9 * kWave + lane] = C.c.z;
+    pg[0 * kWave + lane] = gk.ic.r;
+    pg[1 * kWave + lane] = gk.ic_dir.x;
+    pg[2 * kWave + lane] = gk.ic_dir.y;
+    pg[3 * kWave + lane] = gk.ic_dir.z;
+    pg[4 * kWave + lane] = C.c.x;
+    pg[5 * kWave + lane] = C.c.y;
+    pg[6 * kWave + lane] = C.c.z;
   }
   wave_sync();
   const uint64_t ok_mask = __ballot(gk_ok);
@@ -438,9 +447,9 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     const int jj = __popcll(qual & lanes_below(k));
     // classify reads the circle, its direction from a, and b
     PairGeom g;
-    g.ic = Sph{V3{pg[0 * kWave + k], pg[1 * kWave + k], pg[2 * kWave + k]}, pg[3 * kWave + k]};
-    g.ic_dir = V3{pg[4 * kWave + k], pg[5 * kWave + k], pg[6 * kWave + k]};
-    g.b = Sph{V3{pg[7 * kWave + k], pg[8 * kWave + k], pg[9 * kWave + k]}, readlane_d(C.r, k)};
+    g.ic = Sph{V3{w.pcx[k], w.pcy[k], w.pcz[k]}, pg[0 * kWave + k]};
+    g.ic_dir = V3{pg[1 * kWave + k], pg[2 * kWave + k], pg[3 * kWave + k]};
+    g.b = Sph{V3{pg[4 * kWave + k], pg[5 * kWave + k], pg[6 * kWave + k]}, readlane_d(C.r, k)};
     const int cl = (lane < cnt && lane != k) ? classify(g, C, P, c_has_a) : kSkip;
     const bool kill = __ballot(cl == kKill) != 0ull;
     const uint64_t M = __ballot(cl == kCut);

@@ -76,13 +76,13 @@ struct UnitNormals<true> {
 };
 
 // Restricted contour capacity of a clip group: twice the group size, except
-// 12 for 8-lane groups (contours of 9-12 points are 0.02% of the clipped
+// 11 for 8-lane groups (contours of 9-11 points are 0.02% of the clipped
 // pairs on protein-like input; longer ones go to k_clip_wide), which keeps the
-// wavefront's shared memory under 10 KB with the unit normals (four
-// wavefronts per SIMD).
+// wavefront's shared memory under 10 KB with the unit normals and the circle
+// radii (four wavefronts per SIMD).
 template <int G>
 constexpr int restrict_cap() {
-  return G == 8 ? 12 : 2 * G;
+  return G == 8 ? 11 : 2 * G;
 }
 
 // Shared memory of one k_clip wavefront.
@@ -92,6 +92,10 @@ struct ClipLdsT {
   double pnx[kClipSlots], pny[kClipSlots], pnz[kClipSlots];
   UnitNormals<NN> nn;
   double thr[kClipSlots];  // squared distance from a's centre to the plane point
+  // radius of the circle of the pair (a, neighbor k): with the plane point
+  // (the circle's centre) and the plane normal (the pair's axis) the pair's
+  // geometry, read in phase 2 instead of computed again
+  double ric[kClipSlots];
   int32_t cid[kClipSlots];
   // listed pairs: (pair offset within a << 8) | slot of b; pairs to clip
   // from the front (with their cutting masks), uncut pairs from the back
@@ -314,6 +318,22 @@ __device__ __forceinline__ bool is_dirty(const Pipeline& p, int64_t id) {
   return p.dirty[p.periodic ? canon_id(p, id) : id] != 0;
 }
 
+// Geometry of the pair (a, neighbor k) from the plane table: the
+// intersection circle's centre is the radical plane's point (circle_center
+// and intersection_circle take the same steps), its radius is the table's,
+// and the pair's axis is the plane's normal (both unit(c - a)); the pair is
+// known to have a circle.
+template <class L>
+__device__ __forceinline__ PairGeom table_geom(const L& w, int k, const Sph& A, const Sph& B) {
+  PairGeom g;
+  g.a = A;
+  g.b = B;
+  g.ic = Sph{V3{w.pcx[k], w.pcy[k], w.pcz[k]}, w.ric[k]};
+  g.axis = V3{w.pnx[k], w.pny[k], w.pnz[k]};
+  g.ic_dir = V3{0.0, 0.0, 0.0};  // used by the classification only
+  return g;
+}
+
 // One clip task of a wavefront: ball q (in cell order), share wsub of its
 // qualifying pairs when kWavesPerBall waves split the ball (pair jj of the
 // ball goes to wave jj % kWavesPerBall; each of them builds the plane table,
@@ -394,22 +414,22 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     w.pnz[lane] = P.pn.z;
     if (kKeepNormals) w.nn.put(lane, unit(P.pn));
     w.thr[lane] = norm2(P.pc - A.c);
+    w.ric[lane] = gk.ic.r;
     w.cid[lane] = c0;
   }
-  // the pair geometries, staged in the clip groups' shared memory (unused
-  // until phase 2) for uniform-address reads in the loop below; the circle's
-  // centre is the plane point already in the table (intersection_circle and
-  // circle_center take the same steps)
-  static_assert(sizeof(double) * 7 * kWave <= sizeof(w.grp), "pair geometry fits the clip groups' buffers");
+  // the rest of the pair geometries, staged in the clip groups' shared
+  // memory (unused until phase 2) for uniform-address reads in the loop
+  // below; the circle is the table's plane point and radius
+  // (intersection_circle and circle_center take the same steps)
+  static_assert(sizeof(double) * 6 * kWave <= sizeof(w.grp), "pair geometry fits the clip groups' buffers");
   double* pg = reinterpret_cast<double*>(&w.grp[0]);
   if (lane < cnt) {
-    pg[0 * kWave + lane] = gk.ic.r;
-    pg[1 * kWave + lane] = gk.ic_dir.x;
-    pg[2 * kWave + lane] = gk.ic_dir.y;
-    pg[3 * kWave + lane] = gk.ic_dir.z;
-    pg[4 * kWave + lane] = C.c.x;
-    pg[5 * kWave + lane] = C.c.y;
-    pg[6 * kWave + lane] = C.c.z;
+    pg[0 * kWave + lane] = gk.ic_dir.x;
+    pg[1 * kWave + lane] = gk.ic_dir.y;
+    pg[2 * kWave + lane] = gk.ic_dir.z;
+    pg[3 * kWave + lane] = C.c.x;
+    pg[4 * kWave + lane] = C.c.y;
+    pg[5 * kWave + lane] = C.c.z;
   }
   wave_sync();
   const uint64_t ok_mask = __ballot(gk_ok);
@@ -447,9 +467,9 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     const int jj = __popcll(qual & lanes_below(k));
     // classify reads the circle, its direction from a, and b
     PairGeom g;
-    g.ic = Sph{V3{w.pcx[k], w.pcy[k], w.pcz[k]}, pg[0 * kWave + k]};
-    g.ic_dir = V3{pg[1 * kWave + k], pg[2 * kWave + k], pg[3 * kWave + k]};
-    g.b = Sph{V3{pg[4 * kWave + k], pg[5 * kWave + k], pg[6 * kWave + k]}, readlane_d(C.r, k)};
+    g.ic = Sph{V3{w.pcx[k], w.pcy[k], w.pcz[k]}, w.ric[k]};
+    g.ic_dir = V3{pg[0 * kWave + k], pg[1 * kWave + k], pg[2 * kWave + k]};
+    g.b = Sph{V3{pg[3 * kWave + k], pg[4 * kWave + k], pg[5 * kWave + k]}, readlane_d(C.r, k)};
     const int cl = (lane < cnt && lane != k) ? classify(g, C, P, c_has_a) : kSkip;
     const bool kill = __ballot(cl == kKill) != 0ull;
     const uint64_t M = __ballot(cl == kCut);
@@ -472,8 +492,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
   if (lane < n_uncut) {
     const int it = w.item[kClipSlots - 1 - lane];
     const int32_t b = w.cid[it & 0xff];
-    PairGeom g;
-    pair_setup(A, load_sph(sph4, b), &g);
+    const PairGeom g = table_geom(w, it & 0xff, A, load_sph(sph4, b));
     const PairRec rec = record_uncut(g, p.mode);
     p.rec[base + (it >> 8)] = rec;
     push_incoming(p, base + (it >> 8), b, rec);
@@ -532,8 +551,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
       const int it = w.item[idx];
       const uint64_t M = w.mask[idx];
       const int32_t b = w.cid[it & 0xff];
-      PairGeom g;
-      pair_setup(A, load_sph(sph4, b), &g);
+      const PairGeom g = table_geom(w, it & 0xff, A, load_sph(sph4, b));
       int st = kNone;
       const int64_t pidx = base + (it >> 8);
       const TopoOut tout{p.topo, p.topo != nullptr ? p.topo_slot + pidx : nullptr,

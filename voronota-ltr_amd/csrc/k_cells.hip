@@ -21,8 +21,7 @@ struct Acc {
 };
 
 constexpr int kWave = 64;
-constexpr int kCellsBlock = 64;
-constexpr int kCellsWaves = kCellsBlock / kWave;
+constexpr int kCellsBlock = 64;  // one wavefront per block
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -30,15 +29,20 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wavefront per ball b. Its contributions arrive in the reference's order:
-// first the pairs (a, b) with a < b by ascending pair index (= ascending a),
-// then b's own pairs. Lanes fetch records in parallel into a 64-entry LDS
-// window in that order; then lanes 0-5 each keep one of the six sums
-// (types.rs:589-610: area, arc length, positive and negative solid angles,
-// positive and negative pyramid volumes) and add the window's contributions
-// to it in order, skipping records without a contact. Each sum sees the
-// reference's sequence of additions, so it is bit-identical; the six run side
-// by side instead of one after another.
+// A group of kCellLanes lanes per ball b, kCellBalls balls per wavefront.
+// The ball's contributions arrive in the reference's order: first the pairs
+// (a, b) with a < b by ascending pair index (= ascending a), then b's own
+// pairs. The group's lanes fetch records in parallel into a kCellLanes-entry
+// window of shared memory in that order; then its lanes 0-5 each keep one of
+// the six sums (types.rs:589-610: area, arc length, positive and negative
+// solid angles, positive and negative pyramid volumes) and add the window's
+// contributions to it in order, skipping records without a contact. Each sum
+// sees the reference's sequence of additions, so it is bit-identical; the six
+// run side by side instead of one after another, and four balls' chains of
+// loads and additions overlap in one wavefront.
+constexpr int kCellLanes = 16;
+constexpr int kCellBalls = kWave / kCellLanes;
+
 __device__ __forceinline__ double cell_term(const PairRec& r, bool as_first, int f) {
   if (f == 0) return r.area;
   if (f == 1) return r.arc_length;
@@ -48,16 +52,25 @@ __device__ __forceinline__ double cell_term(const PairRec& r, bool as_first, int
 }
 
 __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
-  __shared__ PairRec win[kCellsWaves][kWave];
+  __shared__ PairRec win[kCellBalls][kCellLanes];
   const int lane = threadIdx.x & (kWave - 1);
-  const int w = threadIdx.x / kWave;
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x) * kCellsWaves + w;
-  if (b >= p.n || pairs_overflowed(p)) return;  // wave-private LDS, no block barrier below
+  const int g = lane / kCellLanes;          // the group (ball) of this lane
+  const int j = lane & (kCellLanes - 1);    // the lane within its group
+  const int gbase = g * kCellLanes;
+  if (pairs_overflowed(p)) return;
+  const int64_t b = xcd_remap(blockIdx.x, gridDim.x) * kCellBalls + g;
+  // whole groups leave together (wave-private shared memory, no block barrier)
+  if (b >= p.n) return;
+  PairRec* wg = win[g];
+  // the group's share of a wavefront ballot
+  auto gballot = [&](bool v) -> uint32_t {
+    return static_cast<uint32_t>((__ballot(v) >> gbase) & ((1ull << kCellLanes) - 1ull));
+  };
   const int cap = p.nb_cap;
   const int nb = p.nb_cnt[b];
   const int nin = min(static_cast<int>(p.in_cnt[b]), cap);
   const int32_t* inc = p.incoming + b * static_cast<int64_t>(cap);
-  const int f = lane < 6 ? lane : 0;  // the sum this lane keeps (lanes 0-5)
+  const int f = j < 6 ? j : 0;  // the sum this lane keeps (lanes 0-5 of the group)
   double sum = 0.0;
   bool added = false;
   // dedup cells: an own record through an image counts only when it is the
@@ -68,82 +81,81 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   // own_base >= 0: the window holds own records own_base, own_base + 1, ...
   auto add_window = [&](int m, bool as_first, int64_t own_base) {
     bool valid = false;
-    if (lane < m) {
-      const PairRec& r = win[w][lane];
+    if (j < m) {
+      const PairRec& r = wg[j];
       valid = (r.flags & kRecValid) && r.area > 0.0;
-      if (valid && own_base >= 0 && p.dd_keys != nullptr && p.pair_b[own_base + lane] >= p.n)
-        valid = dedup_first(p, T, own_base + lane);
+      if (valid && own_base >= 0 && p.dd_keys != nullptr && p.pair_b[own_base + j] >= p.n)
+        valid = dedup_first(p, T, own_base + j);
     }
-    unsigned long long vm = __ballot(valid);
-    added = added || vm != 0ull;
-    while (vm != 0ull) {
-      const int k = __builtin_ctzll(vm);
-      vm &= vm - 1ull;
-      sum = sum + cell_term(win[w][k], as_first, f);
+    uint32_t vm = gballot(valid);
+    added = added || vm != 0u;
+    while (vm != 0u) {
+      const int k = __builtin_ctz(vm);
+      vm &= vm - 1u;
+      sum = sum + cell_term(wg[k], as_first, f);
     }
   };
   const int64_t own0 = p.pair_cnt[b];
   const int nown = static_cast<int>(static_cast<int64_t>(p.pair_cnt[b + 1]) - own0);
-  // the first 64 own records are requested first: their loads overlap the
-  // incoming part
-  // (as 8-byte words: a PairRec local would live in scratch memory)
+  // the first kCellLanes own records are requested first: their loads overlap
+  // the incoming part (as 8-byte words: a PairRec local would live in scratch
+  // memory); own records without a contact, as pair_contact marks them, are
+  // not loaded: their window entries get flags 0
   constexpr int kRecWords = static_cast<int>(sizeof(PairRec) / 8);
   static_assert(sizeof(PairRec) % 8 == 0, "records are moved as 8-byte words");
-  // (own records without a contact, as pair_contact marks them, are not
-  // loaded: their window entries get flags 0)
   const uint8_t* pc = p.pair_contact;
   uint64_t own_first[kRecWords] = {};
-  if (lane < nown && (pc == nullptr || pc[own0 + lane] != 0)) {
+  if (j < nown && (pc == nullptr || pc[own0 + j] != 0)) {
 #pragma unroll
-    for (int k = 0; k < kRecWords; ++k) own_first[k] = reinterpret_cast<const uint64_t*>(p.rec + own0 + lane)[k];
+    for (int k = 0; k < kRecWords; ++k) own_first[k] = reinterpret_cast<const uint64_t*>(p.rec + own0 + j)[k];
   }
-  if (nin <= kWave) {
-    // one entry per lane, ranked by comparisons with the others' (uniform)
-    const int32_t pe = lane < nin ? inc[lane] : 0;
+  if (nin <= kCellLanes) {
+    // one entry per lane, ranked by comparisons with the group's others
+    const int32_t pe = j < nin ? inc[j] : 0;
     int rank = 0;
-    for (int k = 0; k < nin; ++k) rank += __builtin_amdgcn_readlane(pe, k) < pe ? 1 : 0;
-    if (lane < nin) win[w][rank] = p.rec[pe];
+    for (int k = 0; k < nin; ++k) rank += __shfl(pe, k, kCellLanes) < pe ? 1 : 0;
+    if (j < nin) wg[rank] = p.rec[pe];
     wave_sync();
     add_window(nin, false, -1);
     wave_sync();
   } else {
-    for (int c0 = 0; c0 < nin; c0 += kWave) {
-      for (int e = lane; e < nin; e += kWave) {
+    for (int c0 = 0; c0 < nin; c0 += kCellLanes) {
+      for (int e = j; e < nin; e += kCellLanes) {
         const int32_t pe = inc[e];
         int rank = 0;
         for (int k = 0; k < nin; ++k) rank += inc[k] < pe ? 1 : 0;
-        if (rank >= c0 && rank < c0 + kWave) win[w][rank - c0] = p.rec[pe];
+        if (rank >= c0 && rank < c0 + kCellLanes) wg[rank - c0] = p.rec[pe];
       }
       wave_sync();
-      add_window(min(kWave, nin - c0), false, -1);
+      add_window(min(kCellLanes, nin - c0), false, -1);
       wave_sync();
     }
   }
-  for (int c0 = 0; c0 < nown; c0 += kWave) {
+  for (int c0 = 0; c0 < nown; c0 += kCellLanes) {
     if (c0 == 0) {
-      if (lane < nown) {
+      if (j < nown) {
 #pragma unroll
-        for (int k = 0; k < kRecWords; ++k) reinterpret_cast<uint64_t*>(&win[w][lane])[k] = own_first[k];
+        for (int k = 0; k < kRecWords; ++k) reinterpret_cast<uint64_t*>(&wg[j])[k] = own_first[k];
       }
-    } else if (c0 + lane < nown) {
-      if (pc == nullptr || pc[own0 + c0 + lane] != 0)
-        win[w][lane] = p.rec[own0 + c0 + lane];
+    } else if (c0 + j < nown) {
+      if (pc == nullptr || pc[own0 + c0 + j] != 0)
+        wg[j] = p.rec[own0 + c0 + j];
       else
-        win[w][lane].flags = 0u;
+        wg[j].flags = 0u;
     }
     wave_sync();
-    add_window(min(kWave, nown - c0), true, own0 + c0);
+    add_window(min(kCellLanes, nown - c0), true, own0 + c0);
     wave_sync();
   }
   Acc acc;
-  acc.area = __shfl(sum, 0, kWave);
-  acc.arc = __shfl(sum, 1, kWave);
-  acc.sa_pos = __shfl(sum, 2, kWave);
-  acc.sa_neg = __shfl(sum, 3, kWave);
-  acc.pv_pos = __shfl(sum, 4, kWave);
-  acc.pv_neg = __shfl(sum, 5, kWave);
+  acc.area = __shfl(sum, 0, kCellLanes);
+  acc.arc = __shfl(sum, 1, kCellLanes);
+  acc.sa_pos = __shfl(sum, 2, kCellLanes);
+  acc.sa_neg = __shfl(sum, 3, kCellLanes);
+  acc.pv_pos = __shfl(sum, 4, kCellLanes);
+  acc.pv_neg = __shfl(sum, 5, kCellLanes);
   acc.added = added ? 1 : 0;
-  if (lane != 0) return;
+  if (j != 0) return;
 
   CellRec out;
   out.sas = 0.0;
@@ -420,7 +432,7 @@ hipError_t launch_dedup_incoming(const Pipeline& p, hipStream_t s) {
 }
 
 hipError_t launch_cells(const Pipeline& p, hipStream_t s) {
-  if (p.n > 0) hipLaunchKernelGGL(k_cells, dim3(blocks_for(p.n, kCellsWaves)), dim3(kCellsBlock), 0, s, p);
+  if (p.n > 0) hipLaunchKernelGGL(k_cells, dim3(blocks_for(p.n, kCellBalls)), dim3(kCellsBlock), 0, s, p);
   return hipGetLastError();
 }
 

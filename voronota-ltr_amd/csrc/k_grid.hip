@@ -113,10 +113,6 @@ __global__ void k_init(Pipeline p) {
   }
 }
 
-__global__ void k_copy_word(uint32_t* dst, const uint32_t* src) {
-  if (threadIdx.x == 0) *dst = *src;
-}
-
 __global__ void k_copy_strided(uint32_t* dst, const uint32_t* src, int count, int stride) {
   if (static_cast<int>(threadIdx.x) < count) dst[threadIdx.x] = src[threadIdx.x * stride];
 }
@@ -572,11 +568,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* in, 
 }
 
 // Tile sums -> tile offsets; sums[nb] = the total modulo 2^32 and
-// sums[nb + 1] = 1 when the exact total exceeds INT32_MAX. A tile sums at
+// sums[nb + 1] = 1 when the exact total exceeds INT32_MAX, also written to
+// *total_out and *beyond_out when given (a run's report words). A tile sums at
 // most 2048 counts of at most 4096, so 256 tile sums add up to at most 2^31
 // and each round's block total is exact in 32 bits; the running total is
 // kept in 64 bits (ScanTotal).
-__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, int64_t nb) {
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, int64_t nb, uint32_t* total_out,
+                                                          uint32_t* beyond_out) {
   __shared__ uint32_t sh[kScanBlock / kWave];
   ScanTotal carry;
   for (int64_t base = 0; base < nb; base += kScanBlock) {
@@ -590,7 +588,19 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, int64_
   if (threadIdx.x == 0) {
     sums[nb] = carry.lo;
     sums[nb + 1] = carry.beyond_int32();
+    if (total_out != nullptr) *total_out = carry.lo;
+    if (beyond_out != nullptr) *beyond_out = carry.beyond_int32();
   }
+}
+
+// The totals of an empty scan.
+__global__ void k_scan_empty(uint32_t* counts, uint32_t* tmp, uint32_t* total_out, uint32_t* beyond_out) {
+  if (threadIdx.x != 0) return;
+  counts[0] = 0u;
+  tmp[0] = 0u;
+  tmp[1] = 0u;
+  if (total_out != nullptr) *total_out = 0u;
+  if (beyond_out != nullptr) *beyond_out = 0u;
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t* data, int64_t n, const uint32_t* sums) {
@@ -623,15 +633,15 @@ const uint32_t* scan_beyond_word(const uint32_t* tmp, int64_t n) {
   return tmp + (n > 0 ? blocks_for(n, kScanTile) : 0) + 1;
 }
 
-hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream_t s) {
+hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream_t s, uint32_t* total_out,
+                           uint32_t* beyond_out) {
   if (n <= 0) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counts, 1, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, tmp, 2, 0u);
+    hipLaunchKernelGGL(k_scan_empty, dim3(1), dim3(64), 0, s, counts, tmp, total_out, beyond_out);
     return hipGetLastError();
   }
   const int64_t nb = blocks_for(n, kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, counts, n, tmp);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, nb);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, nb, total_out, beyond_out);
   hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, counts, n, tmp);
   return hipGetLastError();
 }
@@ -663,10 +673,6 @@ hipError_t launch_fill_words(uint32_t* a, int64_t n, uint32_t v, hipStream_t s) 
   return hipGetLastError();
 }
 
-hipError_t launch_copy_word(uint32_t* dst, const uint32_t* src, hipStream_t s) {
-  hipLaunchKernelGGL(k_copy_word, dim3(1), dim3(64), 0, s, dst, src);
-  return hipGetLastError();
-}
 
 hipError_t launch_grid(const Pipeline& p, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(p.n_buckets + 1, 256) < 4096 ? blocks_for(p.n_buckets + 1, 256) : 4096),

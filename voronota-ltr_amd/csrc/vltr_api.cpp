@@ -379,11 +379,9 @@ int enqueue_front(vltr_context* ctx, const Pipeline& p, const RunBufs& b, bool e
   VLTR_HIP(launch_scatter(p, s));
   if (events) VLTR_HIP(hipEventRecord(ctx->ev[1], s));
   VLTR_HIP(launch_neighbors(p, s));
-  VLTR_HIP(launch_scan_u32(p.pair_cnt, p.n, b.scan_tmp, s));
-  // the pair total and whether its exact value exceeds INT32_MAX, read
-  // before any pair buffer is sized from it
-  VLTR_HIP(launch_copy_word(p.status + kReportPairs, p.pair_cnt + p.n, s));
-  VLTR_HIP(launch_copy_word(p.status + kReportPairsBeyond, scan_beyond_word(b.scan_tmp, p.n), s));
+  // the pair total and whether its exact value exceeds INT32_MAX go to the
+  // report block, read before any pair buffer is sized from them
+  VLTR_HIP(launch_scan_u32(p.pair_cnt, p.n, b.scan_tmp, s, p.status + kReportPairs, p.status + kReportPairsBeyond));
   return VLTR_OK;
 }
 
@@ -437,12 +435,8 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunBufs& b, const RunExtr
     uint32_t* ecnt = ctx->net_eoff.as<uint32_t>();
     uint32_t* vcnt = ctx->net_voff.as<uint32_t>();
     VLTR_HIP(launch_net_count(p, ecnt, vcnt, s));
-    VLTR_HIP(launch_scan_u32(ecnt, p.n_pairs, b.scan_tmp, s));
-    VLTR_HIP(launch_copy_word(p.status + kReportEdges, ecnt + p.n_pairs, s));
-    VLTR_HIP(launch_copy_word(p.status + kReportEdgesBeyond, scan_beyond_word(b.scan_tmp, p.n_pairs), s));
-    VLTR_HIP(launch_scan_u32(vcnt, p.n_pairs, b.scan_tmp, s));
-    VLTR_HIP(launch_copy_word(p.status + kReportVerts, vcnt + p.n_pairs, s));
-    VLTR_HIP(launch_copy_word(p.status + kReportVertsBeyond, scan_beyond_word(b.scan_tmp, p.n_pairs), s));
+    VLTR_HIP(launch_scan_u32(ecnt, p.n_pairs, b.scan_tmp, s, p.status + kReportEdges, p.status + kReportEdgesBeyond));
+    VLTR_HIP(launch_scan_u32(vcnt, p.n_pairs, b.scan_tmp, s, p.status + kReportVerts, p.status + kReportVertsBeyond));
   }
   // periodic runs: of each canonical pair only its first record in pair order
   // (through an image in full runs, any in contacts-only runs) is reported
@@ -461,18 +455,16 @@ int enqueue_back(vltr_context* ctx, Pipeline& p, const RunBufs& b, const RunExtr
   }
   uint32_t* flags = b.flags;
   VLTR_HIP(launch_flag_contacts(p, out_keys, out_first, flags, s));
-  VLTR_HIP(launch_scan_u32(flags, p.n_pairs, b.scan_tmp, s));
+  VLTR_HIP(launch_scan_u32(flags, p.n_pairs, b.scan_tmp, s, p.status + kReportContacts));
   VLTR_HIP(launch_emit_contacts(p, flags, b.contacts, s));
-  VLTR_HIP(launch_copy_word(p.status + kReportContacts, flags + p.n_pairs, s));
   uint32_t* seg_out = b.seg_out;
   if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 0, seg_out, s));
   if (p.net != nullptr && extras->net_out != nullptr)
     VLTR_HIP(launch_emit_net(p, flags, extras->net_out->as<ContactNet>(), s));
   if (p.mode == 0) {
     VLTR_HIP(launch_flag_cells(p, flags, s));
-    VLTR_HIP(launch_scan_u32(flags, n, b.scan_tmp, s));
+    VLTR_HIP(launch_scan_u32(flags, n, b.scan_tmp, s, p.status + kReportCells));
     VLTR_HIP(launch_emit_cells(p, flags, b.cells, s));
-    VLTR_HIP(launch_copy_word(p.status + kReportCells, flags + n, s));
     if (batched) VLTR_HIP(launch_seg_offsets(p, flags, 1, seg_out + p.n_segs + 1, s));
   }
   if (p.topo != nullptr)
@@ -727,15 +719,13 @@ int device_network(vltr_context* ctx, int64_t n, bool periodic, vltr_result* r) 
   const uint32_t* perm = nullptr;
   VLTR_HIP(launch_net_sort(E, ne, 3 * L.bits, W, &perm, s));
   VLTR_HIP(launch_net_runs(E, perm, ne, flag, s));
-  VLTR_HIP(launch_scan_u32(flag, ne, scan_tmp, s));
+  VLTR_HIP(launch_scan_u32(flag, ne, scan_tmp, s, p.status + kReportEdges));
   VLTR_HIP(launch_net_emit_edges(E, perm, flag, ne, L, out_e, s));
-  VLTR_HIP(launch_copy_word(p.status + kReportEdges, flag + ne, s));
   if (nv > 0) {
     VLTR_HIP(launch_net_sort(V, nv, 4 * L.bits, W, &perm, s));
     VLTR_HIP(launch_net_runs(V, perm, nv, flag, s));
-    VLTR_HIP(launch_scan_u32(flag, nv, scan_tmp, s));
+    VLTR_HIP(launch_scan_u32(flag, nv, scan_tmp, s, p.status + kReportVerts));
     VLTR_HIP(launch_net_emit_vertices(V, perm, flag, nv, L, out_v, s));
-    VLTR_HIP(launch_copy_word(p.status + kReportVerts, flag + nv, s));
   }
   VLTR_HIP(hipMemcpyAsync(hw, p.status, kReportWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   VLTR_HIP(hipStreamSynchronize(s));

@@ -18,14 +18,19 @@ constexpr int kMaxR = 2 * kMaxC;
 // Per-pair record written by the cut kernels (types.rs:380-415 summary).
 // flags: bit0 = contact exists (area > 0), bit1 = central, bit2 = overflow
 // (fast kernel ran out of contour capacity; slow kernel must redo it).
-struct PairRec {
+// One record is one 64-byte memory line: records are written and read at
+// scattered pair indices, and a 56-byte record straddled two lines at six of
+// its eight offsets.
+struct alignas(64) PairRec {
   double area;
   double arc_length;
   double solid_a, solid_b;
   double pyr_a, pyr_b;
   uint32_t flags;
   uint32_t pad;
+  uint64_t spare;
 };
+static_assert(sizeof(PairRec) == 64, "a record is one 64-byte line");
 constexpr uint32_t kRecValid = 1u;
 constexpr uint32_t kRecCentral = 2u;
 constexpr uint32_t kRecOverflow = 4u;
@@ -574,6 +579,7 @@ VLTR_HD PairRec empty_record() {
   rec.area = rec.arc_length = rec.solid_a = rec.solid_b = rec.pyr_a = rec.pyr_b = 0.0;
   rec.flags = 0u;
   rec.pad = 0u;
+  rec.spare = 0u;
   return rec;
 }
 

@@ -217,6 +217,16 @@ int vltr_context_batch_offsets(const vltr_context* ctx, size_t n_structs, uint64
 int vltr_context_set_graphs(vltr_context* ctx, int enable);
 uint64_t vltr_context_graph_replays(const vltr_context* ctx);
 
+/* Direct runs that skipped the synchronisation on the pair total. No
+ * reference counterpart. A call without extras (plain, contacts-only,
+ * batched, host or device balls) that repeats the last such call's ball
+ * count, structure count, probe, mode, periodicity and groups enqueues the
+ * whole pipeline at once, its pair buffers sized 25% above the last pair
+ * total; a run that finds more pairs or longer neighbor lists than that is
+ * redone with the synchronisation (counted in *misses). Results are
+ * identical either way. */
+uint64_t vltr_context_speculative_runs(const vltr_context* ctx, uint64_t* misses);
+
 /* Launch shapes of the contact clip kernel and their autotuner. No reference
  * counterpart: the tuning surface of this implementation. Every shape gives
  * the same records; shapes differ in lanes per contact (8 or 16),

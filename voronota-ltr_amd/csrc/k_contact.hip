@@ -143,13 +143,25 @@ __device__ __forceinline__ void copy_topo(ContactTopo* dst, const ContactTopo* s
   for (int k = 0; k < kTopoWords; ++k) d[k] = s[k];
 }
 
+// A record as four 16-byte streaming stores (the line is written whole and
+// is not read again by this kernel, so it need not stay in the L2).
+__device__ __forceinline__ void store_record(PairRec* dst, const PairRec& r) {
+  using U4 = unsigned int __attribute__((ext_vector_type(4)));
+  auto lo = [](double x) { return static_cast<unsigned int>(static_cast<uint64_t>(__double_as_longlong(x)) & 0xffffffffu); };
+  auto hi = [](double x) { return static_cast<unsigned int>(static_cast<uint64_t>(__double_as_longlong(x)) >> 32); };
+  U4* d = reinterpret_cast<U4*>(dst);
+  __builtin_nontemporal_store(U4{lo(r.area), hi(r.area), lo(r.arc_length), hi(r.arc_length)}, d + 0);
+  __builtin_nontemporal_store(U4{lo(r.solid_a), hi(r.solid_a), lo(r.solid_b), hi(r.solid_b)}, d + 1);
+  __builtin_nontemporal_store(U4{lo(r.pyr_a), hi(r.pyr_a), lo(r.pyr_b), hi(r.pyr_b)}, d + 2);
+  __builtin_nontemporal_store(U4{r.flags, r.pad, 0u, 0u}, d + 3);
+}
 // The record of pair pidx, stored when it holds a contact or when the run
 // keeps every pair's record (updateable steps: the next step copies the
 // records of the pairs it does not clip). Where a run keeps contact bytes the
 // readers ask those which pairs have a contact (has_contact), so the records
 // of the other pairs are not written.
 __device__ __forceinline__ void put_record(const Pipeline& p, int64_t pidx, const PairRec& rec) {
-  if (p.records_all || p.pair_contact == nullptr || is_contact(rec)) p.rec[pidx] = rec;
+  if (p.records_all || p.pair_contact == nullptr || is_contact(rec)) store_record(p.rec + pidx, rec);
 }
 
 // k topology records of the region of this block's XCD: the first of their
@@ -288,7 +300,8 @@ __device__ void keep_ball(const Pipeline& p, int64_t base, int npairs, int64_t o
 
 // The pair list of ball a (tessellation.rs:298-319: its qualifying neighbors
 // in list order, from index base). The qualifying entries of 64 list slots
-// take consecutive indices by a ballot: one coalesced store per 64 slots.
+// take consecutive indices by a ballot: one coalesced store per 64 slots,
+// streamed past the L2 like the records.
 __device__ __forceinline__ void write_pairs(const Pipeline& p, int64_t a, int64_t base, int cnt, const int32_t* nbl,
                                             int lane) {
   int64_t j = 0;
@@ -304,8 +317,8 @@ __device__ __forceinline__ void write_pairs(const Pipeline& p, int64_t a, int64_
     if (q) {
       const int64_t idx = base + j + __popcll(m & lanes_below(lane));
       if (idx < p.pair_cap) {
-        p.pair_a[idx] = static_cast<int32_t>(a);
-        p.pair_b[idx] = c;
+        __builtin_nontemporal_store(static_cast<int32_t>(a), p.pair_a + idx);
+        __builtin_nontemporal_store(c, p.pair_b + idx);
       } else {
         atomicOr(p.status, kStatPairOverflow);
       }
@@ -494,7 +507,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     const int32_t b = w.cid[it & 0xff];
     const PairGeom g = table_geom(w, it & 0xff, A, load_sph(sph4, b));
     const PairRec rec = record_uncut(g, p.mode);
-    p.rec[base + (it >> 8)] = rec;
+    store_record(p.rec + base + (it >> 8), rec);
     push_incoming(p, base + (it >> 8), b, rec);
     if (kOut && p.topo != nullptr) p.topo_slot[base + (it >> 8)] = kTopoSlotUncut;
     if (kOut && p.net != nullptr) p.net[base + (it >> 8)].m = 0;

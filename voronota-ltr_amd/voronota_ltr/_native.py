@@ -86,6 +86,7 @@ SIGNATURES = {
     "vltr_compute_tessellation_batched_device": (C.c_int, [_vp, _vp, _u64p, _sz, C.c_double, C.c_int]),
     "vltr_context_set_graphs": (C.c_int, [_vp, C.c_int]),
     "vltr_context_graph_replays": (C.c_uint64, [_vp]),
+    "vltr_context_speculative_runs": (C.c_uint64, [_vp, C.POINTER(C.c_uint64)]),
     "vltr_clip_variant_count": (C.c_int, []),
     "vltr_clip_variant_name": (C.c_char_p, [C.c_int]),
     "vltr_context_set_clip_variant": (C.c_int, [_vp, C.c_int]),
@@ -343,6 +344,14 @@ def set_graphs(enable, device=0):
 
 def graph_replays(device=0):
     return int(load_library().vltr_context_graph_replays(context(device)))
+
+
+def speculative_runs(device=0):
+    """(runs that skipped the synchronisation on the pair total, of which
+    redone) on the device's context."""
+    misses = C.c_uint64(0)
+    runs = load_library().vltr_context_speculative_runs(context(device), C.byref(misses))
+    return int(runs), int(misses.value)
 
 
 def clip_variants():

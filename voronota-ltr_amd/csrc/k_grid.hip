@@ -20,6 +20,10 @@ constexpr int kWave = 64;
 constexpr int kNbBlock = 64;                  // one wavefront per block
 constexpr int kNbWaves = kNbBlock / kWave;
 constexpr int kListCap = 256;                 // per-wave LDS list (4 KB: eight wavefronts per SIMD)
+// Lists up to this length are put in order by counting ranks (about eight
+// instructions per entry per lane); longer ones of up to a wavefront by a
+// bitonic sort across the lanes (21 compare-exchange steps)
+constexpr int kRankMax = 24;
 
 __device__ __forceinline__ unsigned long long lanemask_lt() {
   const int lane = threadIdx.x & (kWave - 1);
@@ -379,13 +383,32 @@ __global__ __launch_bounds__(kNbBlock) void k_neighbors(Pipeline p) {
   const int cap = p.nb_cap;
   int32_t* out = p.nb + a * static_cast<int64_t>(cap);
   uint32_t pairs = 0;
-  for (int e = lane; e < stored; e += kWave) {
-    const NbEntry me = list[w][e];
-    int rank = 0;
-    for (int k = 0; k < stored; ++k) rank += nb_less(list[w][k], me) ? 1 : 0;
-    if (rank < cap) {
-      out[rank] = me.id;
+  if (stored > kRankMax && stored <= kWave) {
+    // a bitonic sort across the lanes (entry e in lane e, the free lanes
+    // holding entries after every real one); ids are distinct, so the order
+    // is total and equals the rank order below
+    NbEntry me{__builtin_huge_val(), INT32_MAX};
+    if (lane < stored) me = list[w][lane];
+    for (int k = 2; k <= kWave; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const NbEntry other{__shfl_xor(me.key, j, kWave), __shfl_xor(me.id, j, kWave)};
+        const bool ascending = (lane & k) == 0, lower = (lane & j) == 0;
+        if ((lower == ascending) == nb_less(other, me)) me = other;
+      }
+    }
+    if (lane < stored && lane < cap) {
+      out[lane] = me.id;
       if (pair_qualifies(p, a, me.id)) pairs += 1;
+    }
+  } else {
+    for (int e = lane; e < stored; e += kWave) {
+      const NbEntry me = list[w][e];
+      int rank = 0;
+      for (int k = 0; k < stored; ++k) rank += nb_less(list[w][k], me) ? 1 : 0;
+      if (rank < cap) {
+        out[rank] = me.id;
+        if (pair_qualifies(p, a, me.id)) pairs += 1;
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) pairs += __shfl_xor(pairs, off, kWave);

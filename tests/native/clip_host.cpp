@@ -523,24 +523,24 @@ int clip_host_records(const double* sph4, const int64_t* nb_off, const int32_t* 
 }
 
 // The pair total as the device scan forms it (launch_scan_u32 in k_grid.hip):
-// 32-bit tile sums of 2048 counts, added 256 tiles per round into a
-// ScanTotal, then checked_pair_total on the (32-bit total, beyond-int32)
-// report words. Returns that result (-1: the run is refused) and the 32-bit
-// total the offsets would hold.
+// 32-bit tile sums of 2048 counts; the last tile's block adds the tile sums
+// before it in 64 bits, then its own into a ScanTotal; checked_pair_total on
+// the (32-bit total, beyond-int32) report words. Returns that result (-1: the
+// run is refused) and the 32-bit total the offsets would hold.
 int64_t scan_pair_guard(const uint32_t* counts, int64_t n, uint32_t* total32) {
-  const int64_t tile = 2048, round = 256;
+  const int64_t tile = 2048;
   std::vector<uint32_t> tiles;
   for (int64_t b = 0; b < n; b += tile) {
     uint32_t s = 0;
     for (int64_t i = b; i < n && i < b + tile; ++i) s += counts[i];
     tiles.push_back(s);
   }
+  unsigned long long before = 0;
+  for (size_t k = 0; k + 1 < tiles.size(); ++k) before += tiles[k];
   vltr::ScanTotal carry;
-  for (size_t b = 0; b < tiles.size(); b += round) {
-    uint32_t tot = 0;
-    for (size_t k = b; k < tiles.size() && k < b + round; ++k) tot += tiles[k];
-    carry.add(tot);
-  }
+  carry.lo = static_cast<uint32_t>(before);
+  carry.full = before;
+  if (!tiles.empty()) carry.add(tiles.back());
   *total32 = carry.lo;
   return vltr::checked_pair_total(carry.lo, carry.beyond_int32());
 }

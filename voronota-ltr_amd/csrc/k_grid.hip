@@ -590,32 +590,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* in, 
   if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
-// Tile sums -> tile offsets; sums[nb] = the total modulo 2^32 and
-// sums[nb + 1] = 1 when the exact total exceeds INT32_MAX, also written to
-// *total_out and *beyond_out when given (a run's report words). A tile sums at
-// most 2048 counts of at most 4096, so 256 tile sums add up to at most 2^31
-// and each round's block total is exact in 32 bits; the running total is
-// kept in 64 bits (ScanTotal).
-__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, int64_t nb, uint32_t* total_out,
-                                                          uint32_t* beyond_out) {
-  __shared__ uint32_t sh[kScanBlock / kWave];
-  ScanTotal carry;
-  for (int64_t base = 0; base < nb; base += kScanBlock) {
-    const int64_t i = base + threadIdx.x;
-    const uint32_t v = i < nb ? sums[i] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_exclusive_scan(v, &tot, sh);
-    if (i < nb) sums[i] = carry.lo + ex;
-    carry.add(tot);
-  }
-  if (threadIdx.x == 0) {
-    sums[nb] = carry.lo;
-    sums[nb + 1] = carry.beyond_int32();
-    if (total_out != nullptr) *total_out = carry.lo;
-    if (beyond_out != nullptr) *beyond_out = carry.beyond_int32();
-  }
-}
-
 // The totals of an empty scan.
 __global__ void k_scan_empty(uint32_t* counts, uint32_t* tmp, uint32_t* total_out, uint32_t* beyond_out) {
   if (threadIdx.x != 0) return;
@@ -626,9 +600,27 @@ __global__ void k_scan_empty(uint32_t* counts, uint32_t* tmp, uint32_t* total_ou
   if (beyond_out != nullptr) *beyond_out = 0u;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t* data, int64_t n, const uint32_t* sums) {
+// Exclusive scan of tile `blockIdx.x`, offset by the sum of the tile sums
+// before it, which the block adds up itself (at most a few thousand words).
+// The last tile also writes the total modulo 2^32 to data[n] and sums[nb],
+// and 1 to sums[nb + 1] when the exact total exceeds INT32_MAX (the same two
+// words to *total_out and *beyond_out when given: a run's report words). A
+// tile sums at most 2048 counts of at most 4096, so its sum is exact in 32
+// bits; the sums of tiles are added in 64 bits (ScanTotal).
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t* data, int64_t n, uint32_t* sums, int64_t nb,
+                                                           uint32_t* total_out, uint32_t* beyond_out) {
   __shared__ uint32_t sh[kScanBlock / kWave];
-  const int64_t base = blockIdx.x * static_cast<int64_t>(kScanTile) + static_cast<int64_t>(threadIdx.x) * kScanItems;
+  __shared__ unsigned long long sh64[kScanBlock / kWave];
+  const int64_t tile = blockIdx.x;
+  const bool last = tile == static_cast<int64_t>(gridDim.x) - 1;
+  unsigned long long before = 0;
+  for (int64_t k = threadIdx.x; k < tile; k += kScanBlock) before += sums[k];
+  for (int off = kWave / 2; off > 0; off >>= 1) before += __shfl_xor(before, off, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) sh64[threadIdx.x / kWave] = before;
+  __syncthreads();
+  before = 0;
+  for (int k = 0; k < kScanBlock / kWave; ++k) before += sh64[k];
+  const int64_t base = tile * static_cast<int64_t>(kScanTile) + static_cast<int64_t>(threadIdx.x) * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
   for (int k = 0; k < kScanItems; ++k) {
@@ -637,13 +629,23 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(uint32_t* data, int64
     s += v[k];
   }
   uint32_t tot;
-  uint32_t run = sums[blockIdx.x] + block_exclusive_scan(s, &tot, sh);
+  uint32_t run = static_cast<uint32_t>(before) + block_exclusive_scan(s, &tot, sh);
   for (int k = 0; k < kScanItems; ++k) {
     const int64_t i = base + k;
     if (i < n) data[i] = run;
     run += v[k];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) data[n] = sums[gridDim.x];
+  if (last && threadIdx.x == 0) {
+    ScanTotal t;
+    t.lo = static_cast<uint32_t>(before);
+    t.full = before;
+    t.add(tot);
+    data[n] = t.lo;
+    sums[nb] = t.lo;
+    sums[nb + 1] = t.beyond_int32();
+    if (total_out != nullptr) *total_out = t.lo;
+    if (beyond_out != nullptr) *beyond_out = t.beyond_int32();
+  }
 }
 
 inline int64_t blocks_for(int64_t n, int64_t per) { return (n + per - 1) / per; }
@@ -664,8 +666,7 @@ hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream
   }
   const int64_t nb = blocks_for(n, kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, counts, n, tmp);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, nb, total_out, beyond_out);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, counts, n, tmp);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, counts, n, tmp, nb, total_out, beyond_out);
   return hipGetLastError();
 }
 

@@ -110,11 +110,17 @@ VLTR_HD GPt gshfl(G& g, const GPt& v, int src) {
 
 // One cut of the contour by a neighbor's radical plane (contact.rs:484-627;
 // one-lane form: mark_and_cut).
+// The point's offset from the plane point serves both the side test (along
+// pn: halfspace(pc, pn, p) >= 0) and the plane-segment intersections (along
+// the unit normal nn): each point's distance along nn is computed once, by
+// its lane, and travels with the point (plane_segment_d).
 template <class G>
 VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
   constexpr int GS = G::kSize;
   const int j = g.lane();
-  const bool o = j < n && halfspace(cp.pc, cp.pn, c.p) >= 0;
+  const V3 v = c.p - cp.pc;
+  const bool o = j < n && !f_lt(dot(cp.pn, v), 0.0);
+  double dn = dot(cp.nn, v);
   if (o) {
     c.l = cp.cid;
     c.r = cp.cid;
@@ -147,13 +153,15 @@ VLTR_HD int group_cut(G& g, GPt& c, int& n, const CutPlane& cp) {
   start = run ? start : nn - 1;
   const int end = run ? start + 1 : 0;
   c = gshfl(g, c, src & (GS - 1));
+  dn = g.shfl(dn, src & (GS - 1));
   n = nn;
   const int il = start > 0 ? start - 1 : n - 1;
   const int ir = end + 1 < n ? end + 1 : 0;
   // with n >= 3, il != end and ir != start: both intersections use the
   // points as they were before either is written
-  const GPt other = gshfl(g, c, j == start ? il : ir);
-  const V3 x = plane_segment(cp.pc, cp.nn, c.p, other.p);
+  const int os = j == start ? il : ir;
+  const GPt other = gshfl(g, c, os);
+  const V3 x = plane_segment_d(c.p, other.p, dn, g.shfl(dn, os));
   const bool is_s = j == start, is_e = j == end;
   const int32_t nl = is_s ? other.r : (is_e ? c.r : c.l);
   const int32_t nr = is_s ? c.l : (is_e ? other.l : c.r);

@@ -117,13 +117,14 @@ VLTR_HD int halfspace(V3 pp, V3 n, V3 x) {
   return 0;
 }
 // geometry.rs:134-148; `nn` is unit(n) (the reference re-normalizes here).
-VLTR_HD V3 plane_segment(V3 pp, V3 nn, V3 a, V3 b) {
-  const double da = dot(nn, a - pp);
-  const double db = dot(nn, b - pp);
+// plane_segment_d takes the two signed distances da = dot(nn, a - pp) and
+// db = dot(nn, b - pp) already computed.
+VLTR_HD V3 plane_segment_d(V3 a, V3 b, double da, double db) {
   if (fabs(da - db) < kEps) return a;
   const double t = da / (da - db);
   return a + (b - a) * t;
 }
+VLTR_HD V3 plane_segment(V3 pp, V3 nn, V3 a, V3 b) { return plane_segment_d(a, b, dot(nn, a - pp), dot(nn, b - pp)); }
 // geometry.rs:152-154
 VLTR_HD double triangle_area(V3 a, V3 b, V3 c) { return norm(cross(b - a, c - a)) / 2.0; }
 // Rust f64::clamp(-1, 1): NaN passes through.

@@ -58,9 +58,14 @@ __global__ __launch_bounds__(kCellsBlock) void k_cells(Pipeline p) {
   const int j = lane & (kCellLanes - 1);    // the lane within its group
   const int gbase = g * kCellLanes;
   if (pairs_overflowed(p)) return;
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x) * kCellBalls + g;
+  // one structure: balls in cell order, like the neighbor and clip kernels (a
+  // ball's incoming records are those of its neighbors, whose own records the
+  // same XCD reads at about the same time); batches and periodic runs: index
+  // order (each XCD's range then holds whole structures)
+  const int64_t q = xcd_remap(blockIdx.x, gridDim.x) * kCellBalls + g;
   // whole groups leave together (wave-private shared memory, no block barrier)
-  if (b >= p.n) return;
+  if (q >= p.n) return;
+  const int64_t b = (p.periodic || p.n_segs > 1) ? q : static_cast<int64_t>(p.sorted_id[q]);
   PairRec* wg = win[g];
   // the group's share of a wavefront ballot
   auto gballot = [&](bool v) -> uint32_t {

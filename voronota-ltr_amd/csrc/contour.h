@@ -366,6 +366,37 @@ VLTR_HD Plane plane_of(const Sph& a, const Sph& c) {
   return p;
 }
 
+// plane_of(a, c), pair_setup(a, c, g) and contains(c, a) from one evaluation
+// of each quantity: the circle's centre is the plane point, its axis the
+// plane normal and its direction from a the plane direction, and the squared
+// centre distance serves the intersection and containment tests (the same
+// expressions, so the same values). Returns pair_setup's result; g's fields
+// are set in any case. *pc_dist2 = norm2(pl->pc - a.c), the plane point's
+// squared distance from a's centre, which the unit direction also needs.
+VLTR_HD bool plane_pair_setup(const Sph& a, const Sph& c, Plane* pl, PairGeom* g, bool* c_contains_a,
+                              double* pc_dist2) {
+  const V3 cv = c.c - a.c;
+  const double cm2 = norm2(cv);
+  const double cm = sqrt(cm2);
+  const bool apart = !(cm < kEps);
+  const double cg = circle_cos(a, c, cm);
+  pl->pc = apart ? a.c + cv * (a.r * cg / cm) : a.c;
+  pl->pn = cv / cm;
+  const V3 pv = pl->pc - a.c;
+  *pc_dist2 = norm2(pv);
+  pl->pdir = pv / sqrt(*pc_dist2);
+  g->a = a;
+  g->b = c;
+  g->ic = Sph{pl->pc, apart ? a.r * sqrt(fmax(1.0 - cg * cg, 0.0)) : 0.0};
+  g->axis = pl->pn;
+  g->ic_dir = pl->pdir;
+  const double s = a.r + c.r, dac = a.r - c.r, dca = c.r - a.r;
+  const bool meets = f_lt(cm2, s * s);
+  const bool a_has_c = f_ge(a.r, c.r) && f_le(cm2, dac * dac);
+  *c_contains_a = f_ge(c.r, a.r) && f_le(cm2, dca * dca);
+  return meets && !a_has_c && !*c_contains_a && g->ic.r > 0.0;
+}
+
 // What neighbor c does to the contact of a pair (contact.rs:224-276). The
 // answer depends on (a, b, c) only, never on the contour built so far: the
 // pair has no contact as soon as one neighbor kills it, and otherwise its

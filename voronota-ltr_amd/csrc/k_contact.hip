@@ -416,9 +416,8 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
   if (lane < cnt) {
     c0 = nbl[lane];
     C = load_sph(sph4, c0);
-    P = plane_of(A, C);
-    c_has_a = contains(C, A);
-    gk_ok = pair_setup(A, C, &gk);
+    double thr = 0.0;
+    gk_ok = plane_pair_setup(A, C, &P, &gk, &c_has_a, &thr);
     w.pcx[lane] = P.pc.x;
     w.pcy[lane] = P.pc.y;
     w.pcz[lane] = P.pc.z;
@@ -426,7 +425,7 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     w.pny[lane] = P.pn.y;
     w.pnz[lane] = P.pn.z;
     if (kKeepNormals) w.nn.put(lane, unit(P.pn));
-    w.thr[lane] = norm2(P.pc - A.c);
+    w.thr[lane] = thr;
     w.ric[lane] = gk.ic.r;
     w.cid[lane] = c0;
   }

@@ -654,10 +654,6 @@ inline int64_t blocks_for(int64_t n, int64_t per) { return (n + per - 1) / per; 
 
 int64_t scan_tmp_words(int64_t n) { return blocks_for(n, kScanTile) + 2; }
 
-const uint32_t* scan_beyond_word(const uint32_t* tmp, int64_t n) {
-  return tmp + (n > 0 ? blocks_for(n, kScanTile) : 0) + 1;
-}
-
 hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream_t s, uint32_t* total_out,
                            uint32_t* beyond_out) {
   if (n <= 0) {

@@ -260,14 +260,12 @@ hipError_t launch_grid(const Pipeline& p, hipStream_t s);
 hipError_t launch_scatter(const Pipeline& p, hipStream_t s);
 hipError_t launch_neighbors(const Pipeline& p, hipStream_t s);
 hipError_t launch_pairs(const Pipeline& p, hipStream_t s);
-// Exclusive scan of counts[0..n) in place into counts[0..n], counts[n] = total;
-// the total and the beyond-INT32_MAX word (scan_beyond_word) also go to
-// *total_out and *beyond_out when given.
+// Exclusive scan of counts[0..n) in place into counts[0..n], counts[n] = total
+// (modulo 2^32); the total and a word that is 1 when the exact total exceeds
+// INT32_MAX (ScanTotal), else 0, also go to *total_out and *beyond_out when
+// given.
 hipError_t launch_scan_u32(uint32_t* counts, int64_t n, uint32_t* tmp, hipStream_t s, uint32_t* total_out = nullptr,
                            uint32_t* beyond_out = nullptr);
-// The word of tmp where launch_scan_u32 over n counts leaves 1 when the exact
-// total exceeds INT32_MAX (ScanTotal), else 0.
-const uint32_t* scan_beyond_word(const uint32_t* tmp, int64_t n);
 // a[0 .. n) = v.
 hipError_t launch_fill_words(uint32_t* a, int64_t n, uint32_t v, hipStream_t s);
 // dst[k] = src[k * stride], k < count <= 64.

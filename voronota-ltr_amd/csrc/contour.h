@@ -337,8 +337,9 @@ VLTR_HD double contour_area(const A& ct, int m, const Sph& ic, V3* bary) {
 // Geometry of one candidate pair (contact.rs:202-218).
 struct PairGeom {
   Sph a, b, ic;
-  V3 axis;    // unit a -> b
-  V3 ic_dir;  // unit a -> intersection-circle centre
+  V3 axis;       // unit a -> b
+  V3 ic_dir;     // unit a -> intersection-circle centre
+  V3 axis_unit;  // unit(axis), the hexagon's rotation axis (rotate normalizes again)
 };
 
 // contact.rs:202-218: false when the pair cannot have a contact.
@@ -348,6 +349,7 @@ VLTR_HD bool pair_setup(const Sph& a, const Sph& b, PairGeom* g) {
   g->b = b;
   g->ic = intersection_circle(a, b);
   g->axis = unit(b.c - a.c);
+  g->axis_unit = unit(g->axis);
   if (g->ic.r <= 0.0) return false;
   g->ic_dir = unit(g->ic.c - a.c);
   return true;
@@ -389,6 +391,7 @@ VLTR_HD bool plane_pair_setup(const Sph& a, const Sph& c, Plane* pl, PairGeom* g
   g->b = c;
   g->ic = Sph{pl->pc, apart ? a.r * sqrt(fmax(1.0 - cg * cg, 0.0)) : 0.0};
   g->axis = pl->pn;
+  g->axis_unit = unit(pl->pn);
   g->ic_dir = pl->pdir;
   const double s = a.r + c.r, dac = a.r - c.r, dca = c.r - a.r;
   const bool meets = f_lt(cm2, s * s);
@@ -466,7 +469,7 @@ VLTR_HD int apply_cut(S& ct, int& n, bool& initialized, const PairGeom& g, V3 pc
   if (!initialized) {
     const V3 first = (any_normal(g.axis) * g.ic.r) * kHexagonScale;
     ct.set(0, g.ic.c + first, a_id, a_id);
-    for (int h = 0; h < 5; ++h) ct.set(h + 1, g.ic.c + rotate(g.axis, hex_s[h], hex_c[h], first), a_id, a_id);
+    for (int h = 0; h < 5; ++h) ct.set(h + 1, g.ic.c + rotate_unit(g.axis, g.axis_unit, hex_s[h], hex_c[h], first), a_id, a_id);
     n = 6;
     initialized = true;
   } else if (!cuttable(ct, n, g.a.c, pc)) {

@@ -487,7 +487,7 @@ VLTR_HD PairRec group_clip_words(G& g, S* s, const PairGeom& pg, int32_t a_id, i
     if (j == 0) {
       c.p = pg.ic.c + first;
     } else if (j < 6) {
-      c.p = pg.ic.c + rotate(pg.axis, hex_s[j - 1], hex_c[j - 1], first);
+      c.p = pg.ic.c + rotate_unit(pg.axis, pg.axis_unit, hex_s[j - 1], hex_c[j - 1], first);
     }
     n = 6;
   }

@@ -344,6 +344,7 @@ __device__ __forceinline__ PairGeom table_geom(const L& w, int k, const Sph& A, 
   g.ic = Sph{V3{w.pcx[k], w.pcy[k], w.pcz[k]}, w.ric[k]};
   g.axis = V3{w.pnx[k], w.pny[k], w.pnz[k]};
   g.ic_dir = V3{0.0, 0.0, 0.0};  // used by the classification only
+  g.axis_unit = w.nn.get(k, g.axis);
   return g;
 }
 

@@ -143,28 +143,27 @@ VLTR_HD double directed_angle(V3 o, V3 a, V3 b, V3 c) {
 }
 // geometry.rs:179-193
 VLTR_HD V3 any_normal(V3 a) {
-  V3 b = a;
-  if (!f_eq(b.x, 0.0) && (!f_eq(b.y, 0.0) || !f_eq(b.z, 0.0))) {
-    b.x = -b.x;
+  const bool nx = !f_eq(a.x, 0.0), ny = !f_eq(a.y, 0.0), nz = !f_eq(a.z, 0.0);
+  const bool flip_x = nx && (ny || nz);
+  const bool flip_y = !flip_x && ny && (nx || nz);
+  if (flip_x || flip_y) {
+    // one normalization for both cases: a with its x (or else y) negated
+    const V3 b{flip_x ? -a.x : a.x, flip_x ? a.y : -a.y, a.z};
     return unit(cross(a, b));
-  } else if (!f_eq(b.y, 0.0) && (!f_eq(b.x, 0.0) || !f_eq(b.z, 0.0))) {
-    b.y = -b.y;
-    return unit(cross(a, b));
-  } else if (!f_eq(b.x, 0.0)) {
-    return V3{0.0, 1.0, 0.0};
   }
-  return V3{1.0, 0.0, 0.0};
+  return nx ? V3{0.0, 1.0, 0.0} : V3{1.0, 0.0, 0.0};
 }
 // geometry.rs:196-203 with the unit quaternion (cos t/2, u sin t/2) given by
 // its sine/cosine (the hexagon angles are fixed, so they are precomputed).
-VLTR_HD V3 rotate(V3 axis, double s_half, double c_half, V3 p) {
+// rotate_unit takes u = unit(axis) already computed.
+VLTR_HD V3 rotate_unit(V3 axis, V3 u, double s_half, double c_half, V3 p) {
   if (norm2(axis) <= 0.0) return p;
-  const V3 u = unit(axis);
   const V3 qv = u * s_half;
   const V3 t = cross(qv, p) * 2.0;
   const V3 cr = cross(qv, t);
   return (t * c_half + cr) + p;
 }
+VLTR_HD V3 rotate(V3 axis, double s_half, double c_half, V3 p) { return rotate_unit(axis, unit(axis), s_half, c_half, p); }
 // geometry.rs:243-255
 VLTR_HD bool project_inside_segment(V3 o, V3 a, V3 b, V3* out) {
   const V3 v = unit(b - a);

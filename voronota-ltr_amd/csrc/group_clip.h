@@ -353,11 +353,13 @@ VLTR_HD double group_sum2(G& g, double t1a, double t1b, double t2a, double t2b, 
   return acc;
 }
 
-// Per-vertex turn of the Gauss-Bonnet sum (contact.rs:797-837): first and
-// second addend (second is +0.0 when absent). The three cases differ only in
-// the dihedral's two outer points, so min_dihedral is evaluated once.
+// Per-vertex turn of the Gauss-Bonnet sum (contact.rs:797-837) of the sphere
+// centred at oc against the other sphere's centre qc: first and second addend
+// (second is +0.0 when absent). ratio = norm(icc - oc) / (the sphere's
+// radius), the same for every vertex. The three cases differ only in the
+// dihedral's two outer points, so min_dihedral is evaluated once.
 template <class S>
-VLTR_HD void turn_terms(const Sph& a, const Sph& b, const Sph& ic, const S* s, int i, int m, double* t1, double* t2) {
+VLTR_HD void turn_terms(V3 oc, V3 qc, V3 icc, double ratio, const S* s, int i, int m, double* t1, double* t2) {
   const int pi = i > 0 ? i - 1 : m - 1;
   const int ni = i + 1 < m ? i + 1 : 0;
   const V3 p0 = lds_p(s, pi), p1 = lds_p(s, i), p2 = lds_p(s, ni);
@@ -367,30 +369,38 @@ VLTR_HD void turn_terms(const Sph& a, const Sph& b, const Sph& ic, const S* s, i
   if (a0 > 0.0 || a1 > 0.0) {
     const bool before = a0 > 0.0;
     const double an = before ? a0 : a1;
-    V3 d = cross(b.c - a.c, p1 - ic.c);
+    V3 d = cross(qc - oc, p1 - icc);
     const double t = dot(d, (before ? p0 : p2) - p1);
     if ((an < kPi && t < 0.0) || (an > kPi && t > 0.0)) d = -d;
     if (before) {
       b1 = p1 + d;
     } else {
       b2 = p1 + d;
-      *t2 = a1 * (norm(ic.c - a.c) / a.r);
+      *t2 = a1 * ratio;
     }
   }
-  *t1 = kPi - min_dihedral(a.c, p1, b1, b2);
+  *t1 = kPi - min_dihedral(oc, p1, b1, b2);
+}
+
+// Solid angle of the sphere centred at oc from its turn sum, with its sign
+// (contact.rs:836-847).
+VLTR_HD double signed_solid(double turn, V3 oc, V3 qc, V3 icc) {
+  const double solid = kTau - turn;
+  return dot(icc - oc, icc - qc) > 0.0 && norm2(oc - icc) < norm2(qc - icc) ? -solid : solid;
 }
 
 template <class G, class S>
 VLTR_HD double group_solid_angle(G& g, const Sph& a, const Sph& b, const Sph& ic, const S* s, int m) {
   constexpr int GS = G::kSize;
   const int j = g.lane();
+  const double ratio = norm(ic.c - a.c) / a.r;
   // entries j and j + GS of the lane (registers, not an indexed array)
   double t1a = 0.0, t2a = 0.0, t1b = 0.0, t2b = 0.0;
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
     const int t = j + h * GS;
     double u1 = 0.0, u2 = 0.0;
-    if (t < m) turn_terms(a, b, ic, s, t, m, &u1, &u2);
+    if (t < m) turn_terms(a.c, b.c, ic.c, ratio, s, t, m, &u1, &u2);
     if (h == 0) {
       t1a = u1;
       t2a = u2;
@@ -400,11 +410,7 @@ VLTR_HD double group_solid_angle(G& g, const Sph& a, const Sph& b, const Sph& ic
     }
   }
   const double turn = group_sum2(g, t1a, t1b, t2a, t2b, m);
-  double solid = kTau - turn;
-  const V3 ic_a = ic.c - a.c;
-  const V3 ic_b = ic.c - b.c;
-  if (dot(ic_a, ic_b) > 0.0 && norm2(a.c - ic.c) < norm2(b.c - ic.c)) solid = -solid;
-  return solid;
+  return signed_solid(turn, a.c, b.c, ic.c);
 }
 
 // Area and centrality terms of restricted-contour entry t (contact.rs:773-793,
@@ -471,7 +477,7 @@ VLTR_HD void write_topology(G& g, const S* s, const PairGeom& pg, int32_t a_id, 
 template <class G, class S, class Words, class Fetch>
 VLTR_HD PairRec group_clip_words(G& g, S* s, const PairGeom& pg, int32_t a_id, int nw, Words&& words, Fetch&& f,
                                  const double* hex_s, const double* hex_c, int mode, int* status,
-                                 const TopoOut* tout = nullptr) {
+                                 const TopoOut* tout = nullptr, int* defer_m = nullptr) {
   constexpr int GS = G::kSize;
   const int j = g.lane();
   GPt c{pg.ic.c, a_id, a_id};
@@ -567,7 +573,9 @@ VLTR_HD PairRec group_clip_words(G& g, S* s, const PairGeom& pg, int32_t a_id, i
   rec.area = area;
   rec.arc_length = sum_arc * pg.ic.r;
   rec.flags = kRecValid | (central ? kRecCentral : 0u);
-  if (mode == 0) {
+  if (defer_m != nullptr) {
+    *defer_m = m;  // the caller computes the solid angles from s[0..m)
+  } else if (mode == 0) {
     rec.solid_a = group_solid_angle(g, pg.a, pg.b, pg.ic, s, m);
     rec.solid_b = group_solid_angle(g, pg.b, pg.a, pg.ic, s, m);
     const double sa = rec.solid_a < 0.0 ? -1.0 : 1.0;
@@ -587,9 +595,9 @@ struct MaskPair {
 template <class G, class S, class Fetch>
 VLTR_HD PairRec group_clip(G& g, S* s, const PairGeom& pg, int32_t a_id, uint64_t mlo, uint64_t mhi, Fetch&& f,
                            const double* hex_s, const double* hex_c, int mode, int* status,
-                           const TopoOut* tout = nullptr) {
+                           const TopoOut* tout = nullptr, int* defer_m = nullptr) {
   return group_clip_words(g, s, pg, a_id, mhi != 0ull ? 2 : 1, MaskPair{mlo, mhi}, f, hex_s, hex_c, mode, status,
-                          tout);
+                          tout, defer_m);
 }
 
 }  // namespace vltr

@@ -348,6 +348,94 @@ __device__ __forceinline__ PairGeom table_geom(const L& w, int k, const Sph& A, 
   return g;
 }
 
+// Solid angles and pyramid volumes of a round's contacts (contact.rs:797-847
+// and the record's last two terms), the per-vertex turns of every group and
+// both spheres spread over the whole wavefront: task q is vertex t of group
+// h's contour for sphere a (the group's first c_h tasks) or b (its next c_h),
+// groups in order, so a round needs about one pass however the contour sizes
+// fall among its groups. The contours stay in the groups' shared buffers.
+// Lanes 0 and 1 of a group then add the turns of sphere a and b in vertex
+// order, as the one-lane code does. c = the contour size of this lane's group
+// when its pair has a contact, else 0; B and icc = its pair's second sphere
+// and circle centre; A = the wavefront's ball.
+template <int GS, class GL>
+__device__ __forceinline__ void wave_solid(const GL* grp, const int lane, const Sph& A, const Sph& B, const V3 icc,
+                                           const int c, PairRec* rec) {
+  constexpr int NG = kWave / GS;
+  int pref[NG + 1], cg[NG];
+  pref[0] = 0;
+  int cmax = 0;
+#pragma unroll
+  for (int h = 0; h < NG; ++h) {
+    cg[h] = __builtin_amdgcn_readlane(c, h * GS);
+    pref[h + 1] = pref[h] + 2 * cg[h];
+    cmax = cg[h] > cmax ? cg[h] : cmax;
+  }
+  const int T = pref[NG];
+  if (T == 0) return;
+  wave_sync();  // the groups' contours are in shared memory
+  const int j = lane & (GS - 1);
+  const int hg = lane / GS;
+  // lane j of a group offers sphere b's values for odd j, sphere a's for even
+  const bool jb = (j & 1) != 0;
+  const V3 oc = jb ? B.c : A.c;
+  const V3 qc = jb ? A.c : B.c;
+  const double nrm = norm(icc - oc);
+  const double ratio = nrm / (jb ? B.r : A.r);
+  int p_own = 0;
+#pragma unroll
+  for (int h = 0; h < NG; ++h) p_own = hg == h ? pref[h] : p_own;
+  // lanes 0 and 1 of the group sum the tasks [lo, hi) of sphere a and b
+  const int my_lo = p_own + (jb ? c : 0);
+  const int my_hi = j < 2 ? my_lo + c : my_lo;
+  double acc = 0.0;
+  for (int base = 0; base < T; base += kWave) {
+    const int q = base + lane;
+    // the group whose tasks hold q: the last one starting at or before q (an
+    // empty group starts where the next one does)
+    int h = 0;
+#pragma unroll
+    for (int k = 1; k < NG; ++k) h = q >= pref[k] ? k : h;
+    int ph = 0, ch = 0;
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+      ph = h == k ? pref[k] : ph;
+      ch = h == k ? cg[k] : ch;
+    }
+    const int rel = q - ph;
+    const bool sb = rel >= ch;
+    const int t = sb ? rel - ch : rel;
+    const int src = q < T ? h * GS + (sb ? 1 : 0) : lane;
+    const V3 bc{__shfl(B.c.x, src, kWave), __shfl(B.c.y, src, kWave), __shfl(B.c.z, src, kWave)};
+    const V3 ic{__shfl(icc.x, src, kWave), __shfl(icc.y, src, kWave), __shfl(icc.z, src, kWave)};
+    const double r_src = __shfl(ratio, src, kWave);
+    double t1 = 0.0, t2 = 0.0;
+    if (q < T) turn_terms(sb ? bc : A.c, sb ? A.c : bc, ic, r_src, grp + h, t, ch, &t1, &t2);
+    const int lo = my_lo > base ? my_lo : base;
+    const int hi = my_hi < base + kWave ? my_hi : base + kWave;
+    for (int i = 0; i < cmax; ++i) {
+      const int sq = lo + i;
+      const int sl = (sq - base) & (kWave - 1);
+      const double v1 = __shfl(t1, sl, kWave), v2 = __shfl(t2, sl, kWave);
+      if (sq < hi) {
+        acc = acc + v1;
+        acc = acc + v2;
+      }
+    }
+  }
+  const double solid = signed_solid(acc, oc, qc, icc);
+  const double pyr = nrm * rec->area / 3.0 * (solid < 0.0 ? -1.0 : 1.0);
+  const int src_b = (lane & ~(GS - 1)) | 1;
+  const double solid_b = __shfl(solid, src_b, kWave), pyr_b = __shfl(pyr, src_b, kWave);
+  if (c > 0 && j == 0) {
+    rec->solid_a = solid;
+    rec->pyr_a = pyr;
+    rec->solid_b = solid_b;
+    rec->pyr_b = pyr_b;
+  }
+  wave_sync();  // read before the next round's groups write their contours
+}
+
 // One clip task of a wavefront: ball q (in cell order), share wsub of its
 // qualifying pairs when kWavesPerBall waves split the ball (pair jj of the
 // ball goes to wave jj % kWavesPerBall; each of them builds the plane table,
@@ -560,26 +648,35 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
   } fetch{w};
   for (int r0 = 0; r0 < n_clip; r0 += kWave / kClipGroup) {
     const int idx = r0 + grp;
-    if (idx < n_clip) {
+    const bool act = idx < n_clip;
+    PairRec rec = empty_record();
+    int st = kNone, m = 0;
+    int32_t b = 0;
+    int64_t pidx = 0;
+    Sph B{A.c, A.r};
+    V3 icc = A.c;
+    if (act) {
       const int it = w.item[idx];
       const uint64_t M = w.mask[idx];
-      const int32_t b = w.cid[it & 0xff];
+      b = w.cid[it & 0xff];
       const PairGeom g = table_geom(w, it & 0xff, A, load_sph(sph4, b));
-      int st = kNone;
-      const int64_t pidx = base + (it >> 8);
+      pidx = base + (it >> 8);
       const TopoOut tout{p.topo, p.topo != nullptr ? p.topo_slot + pidx : nullptr,
                          p.topo != nullptr ? topo_record(p, tbase + static_cast<uint32_t>(idx)) : kTopoSlotFull, sph4,
                          p.net != nullptr ? p.net + pidx : nullptr};
       const bool want = kOut && (p.topo != nullptr || p.net != nullptr);
-      const PairRec rec = group_clip(G, gs, g, static_cast<int32_t>(a), M, 0ull, fetch, c_hex_s, c_hex_c, p.mode, &st,
-                                     want ? &tout : nullptr);
-      if (G.lane() == 0) {
-        if (st == kOverflow) {
-          push_wide(p, pidx);
-        } else {
-          put_record(p, pidx, rec);
-          push_incoming(p, pidx, b, rec);
-        }
+      rec = group_clip(G, gs, g, static_cast<int32_t>(a), M, 0ull, fetch, c_hex_s, c_hex_c, p.mode, &st,
+                       want ? &tout : nullptr, &m);
+      B = g.b;
+      icc = g.ic.c;
+    }
+    if (p.mode == 0) wave_solid<kClipGroup>(w.grp, lane, A, B, icc, st == kContact ? m : 0, &rec);
+    if (act && G.lane() == 0) {
+      if (st == kOverflow) {
+        push_wide(p, pidx);
+      } else {
+        put_record(p, pidx, rec);
+        push_incoming(p, pidx, b, rec);
       }
     }
   }

@@ -651,15 +651,15 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
     const bool act = idx < n_clip;
     PairRec rec = empty_record();
     int st = kNone, m = 0;
-    int32_t b = 0;
+    int32_t b = static_cast<int32_t>(a);
     int64_t pidx = 0;
-    Sph B{A.c, A.r};
-    V3 icc = A.c;
+    int slot = 0;
     if (act) {
       const int it = w.item[idx];
       const uint64_t M = w.mask[idx];
-      b = w.cid[it & 0xff];
-      const PairGeom g = table_geom(w, it & 0xff, A, load_sph(sph4, b));
+      slot = it & 0xff;
+      b = w.cid[slot];
+      const PairGeom g = table_geom(w, slot, A, load_sph(sph4, b));
       pidx = base + (it >> 8);
       const TopoOut tout{p.topo, p.topo != nullptr ? p.topo_slot + pidx : nullptr,
                          p.topo != nullptr ? topo_record(p, tbase + static_cast<uint32_t>(idx)) : kTopoSlotFull, sph4,
@@ -667,10 +667,12 @@ __device__ __forceinline__ void clip_ball(const Pipeline& p, ClipLdsT<kClipGroup
       const bool want = kOut && (p.topo != nullptr || p.net != nullptr);
       rec = group_clip(G, gs, g, static_cast<int32_t>(a), M, 0ull, fetch, c_hex_s, c_hex_c, p.mode, &st,
                        want ? &tout : nullptr, &m);
-      B = g.b;
-      icc = g.ic.c;
     }
-    if (p.mode == 0) wave_solid<kClipGroup>(w.grp, lane, A, B, icc, st == kContact ? m : 0, &rec);
+    // the pair's second sphere and circle centre read again rather than
+    // kept in registers through the clip
+    if (p.mode == 0)
+      wave_solid<kClipGroup>(w.grp, lane, A, load_sph(sph4, b), V3{w.pcx[slot], w.pcy[slot], w.pcz[slot]},
+                             st == kContact ? m : 0, &rec);
     if (act && G.lane() == 0) {
       if (st == kOverflow) {
         push_wide(p, pidx);

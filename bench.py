@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # KiB per dispatch, factors from a 256 MiB device copy read with the same
 # counters (profiles/k_clip_100k_counters.txt). Other shapes and workloads:
 # not measured.
-K_CLIP_TRAFFIC_BYTES_100K = {"g8w1o4nq": 179.8e6, "g8w1o4q": 183.6e6}
+K_CLIP_TRAFFIC_BYTES_100K = {"g8w1o4nq": 180.6e6, "g8w1o4q": 192.5e6}
 
 
 def work_counts(n, seed, probe):
